@@ -1,0 +1,202 @@
+// oracle/ref_harness.cpp — a C-ABI shim over the REAL reference CpuBackend.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with the
+// reference's own CPU sources where they lie under /root/reference (no copy of them
+// is kept in this repository) into oracle/_ref/libsteppe_ref.so.  Tests use it to
+// pin the restated oracle (cpu_ref.cpp) bit-for-bit, and bench.py may time it as
+// the "reference" CPU baseline.  The product library never links it.
+//
+// Entry points (each calls the reference function named beside it):
+//   ref_decode_af        ComputeBackend::decode_af          src/device/cpu/cpu_backend.cpp:267
+//   ref_detect_ploidy    detect_sample_ploidy_device        src/device/cpu/cpu_backend.cpp:1200
+//   ref_assign_blocks    core::assign_blocks                src/core/domain/block_partition_rule.cpp:56
+//   ref_f2_blocks        ComputeBackend::compute_f2_blocks  src/device/cpu/cpu_backend.cpp:217
+//   ref_qpadm            steppe::run_qpadm (host tensor)    src/core/qpadm/qpadm_fit.cpp:216
+//   ref_qpgraph          steppe::run_qpgraph (host tensor)  src/core/qpadm/qpgraph_fit.cpp:127
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <span>
+#include <string>
+#include <vector>
+
+#include "core/domain/block_partition_rule.hpp"
+#include "core/internal/views.hpp"
+#include "device/backend.hpp"
+#include "device/backend_factory.hpp"
+#include "device/resources.hpp"
+#include "steppe/fstats.hpp"
+#include "steppe/qpadm.hpp"
+#include "steppe/qpgraph.hpp"
+
+#include "stp_qpadm_result.h"
+
+namespace {
+
+steppe::device::Resources cpu_resources() {
+    steppe::device::Resources res;
+    steppe::device::PerGpuResources cpu;
+    cpu.device_id = -1;
+    cpu.backend = steppe::device::make_cpu_backend();
+    res.gpus.push_back(std::move(cpu));
+    return res;
+}
+
+steppe::DecodeTileView tile_view(const std::uint8_t* packed, std::size_t bpr, std::size_t n_snp,
+                                 std::size_t n_ind, const std::size_t* offs, int n_pop,
+                                 const int* sample_ploidy, int ploidy, int detect) {
+    steppe::DecodeTileView t;
+    t.packed = packed;
+    t.bytes_per_record = bpr;
+    t.n_snp = n_snp;
+    t.n_individuals = n_ind;
+    t.pop_offsets = offs;
+    t.n_pop = n_pop;
+    t.sample_ploidy = sample_ploidy;
+    t.ploidy = ploidy;
+    t.detect_ploidy_on_device = detect != 0;
+    return t;
+}
+
+template <class T>
+void put(T* dst, const std::vector<T>& v, std::size_t cap) {
+    for (std::size_t k = 0; k < v.size() && k < cap; ++k) dst[k] = v[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_decode_af(const std::uint8_t* packed, std::size_t bpr, std::size_t n_snp,
+                  std::size_t n_ind, const std::uint64_t* pop_offsets, int n_pop,
+                  const std::int32_t* sample_ploidy, int ploidy, int detect, double* q,
+                  double* v, double* n) {
+    try {
+        std::vector<std::size_t> offs(pop_offsets, pop_offsets + n_pop + 1);
+        auto be = steppe::device::make_cpu_backend();
+        const auto r = be->decode_af(tile_view(packed, bpr, n_snp, n_ind, offs.data(), n_pop,
+                                               sample_ploidy, ploidy, detect));
+        std::memcpy(q, r.q.data(), r.q.size() * sizeof(double));
+        std::memcpy(v, r.v.data(), r.v.size() * sizeof(double));
+        std::memcpy(n, r.n.data(), r.n.size() * sizeof(double));
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_detect_ploidy(const std::uint8_t* packed, std::size_t bpr, std::size_t n_snp,
+                      std::size_t n_ind, std::int32_t* out) {
+    try {
+        auto be = steppe::device::make_cpu_backend();
+        const auto p = be->detect_sample_ploidy_device(
+            tile_view(packed, bpr, n_snp, n_ind, nullptr, 0, nullptr, 2, 1));
+        for (std::size_t g = 0; g < p.size(); ++g) out[g] = p[g];
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_assign_blocks(const std::int32_t* chrom, const double* genpos, std::int64_t M,
+                      double blgsize, const double* physpos, double bp_window,
+                      std::int32_t* out_block_id) {
+    try {
+        std::vector<int> c(chrom, chrom + M);
+        std::span<const double> phys;
+        if (physpos) phys = std::span<const double>(physpos, static_cast<std::size_t>(M));
+        const auto bp = steppe::core::assign_blocks(
+            c, std::span<const double>(genpos, static_cast<std::size_t>(M)), blgsize, phys,
+            bp_window);
+        for (std::size_t s = 0; s < bp.block_id.size(); ++s) out_block_id[s] = bp.block_id[s];
+        return bp.n_block;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_f2_blocks(const double* Q, const double* V, const double* N, int P, std::int64_t M,
+                  const std::int32_t* block_id, int nb, double* f2, double* vpair,
+                  std::int32_t* block_sizes) {
+    try {
+        auto be = steppe::device::make_cpu_backend();
+        const steppe::core::MatView q{Q, P, static_cast<long>(M)};
+        const steppe::core::MatView v{V, P, static_cast<long>(M)};
+        const steppe::core::MatView n{N, P, static_cast<long>(M)};
+        std::vector<int> bid(block_id, block_id + M);
+        const auto t = be->compute_f2_blocks(q, v, n, bid.data(), nb, steppe::Precision{});
+        std::memcpy(f2, t.f2.data(), t.f2.size() * sizeof(double));
+        std::memcpy(vpair, t.vpair.data(), t.vpair.size() * sizeof(double));
+        for (int b = 0; b < nb; ++b) block_sizes[b] = t.block_sizes[static_cast<std::size_t>(b)];
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// One qpAdm model through the reference's public driver on a host f2 tensor.
+// left excludes the target (left_idx = [target] + left inside the driver).
+int ref_qpadm(const double* f2, const double* vpair, const std::int32_t* block_sizes, int P,
+              int nb, int target, const std::int32_t* left, int n_left,
+              const std::int32_t* right, int n_right, double fudge, int als_iterations,
+              int rank, double rank_alpha, int jackknife, stp_qpadm_result* out) {
+    try {
+        std::memset(out, 0, sizeof(*out));
+        steppe::F2BlockTensor t;
+        t.P = P;
+        t.n_block = nb;
+        const std::size_t tot = static_cast<std::size_t>(P) * P * nb;
+        t.f2.assign(f2, f2 + tot);
+        if (vpair) t.vpair.assign(vpair, vpair + tot);
+        t.block_sizes.assign(block_sizes, block_sizes + nb);
+        steppe::QpAdmModel m;
+        m.target = target;
+        m.left.assign(left, left + n_left);
+        m.right.assign(right, right + n_right);
+        m.model_index = 0;
+        steppe::QpAdmOptions o;
+        o.fudge = fudge;
+        o.als_iterations = als_iterations;
+        o.rank = rank;
+        o.rank_alpha = rank_alpha;
+        o.jackknife = static_cast<steppe::JackknifePolicy>(jackknife);
+        auto res = cpu_resources();
+        const steppe::QpAdmResult r = steppe::run_qpadm(t, m, o, res);
+        out->status = static_cast<int>(r.status);
+        out->nl = static_cast<int>(r.weight.size());
+        out->dof = r.dof;
+        out->est_rank = r.est_rank;
+        out->f4rank = r.f4rank;
+        out->p = r.p;
+        out->chisq = r.chisq;
+        put(out->weight, r.weight, STP_QPADM_MAX_NL);
+        put(out->se, r.se, STP_QPADM_MAX_NL);
+        put(out->z, r.z, STP_QPADM_MAX_NL);
+        out->n_rank = static_cast<int>(r.rank_chisq.size());
+        put(out->rank_chisq, r.rank_chisq, STP_QPADM_MAX_RANK);
+        put(out->rank_dof, r.rank_dof, STP_QPADM_MAX_RANK);
+        put(out->rankdrop_f4rank, r.rankdrop_f4rank, STP_QPADM_MAX_RANK);
+        put(out->rankdrop_dof, r.rankdrop_dof, STP_QPADM_MAX_RANK);
+        put(out->rankdrop_dofdiff, r.rankdrop_dofdiff, STP_QPADM_MAX_RANK);
+        put(out->rankdrop_chisq, r.rankdrop_chisq, STP_QPADM_MAX_RANK);
+        put(out->rankdrop_p, r.rankdrop_p, STP_QPADM_MAX_RANK);
+        put(out->rankdrop_chisqdiff, r.rankdrop_chisqdiff, STP_QPADM_MAX_RANK);
+        put(out->rankdrop_p_nested, r.rankdrop_p_nested, STP_QPADM_MAX_RANK);
+        out->n_popdrop = static_cast<int>(r.popdrop_pat.size());
+        for (std::size_t k = 0; k < r.popdrop_pat.size() && k < STP_QPADM_MAX_POPDROP; ++k) {
+            std::strncpy(out->popdrop_pat[k], r.popdrop_pat[k].c_str(), STP_QPADM_MAX_NL);
+            out->popdrop_wt[k] = r.popdrop_wt[k];
+            out->popdrop_dof[k] = r.popdrop_dof[k];
+            out->popdrop_f4rank[k] = r.popdrop_f4rank[k];
+            out->popdrop_chisq[k] = r.popdrop_chisq[k];
+            out->popdrop_p[k] = r.popdrop_p[k];
+            out->popdrop_feasible[k] = r.popdrop_feasible[k] != 0;
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+"""Synthetic genotype panels for the f-statistics workloads (host, numpy).
+
+Generates packed 2-bit genotype tiles in the layout the decode path consumes
+(individual-major records, four codes per byte, first SNP in the two most
+significant bits, code = reference-allele count, 3 = missing; the reference's
+DecodeTileView, src/device/backend.hpp:214-235) together with the population
+offsets, per-sample ploidy and the jackknife block partition.
+
+Distributions (seeded, numpy PCG64):
+  * ancestral frequency p0 ~ Uniform(0.05, 0.95) per SNP;
+  * population frequency by Balding-Nichols, p ~ Beta(p0 (1-F)/F, (1-p0)(1-F)/F);
+  * diploid genotype ~ Binomial(2, p); pseudo-haploid genotype = 2 * Bernoulli(p)
+    (a single allele per site, coded 0/2, so ploidy detection returns 1);
+  * each genotype missing (code 3) independently at the population's rate.
+
+This module builds the small panels the CPU tests need; the large panels
+(hundreds of populations, 1.2M SNPs) are generated on the GPU by the library's
+synthetic-panel kernel with the same distributions.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SEED = 20261202
+
+
+@dataclass
+class Panel:
+    packed: np.ndarray            # uint8 [n_ind, bytes_per_record]
+    n_snp: int
+    pop_offsets: np.ndarray       # uint64 [P+1]
+    sample_ploidy: np.ndarray     # int32 [n_ind] (the generating ploidy)
+    block_id: np.ndarray          # int32 [M], contiguous non-decreasing
+    n_block: int
+    pop_names: list = field(default_factory=list)
+    chrom: np.ndarray | None = None
+    genpos: np.ndarray | None = None
+
+    @property
+    def n_ind(self) -> int:
+        return int(self.packed.shape[0])
+
+    @property
+    def n_pop(self) -> int:
+        return int(self.pop_offsets.shape[0] - 1)
+
+    @property
+    def bytes_per_record(self) -> int:
+        return int(self.packed.shape[1])
+
+
+def pack_codes(codes: np.ndarray, bytes_per_record: int | None = None) -> np.ndarray:
+    """Pack int codes [n_ind, M] (0..3) into MSB-first 2-bit records."""
+    n_ind, M = codes.shape
+    nb = (M + 3) // 4
+    bpr = nb if bytes_per_record is None else int(bytes_per_record)
+    if bpr < nb:
+        raise ValueError("bytes_per_record too small")
+    padded = np.zeros((n_ind, nb * 4), dtype=np.uint8)
+    padded[:, :M] = codes
+    q = padded.reshape(n_ind, nb, 4)
+    by = (q[:, :, 0] << 6) | (q[:, :, 1] << 4) | (q[:, :, 2] << 2) | q[:, :, 3]
+    out = np.zeros((n_ind, bpr), dtype=np.uint8)
+    out[:, :nb] = by
+    return out
+
+
+def unpack_codes(packed: np.ndarray, M: int) -> np.ndarray:
+    shifts = np.array([6, 4, 2, 0], dtype=np.uint8)
+    c = (packed[:, :, None] >> shifts[None, None, :]) & 3
+    return c.reshape(packed.shape[0], -1)[:, :M]
+
+
+def genetic_map(M: int, n_chrom: int, total_morgans: float, rng) -> tuple:
+    """Uniform SNP positions on n_chrom chromosomes at 1 cM/Mb; returns (chrom,
+    genpos in Morgans, physpos in bp), sorted within chromosome."""
+    w = np.arange(n_chrom, 0, -1, dtype=np.float64) + 8.0
+    lengths = total_morgans * w / w.sum()
+    counts = np.floor(M * lengths / lengths.sum()).astype(np.int64)
+    counts[: M - counts.sum()] += 1
+    chrom, gen = [], []
+    for c in range(n_chrom):
+        gen.append(np.sort(rng.uniform(0.0, lengths[c], counts[c])))
+        chrom.append(np.full(counts[c], c + 1, dtype=np.int32))
+    genpos = np.concatenate(gen)
+    return np.concatenate(chrom), genpos, genpos * 1e8
+
+
+def assign_blocks(chrom, genpos, blgsize: float) -> tuple:
+    """Jackknife block walk (src/core/domain/block_partition_rule.cpp:14-43)."""
+    M = len(chrom)
+    out = np.empty(M, dtype=np.int32)
+    blk, cur, start = -1, None, -1.0e20
+    for s in range(M):
+        if chrom[s] != cur or genpos[s] - start >= blgsize:
+            blk += 1
+            cur = chrom[s]
+            start = genpos[s]
+        out[s] = blk
+    return out, blk + 1
+
+
+def blocks_from_sizes(sizes) -> np.ndarray:
+    sizes = np.asarray(sizes, dtype=np.int64)
+    return np.repeat(np.arange(len(sizes), dtype=np.int32), sizes)
+
+
+def uniform_block_sizes(M: int, lo: int, hi: int, rng) -> np.ndarray:
+    sizes = []
+    tot = 0
+    while tot < M:
+        s = int(rng.integers(lo, hi + 1))
+        s = min(s, M - tot)
+        sizes.append(s)
+        tot += s
+    if len(sizes) > 1 and sizes[-1] < lo:
+        sizes[-2] += sizes[-1]
+        sizes.pop()
+    return np.asarray(sizes, dtype=np.int64)
+
+
+def lognormal_block_sizes(M: int, median: float, sigma: float, lo: int, hi: int, rng):
+    sizes = []
+    tot = 0
+    while tot < M:
+        s = int(np.clip(np.rint(rng.lognormal(np.log(median), sigma)), lo, hi))
+        s = min(s, M - tot)
+        sizes.append(s)
+        tot += s
+    if len(sizes) > 1 and sizes[-1] < lo:
+        sizes[-2] += sizes[-1]
+        sizes.pop()
+    return np.asarray(sizes, dtype=np.int64)
+
+
+def make_panel(pop_sizes, M: int, fst, missing, haploid, block_id, n_block, rng,
+               bytes_per_record: int | None = None, chrom=None, genpos=None) -> Panel:
+    pop_sizes = np.asarray(pop_sizes, dtype=np.int64)
+    P = len(pop_sizes)
+    fst = np.broadcast_to(np.asarray(fst, dtype=np.float64), (P,))
+    missing = np.broadcast_to(np.asarray(missing, dtype=np.float64), (P,))
+    haploid = np.broadcast_to(np.asarray(haploid, dtype=bool), (P,))
+    p0 = rng.uniform(0.05, 0.95, M)
+    codes = np.empty((int(pop_sizes.sum()), M), dtype=np.uint8)
+    ploidy = np.empty(int(pop_sizes.sum()), dtype=np.int32)
+    offs = np.zeros(P + 1, dtype=np.uint64)
+    g = 0
+    for i in range(P):
+        F = fst[i]
+        a = p0 * (1.0 - F) / F
+        b = (1.0 - p0) * (1.0 - F) / F
+        p = rng.beta(a, b)
+        n = int(pop_sizes[i])
+        if haploid[i]:
+            c = 2 * (rng.random((n, M)) < p[None, :]).astype(np.uint8)
+            ploidy[g:g + n] = 1
+        else:
+            c = rng.binomial(2, np.broadcast_to(p, (n, M))).astype(np.uint8)
+            ploidy[g:g + n] = 2
+        miss = rng.random((n, M)) < missing[i]
+        c[miss] = 3
+        codes[g:g + n] = c
+        g += n
+        offs[i + 1] = g
+    packed = pack_codes(codes, bytes_per_record)
+    return Panel(packed=packed, n_snp=M, pop_offsets=offs, sample_ploidy=ploidy,
+                 block_id=np.asarray(block_id, dtype=np.int32), n_block=int(n_block),
+                 pop_names=[f"pop{i:04d}" for i in range(P)], chrom=chrom, genpos=genpos)
+
+
+def config1(seed: int = SEED, M: int = 50_000, P: int = 8, n_per_pop: int = 20) -> Panel:
+    """8 populations x 20 diploid individuals, 50,000 SNPs on 22 chromosomes,
+    F_ST 0.05, 5% missing, 5 cM blocks at 1 cM/Mb (about 100 blocks)."""
+    rng = np.random.default_rng(seed)
+    chrom, genpos, _ = genetic_map(M, 22, 5.0, rng)
+    block_id, nb = assign_blocks(chrom, genpos, 0.05)
+    return make_panel([n_per_pop] * P, M, 0.05, 0.05, False, block_id, nb, rng,
+                      chrom=chrom, genpos=genpos)
+
+
+def scaled_config(P: int, M: int, seed: int = SEED, block_lo: int = 200,
+                  block_hi: int = 3000, lognormal: bool = False,
+                  bytes_per_record: int | None = None) -> Panel:
+    """The large-config recipe at a reduced size: population sizes U{1..60}, 40%
+    pseudo-haploid populations with missingness ~ Beta(2,2) (diploid ones 5%),
+    F_ST ~ U(0.02, 0.15) per population; blocks U{block_lo..block_hi} SNPs, or
+    LogNormal(ln 1500, 0.6) clipped to [50, 8000] when lognormal is set."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(1, 61, P)
+    haploid = rng.random(P) < 0.4
+    missing = np.where(haploid, rng.beta(2.0, 2.0, P), 0.05)
+    fst = rng.uniform(0.02, 0.15, P)
+    if lognormal:
+        bs = lognormal_block_sizes(M, 1500.0, 0.6, 50, 8000, rng)
+    else:
+        bs = uniform_block_sizes(M, block_lo, block_hi, rng)
+    return make_panel(sizes, M, fst, missing, haploid, blocks_from_sizes(bs), len(bs), rng,
+                      bytes_per_record=bytes_per_record)

@@ -1,0 +1,143 @@
+"""ctypes front end to the CPU parity checkers under oracle/ (test-side only).
+
+``cref`` is the restated oracle (oracle/libcpu_ref.so); ``ref`` is the real
+reference CpuBackend compiled from its own sources (oracle/_ref/libsteppe_ref.so),
+present wherever oracle/Makefile's ``ref`` target could run.  Both expose the same
+argument conventions, so tests call them through the same Python wrappers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle")
+CREF_SO = os.path.join(ORACLE, "libcpu_ref.so")
+REF_SO = os.path.join(ORACLE, "_ref", "libsteppe_ref.so")
+
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+
+
+def _ptr_or_null(a, dtype):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=dtype).ctypes.data_as(C.c_void_p)
+
+
+def ensure_built():
+    if not os.path.exists(CREF_SO):
+        subprocess.run(["make", "-s", "-C", ORACLE, "all"], check=True)
+    if not os.path.exists(REF_SO) and os.path.isdir("/root/reference"):
+        subprocess.run(["make", "-s", "-C", ORACLE, "ref"], check=True)
+
+
+class _Lib:
+    def __init__(self, path: str, prefix: str):
+        self.lib = C.CDLL(path)
+        self.prefix = prefix
+        self.path = path
+
+    def fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    # ---- decode -------------------------------------------------------------
+    def decode_af(self, panel, ploidy_mode="sample"):
+        P, M = panel.n_pop, panel.n_snp
+        q = np.zeros(P * M, np.float64)
+        v = np.zeros_like(q)
+        n = np.zeros_like(q)
+        sp, pl, det = self._ploidy_args(panel, ploidy_mode)
+        f = self.fn("decode_af")
+        f.restype = C.c_int
+        rc = f(panel.packed.ctypes.data_as(C.c_void_p), C.c_size_t(panel.bytes_per_record),
+               C.c_size_t(M), C.c_size_t(panel.n_ind),
+               panel.pop_offsets.ctypes.data_as(C.c_void_p), C.c_int(P), sp, C.c_int(pl),
+               C.c_int(det), q.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p),
+               n.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        return (q.reshape(M, P).T, v.reshape(M, P).T, n.reshape(M, P).T)
+
+    def detect_ploidy(self, panel):
+        out = np.zeros(panel.n_ind, np.int32)
+        f = self.fn("detect_ploidy")
+        rc = f(panel.packed.ctypes.data_as(C.c_void_p), C.c_size_t(panel.bytes_per_record),
+               C.c_size_t(panel.n_snp), C.c_size_t(panel.n_ind), out.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        return out
+
+    @staticmethod
+    def _ploidy_args(panel, mode):
+        if mode == "sample":
+            sp = np.ascontiguousarray(panel.sample_ploidy, np.int32)
+            panel._sp_keep = sp
+            return sp.ctypes.data_as(C.c_void_p), 2, 0
+        if mode == "detect":
+            return None, 2, 1
+        return None, int(mode), 0
+
+    # ---- blocks -------------------------------------------------------------
+    def assign_blocks(self, chrom, genpos, blgsize, physpos=None, bp_window=2.0e6):
+        chrom = np.ascontiguousarray(chrom, np.int32)
+        genpos = np.ascontiguousarray(genpos, np.float64)
+        M = len(chrom)
+        out = np.zeros(M, np.int32)
+        f = self.fn("assign_blocks")
+        f.restype = C.c_int
+        ph = None if physpos is None else np.ascontiguousarray(physpos, np.float64)
+        nb = f(chrom.ctypes.data_as(C.c_void_p), genpos.ctypes.data_as(C.c_void_p),
+               C.c_int64(M), C.c_double(blgsize),
+               None if ph is None else ph.ctypes.data_as(C.c_void_p), C.c_double(bp_window),
+               out.ctypes.data_as(C.c_void_p))
+        return out, nb
+
+    # ---- f2 -----------------------------------------------------------------
+    def f2_blocks(self, Q, V, N, block_id, nb, n_threads=1):
+        """Q/V/N as [P, M] arrays; returns f2, vpair as (P, P, nb) F-order views and
+        block_sizes."""
+        P, M = Q.shape
+        q = np.ascontiguousarray(Q.T, np.float64)   # memory i + P*s
+        v = np.ascontiguousarray(V.T, np.float64)
+        n = np.ascontiguousarray(N.T, np.float64)
+        bid = np.ascontiguousarray(block_id, np.int32)
+        f2 = np.zeros(P * P * nb, np.float64)
+        vp = np.zeros_like(f2)
+        bs = np.zeros(nb, np.int32)
+        args = [q.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p),
+                n.ctypes.data_as(C.c_void_p), C.c_int(P), C.c_int64(M),
+                bid.ctypes.data_as(C.c_void_p), C.c_int(nb), f2.ctypes.data_as(C.c_void_p),
+                vp.ctypes.data_as(C.c_void_p), bs.ctypes.data_as(C.c_void_p)]
+        f = self.fn("f2_blocks")
+        f.restype = C.c_int
+        if self.prefix == "cref_":
+            args.append(C.c_int(n_threads))
+        rc = f(*args)
+        assert rc == 0
+        return (f2.reshape(nb, P, P).transpose(2, 1, 0), vp.reshape(nb, P, P).transpose(2, 1, 0),
+                bs)
+
+
+_cache = {}
+
+
+def cref():
+    ensure_built()
+    if "cref" not in _cache:
+        _cache["cref"] = _Lib(CREF_SO, "cref_")
+    return _cache["cref"]
+
+
+def ref():
+    """The real reference library, or None when it could not be built here."""
+    ensure_built()
+    if not os.path.exists(REF_SO):
+        return None
+    if "ref" not in _cache:
+        _cache["ref"] = _Lib(REF_SO, "ref_")
+    return _cache["ref"]
