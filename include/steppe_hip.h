@@ -1,0 +1,165 @@
+/* steppe_hip.h — the C ABI of the MI355X f-statistics engine (libsteppe_hip.so).
+ *
+ * This is the drop-in boundary for the reference's f-statistics hot path: genotype
+ * decode -> per-population counts -> per-jackknife-block f2 tensor held on the GPU
+ * (an "f2 handle") -> fits.  Each entry point names the reference interface it
+ * replaces (paths relative to the reference repository root).  Plain pointers and
+ * sizes only; "d_" pointers are device (HBM) pointers, all others host pointers.
+ *
+ * Conventions
+ *   - Every function returns STP_OK (0) or a negative status; stp_last_error()
+ *     returns the calling thread's last error message (like the reference's
+ *     exceptions, src/device/cuda/check.cuh, surfaced as ValueError in Python).
+ *   - Calls are synchronous: a function returns after its device work finished,
+ *     as the reference's CudaBackend syncs its stream before returning
+ *     (src/device/cuda/cuda_backend.cuh:576-602).  A context is used by one
+ *     thread at a time.
+ *   - f2 tensors are [P x P x n_block] float64 at i + P*j + P*P*b
+ *     (include/steppe/fstats.hpp:19-60); Q/V/N matrices are [P x M] at i + P*s.
+ */
+#ifndef STEPPE_HIP_H
+#define STEPPE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "stp_qpadm_result.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define STP_OK 0
+#define STP_ERR_INVALID (-1)  /* invalid argument (reference: std::invalid_argument)  */
+#define STP_ERR_DEVICE (-2)   /* HIP runtime / kernel failure (reference: CudaError)   */
+#define STP_ERR_OOM (-3)      /* device allocation failed (reference: Status::DeviceOom) */
+#define STP_ERR_IO (-4)       /* file could not be opened / read / written             */
+#define STP_ERR_FORMAT (-5)   /* not a valid STPF2BK1 file                              */
+#define STP_ERR_INTERNAL (-9)
+
+/* f2 arithmetic forms (reference: steppe::Precision, include/steppe/config.hpp:74-110). */
+#define STP_PREC_FP64 0        /* FP64 MFMA contraction                                 */
+#define STP_PREC_SPLIT_BF16 1  /* fixed-point BF16 slices on BF16 MFMA, FP64 recombination */
+
+typedef struct stp_context stp_context;
+typedef struct stp_f2_handle stp_f2_handle;
+
+/* Packed genotype tile (reference: DecodeTileView, src/device/backend.hpp:214-235).
+ * Individual-major records of bytes_per_record bytes, 4 codes per byte, first SNP in
+ * the two most significant bits, code = reference-allele count, 3 = missing.
+ * Individuals of population p are [pop_offsets[p], pop_offsets[p+1]) (host array).
+ * Ploidy: sample_ploidy (host, n_individuals entries) if non-NULL, else on-device
+ * detection if detect_ploidy_on_device, else the scalar ploidy for every sample.
+ * packed is a device pointer when packed_on_device is non-zero, else a host pointer. */
+typedef struct stp_tile {
+    const uint8_t* packed;
+    size_t bytes_per_record;
+    size_t n_snp;
+    size_t n_individuals;
+    const uint64_t* pop_offsets;
+    int32_t n_pop;
+    const int32_t* sample_ploidy;
+    int32_t ploidy;
+    int32_t detect_ploidy_on_device;
+    int32_t packed_on_device;
+} stp_tile;
+
+/* ---- context ------------------------------------------------------------------ */
+
+/* Open the engine on one HIP device (reference: build_resources + make_cuda_backend,
+ * src/device/resources.hpp:70, src/device/backend_factory.hpp:18-24). */
+int stp_open(int device, stp_context** out);
+int stp_close(stp_context* ctx);
+const char* stp_last_error(void);
+/* Name and compute-unit count of the context's device (reference: BackendCapabilities,
+ * src/device/backend.hpp:1649). */
+int stp_device_info(stp_context* ctx, char* name, int name_len, int32_t* n_cu);
+
+/* ---- decode ------------------------------------------------------------------- */
+
+/* Per-sample ploidy detection: 2 if a heterozygote occurs among the first
+ * min(1000, n_snp) SNPs, else 1 (reference: ComputeBackend::detect_sample_ploidy_device,
+ * src/device/backend.hpp:765; src/device/cpu/cpu_backend.cpp:73-95). out: host int32[n_individuals]. */
+int stp_detect_ploidy(stp_context* ctx, const stp_tile* tile, int32_t* out_ploidy);
+
+/* Integer decode (reference: ComputeBackend::decode_af, src/device/backend.hpp:736,
+ * in its exact integer form): d_a2[p*ld + s] = 2*ac, d_n[p*ld + s] = n as uint16,
+ * ld >= n_snp and a multiple of 16.  q = a2 / (2 n) equals the reference's ac / n. */
+int stp_decode_counts(stp_context* ctx, const stp_tile* tile, uint16_t* d_a2, uint16_t* d_n,
+                      int64_t ld);
+
+/* Floating decode, bit-identical to the reference's DecodeResult{q, v, n}
+ * (src/device/backend.hpp:250-256): host [P x M] arrays at i + P*s. */
+int stp_decode_af(stp_context* ctx, const stp_tile* tile, double* q, double* v, double* n);
+
+/* ---- f2 ----------------------------------------------------------------------- */
+
+/* Per-block f2 from device counts (reference: ComputeBackend::compute_f2_blocks_device,
+ * src/device/backend.hpp:694, fed by the decode).  block_id: host int32[M], contiguous
+ * non-decreasing in [0, n_block).  Writes device d_f2, d_vpair [P x P x n_block]. */
+int stp_f2_blocks_from_counts(stp_context* ctx, const uint16_t* d_a2, const uint16_t* d_n,
+                              int64_t ld, int32_t P, int64_t M, const int32_t* block_id,
+                              int32_t n_block, int32_t precision, double* d_f2, double* d_vpair);
+
+/* Per-block f2 from host FP64 Q/V/N [P x M] (reference: ComputeBackend::compute_f2_blocks,
+ * src/device/backend.hpp:687).  Host outputs f2, vpair [P x P x n_block], block_sizes[n_block]. */
+int stp_compute_f2_blocks(stp_context* ctx, const double* Q, const double* V, const double* N,
+                          int32_t P, int64_t M, const int32_t* block_id, int32_t n_block,
+                          int32_t precision, double* f2, double* vpair, int32_t* block_sizes);
+
+/* Genotypes -> resident f2 handle in one call: decode, block assembly and the f2
+ * contraction on the device (reference: run_extract_f2's compute core,
+ * src/app/extract_f2_core.cpp:93-407, with the filter keeping every SNP). */
+int stp_extract_f2(stp_context* ctx, const stp_tile* tile, const int32_t* block_id,
+                   int32_t n_block, int32_t precision, stp_f2_handle** out);
+
+/* ---- f2 handle (reference: DeviceF2Blocks, src/device/device_f2_blocks.hpp:18-53) ---- */
+
+/* Upload a host tensor (reference: upload_f2_blocks_to_device, device_f2_blocks.hpp:53).
+ * vpair may be NULL (treated as all-present). */
+int stp_f2_handle_upload(stp_context* ctx, const double* f2, const double* vpair,
+                         const int32_t* block_sizes, int32_t P, int32_t n_block,
+                         stp_f2_handle** out);
+int stp_f2_handle_shape(const stp_f2_handle* h, int32_t* P, int32_t* n_block);
+int stp_f2_handle_block_sizes(const stp_f2_handle* h, int32_t* block_sizes);
+/* Device-resident tensors (reference: DeviceF2Blocks::f2_device / vpair_device). */
+const double* stp_f2_handle_f2_device(const stp_f2_handle* h);
+const double* stp_f2_handle_vpair_device(const stp_f2_handle* h);
+/* Copy back to host (reference: DeviceF2Blocks::to_host). */
+int stp_f2_handle_download(const stp_f2_handle* h, double* f2, double* vpair);
+int stp_f2_handle_free(stp_f2_handle* h);
+
+/* ---- STPF2BK1 f2.bin (reference: src/device/f2_disk_format.hpp:17-50,
+ *      src/app/f2_dir_writer.cpp:306-440, src/app/f2_dir_io.cpp:52-158) ------------- */
+
+int stp_f2bin_write(const char* path, const double* f2, const double* vpair,
+                    const int32_t* block_sizes, int32_t P, int32_t n_block);
+int stp_f2bin_read_header(const char* path, int32_t* P, int32_t* n_block);
+int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_sizes);
+
+/* ---- instrumentation (no counterpart in the reference) ---------------------------- */
+
+/* Enable per-stage device timing: HIP events recorded on the context's own stream
+ * around the decode, prep and f2 contraction launches. */
+int stpx_timing(stp_context* ctx, int32_t enable);
+/* Accumulated milliseconds and launch counts per stage [decode, prep, f2, count]
+ * (f2 = the contraction kernel, count = the pair-count kernel); reset
+ * clears them. */
+int stpx_timing_read(stp_context* ctx, double* ms, int64_t* count, int32_t reset);
+
+/* ---- synthetic panels (bench/test input; no counterpart in the reference) ------------ */
+
+/* Generate a packed panel directly in device memory.  Per population: F_ST fst[p],
+ * missing rate pop_missing[p]; per individual: population ind_pop[g] and
+ * ind_haploid[g] (1 = pseudo-haploid).  d_packed must hold n_ind * bytes_per_record
+ * bytes; bytes_per_record is a multiple of 4 and >= ceil(M/4). */
+int stpx_synth_panel(stp_context* ctx, uint64_t seed, int32_t P, int64_t M, int64_t n_ind,
+                     int64_t bytes_per_record, const float* fst, const float* pop_missing,
+                     const int32_t* ind_pop, const int8_t* ind_haploid, uint8_t* d_packed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STEPPE_HIP_H */

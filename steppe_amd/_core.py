@@ -1,0 +1,391 @@
+"""ctypes binding of libsteppe_hip.so — the Python face of the C ABI.
+
+Mirrors the reference's nanobind module ``steppe._core`` for the f-statistics
+path (bindings/bind_f2handle.cpp:14-47, bindings/bind_fstats.cpp:131-215):
+``F2Handle``, ``read_f2``, ``run_extract_f2`` and the fit entry points, with the
+same argument names and error behaviour (ValueError for invalid input or device
+faults, KeyError for unknown population names).
+
+There is no CPU fallback: importing works anywhere, but every call needs the
+built HIP library and a visible GPU, and raises when either is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsteppe_hip.so")
+
+STP_PREC_FP64 = 0
+STP_PREC_SPLIT_BF16 = 1
+
+_PRECISIONS = {
+    None: STP_PREC_SPLIT_BF16,
+    "split_bf16": STP_PREC_SPLIT_BF16, "split": STP_PREC_SPLIT_BF16,
+    "emu": STP_PREC_SPLIT_BF16, "emu40": STP_PREC_SPLIT_BF16, "emulated_fp64": STP_PREC_SPLIT_BF16,
+    "fp64": STP_PREC_FP64, "native": STP_PREC_FP64,
+}
+PRECISION_NAMES = {STP_PREC_FP64: "fp64", STP_PREC_SPLIT_BF16: "split_bf16"}
+
+
+class Tile(C.Structure):
+    """stp_tile (include/steppe_hip.h) — the reference's DecodeTileView."""
+    _fields_ = [
+        ("packed", C.c_void_p),
+        ("bytes_per_record", C.c_size_t),
+        ("n_snp", C.c_size_t),
+        ("n_individuals", C.c_size_t),
+        ("pop_offsets", C.c_void_p),
+        ("n_pop", C.c_int32),
+        ("sample_ploidy", C.c_void_p),
+        ("ploidy", C.c_int32),
+        ("detect_ploidy_on_device", C.c_int32),
+        ("packed_on_device", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the engine; raise loudly when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"steppe_amd: HIP engine not built ({LIB_PATH} missing); "
+                           "run `python -m steppe_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, sz, dp = C.c_void_p, C.c_int32, C.c_int64, C.c_size_t, C.c_double
+    sig = {
+        "stp_open": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "stp_close": (C.c_int, [vp]),
+        "stp_last_error": (C.c_char_p, []),
+        "stp_device_info": (C.c_int, [vp, C.c_char_p, C.c_int, C.POINTER(i32)]),
+        "stp_detect_ploidy": (C.c_int, [vp, C.POINTER(Tile), vp]),
+        "stp_decode_counts": (C.c_int, [vp, C.POINTER(Tile), vp, vp, i64]),
+        "stp_decode_af": (C.c_int, [vp, C.POINTER(Tile), vp, vp, vp]),
+        "stp_f2_blocks_from_counts": (C.c_int, [vp, vp, vp, i64, i32, i64, vp, i32, i32, vp, vp]),
+        "stp_compute_f2_blocks": (C.c_int, [vp, vp, vp, vp, i32, i64, vp, i32, i32, vp, vp, vp]),
+        "stp_extract_f2": (C.c_int, [vp, C.POINTER(Tile), vp, i32, i32, C.POINTER(vp)]),
+        "stp_f2_handle_upload": (C.c_int, [vp, vp, vp, vp, i32, i32, C.POINTER(vp)]),
+        "stp_f2_handle_shape": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32)]),
+        "stp_f2_handle_block_sizes": (C.c_int, [vp, vp]),
+        "stp_f2_handle_f2_device": (vp, [vp]),
+        "stp_f2_handle_vpair_device": (vp, [vp]),
+        "stp_f2_handle_download": (C.c_int, [vp, vp, vp]),
+        "stp_f2_handle_free": (C.c_int, [vp]),
+        "stp_f2bin_write": (C.c_int, [C.c_char_p, vp, vp, vp, i32, i32]),
+        "stp_f2bin_read_header": (C.c_int, [C.c_char_p, C.POINTER(i32), C.POINTER(i32)]),
+        "stp_f2bin_read": (C.c_int, [C.c_char_p, vp, vp, vp]),
+        "stpx_timing": (C.c_int, [vp, i32]),
+        "stpx_timing_read": (C.c_int, [vp, vp, vp, i32]),
+        "stpx_synth_panel": (C.c_int, [vp, C.c_uint64, i32, i64, i64, i64, vp, vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    del sz, dp
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = ""):
+    if rc == 0:
+        return
+    msg = lib().stp_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == -3:
+        raise MemoryError(text)
+    if rc in (-4, -5):
+        raise ValueError(text)
+    raise ValueError(text)
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One engine context per HIP device (the reference's per-GPU backend)."""
+    _by_device: dict = {}
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().stp_open(int(device), C.byref(h)), f"open device {device}")
+        self.h = h
+        self.device = int(device)
+
+    @classmethod
+    def get(cls, device: int = 0) -> "Context":
+        ctx = cls._by_device.get(int(device))
+        if ctx is None:
+            ctx = cls(device)
+            cls._by_device[int(device)] = ctx
+        return ctx
+
+    STAGES = ("decode", "prep", "f2", "count")
+
+    def timing(self, enable: bool = True):
+        check(lib().stpx_timing(self.h, 1 if enable else 0))
+
+    def read_timing(self, reset: bool = False) -> dict:
+        ms = np.zeros(4, np.float64)
+        cnt = np.zeros(4, np.int64)
+        check(lib().stpx_timing_read(self.h, _ptr(ms), _ptr(cnt), 1 if reset else 0))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.STAGES)}
+
+    def info(self):
+        buf = C.create_string_buffer(64)
+        n = C.c_int32()
+        check(lib().stp_device_info(self.h, buf, 64, C.byref(n)))
+        return buf.value.decode(), int(n.value)
+
+
+def make_tile(packed, n_snp: int, pop_offsets, sample_ploidy=None, ploidy: int = 2,
+              detect: bool = False, bytes_per_record: int | None = None,
+              n_individuals: int | None = None, device_ptr: int | None = None):
+    """Build a Tile.  ``packed`` is a uint8 [n_ind, bytes_per_record] host array, or
+    None with ``device_ptr`` (plus bytes_per_record, n_individuals) for an
+    HBM-resident panel.  Returns (tile, keepalive)."""
+    keep = []
+    offs = np.ascontiguousarray(pop_offsets, dtype=np.uint64)
+    keep.append(offs)
+    t = Tile()
+    if device_ptr is not None:
+        t.packed = C.c_void_p(int(device_ptr))
+        t.bytes_per_record = int(bytes_per_record)
+        t.n_individuals = int(n_individuals)
+        t.packed_on_device = 1
+    else:
+        arr = np.ascontiguousarray(packed, dtype=np.uint8)
+        keep.append(arr)
+        t.packed = _ptr(arr)
+        t.bytes_per_record = int(arr.shape[1]) if bytes_per_record is None else int(bytes_per_record)
+        t.n_individuals = int(arr.shape[0])
+        t.packed_on_device = 0
+    t.n_snp = int(n_snp)
+    t.pop_offsets = _ptr(offs)
+    t.n_pop = int(offs.shape[0] - 1)
+    if sample_ploidy is not None:
+        sp = np.ascontiguousarray(sample_ploidy, dtype=np.int32)
+        keep.append(sp)
+        t.sample_ploidy = _ptr(sp)
+    else:
+        t.sample_ploidy = None
+    t.ploidy = int(ploidy)
+    t.detect_ploidy_on_device = 1 if detect else 0
+    return t, keep
+
+
+def tile_from_panel(panel, ploidy="sample"):
+    if ploidy == "sample":
+        return make_tile(panel.packed, panel.n_snp, panel.pop_offsets, panel.sample_ploidy)
+    if ploidy == "auto":
+        return make_tile(panel.packed, panel.n_snp, panel.pop_offsets, None, detect=True)
+    return make_tile(panel.packed, panel.n_snp, panel.pop_offsets, None, ploidy=int(ploidy))
+
+
+class F2Handle:
+    """Resident f2 tensor plus population labels (bindings/bind_f2handle.cpp:24-47).
+
+    Unlike the reference's handle, which keeps a host tensor and uploads it on every
+    fit, this one keeps the tensor resident in HBM; fits read it in place."""
+
+    def __init__(self, raw: C.c_void_p, pops, device: int):
+        self._h = raw
+        self._pops = list(pops)
+        self._device = int(device)
+        P, nb = C.c_int32(), C.c_int32()
+        check(lib().stp_f2_handle_shape(raw, C.byref(P), C.byref(nb)))
+        self._P, self._nb = int(P.value), int(nb.value)
+        if len(self._pops) != self._P:
+            raise ValueError(f"pops has {len(self._pops)} labels but P = {self._P}")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.stp_f2_handle_free(h)
+            self._h = None
+
+    @property
+    def pops(self):
+        return list(self._pops)
+
+    @property
+    def P(self) -> int:
+        return self._P
+
+    @property
+    def n_block(self) -> int:
+        return self._nb
+
+    @property
+    def device(self) -> int:
+        return self._device
+
+    @property
+    def block_sizes(self):
+        out = np.zeros(self._nb, np.int32)
+        check(lib().stp_f2_handle_block_sizes(self._h, _ptr(out)))
+        return out.tolist()
+
+    @property
+    def raw(self):
+        return self._h
+
+    def device_pointers(self):
+        L = lib()
+        return int(L.stp_f2_handle_f2_device(self._h)), int(L.stp_f2_handle_vpair_device(self._h))
+
+    def _download(self, which: str) -> np.ndarray:
+        n = self._P * self._P * self._nb
+        out = np.empty(n, np.float64)
+        if which == "f2":
+            check(lib().stp_f2_handle_download(self._h, _ptr(out), None))
+        else:
+            check(lib().stp_f2_handle_download(self._h, None, _ptr(out)))
+        return out.reshape((self._P, self._P, self._nb), order="F")
+
+    def _f2_numpy(self) -> np.ndarray:
+        """f2 as float64, F-contiguous (P, P, n_block). COPY."""
+        return self._download("f2")
+
+    def _vpair_numpy(self) -> np.ndarray:
+        """vpair as float64, F-contiguous (P, P, n_block). COPY."""
+        return self._download("vpair")
+
+
+def upload_f2(f2, vpair, block_sizes, pops, device: int = 0) -> F2Handle:
+    """Host tensor -> resident handle (the reference's upload_f2_blocks_to_device)."""
+    f2 = np.asarray(f2, dtype=np.float64)
+    if f2.ndim != 3 or f2.shape[0] != f2.shape[1]:
+        raise ValueError("f2 must have shape (P, P, n_block)")
+    P, _, nb = f2.shape
+    fl = np.asfortranarray(f2).ravel(order="F")
+    vp = None if vpair is None else np.asfortranarray(np.asarray(vpair, np.float64)).ravel(order="F")
+    bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+    if bs.shape != (nb,):
+        raise ValueError("block_sizes must have n_block entries")
+    ctx = Context.get(device)
+    raw = C.c_void_p()
+    check(lib().stp_f2_handle_upload(ctx.h, _ptr(fl), None if vp is None else _ptr(vp), _ptr(bs),
+                                     int(P), int(nb), C.byref(raw)), "upload")
+    return F2Handle(raw, pops, device)
+
+
+def _sha256(path: str) -> str:
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def write_f2_dir(dir_path: str, f2, vpair, block_sizes, pops, meta: dict | None = None) -> str:
+    """Write an STPF2BK1 f2 directory: f2.bin, pops.txt, meta.json
+    (src/app/f2_dir_writer.cpp:306-440)."""
+    f2 = np.asarray(f2, dtype=np.float64)
+    P, _, nb = f2.shape
+    if len(pops) != P:
+        raise ValueError(f"write_f2_dir: pop_labels has {len(pops)} entries but f2 tensor P={P}")
+    os.makedirs(dir_path, exist_ok=True)
+    fl = np.asfortranarray(f2).ravel(order="F")
+    vp = np.asfortranarray(np.asarray(vpair, np.float64)).ravel(order="F")
+    bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+    bin_path = os.path.join(dir_path, "f2.bin")
+    check(lib().stp_f2bin_write(bin_path.encode(), _ptr(fl), _ptr(vp), _ptr(bs), int(P), int(nb)),
+          "write_f2_dir")
+    pops_path = os.path.join(dir_path, "pops.txt")
+    with open(pops_path, "w") as f:
+        for s in pops:
+            f.write(f"{s}\n")
+    m = {"format": "STPF2BK1", "meta_schema_version": 1, "steppe_version": "steppe_amd",
+         "P": int(P), "n_block": int(nb)}
+    m.update(meta or {})
+    m["pops_sha256"] = "sha256:" + _sha256(pops_path)
+    m["f2_cache_id"] = "sha256:" + _sha256(bin_path)
+    with open(os.path.join(dir_path, "meta.json"), "w") as f:
+        json.dump(m, f, indent=2)
+        f.write("\n")
+    return dir_path
+
+
+def read_f2_host(dir_path: str):
+    """Read f2.bin + pops.txt into host arrays (src/app/f2_dir_io.cpp:52-158)."""
+    bin_path = os.path.join(dir_path, "f2.bin")
+    if not os.path.exists(bin_path):
+        raise ValueError(f"read_f2: no f2.bin in {dir_path}")
+    P, nb = C.c_int32(), C.c_int32()
+    check(lib().stp_f2bin_read_header(bin_path.encode(), C.byref(P), C.byref(nb)), "read_f2")
+    P, nb = int(P.value), int(nb.value)
+    f2 = np.empty(P * P * nb, np.float64)
+    vp = np.empty_like(f2)
+    bs = np.empty(nb, np.int32)
+    check(lib().stp_f2bin_read(bin_path.encode(), _ptr(f2), _ptr(vp), _ptr(bs)), "read_f2")
+    pops_path = os.path.join(dir_path, "pops.txt")
+    if os.path.exists(pops_path):
+        with open(pops_path) as f:
+            pops = [ln.rstrip("\n") for ln in f if ln.strip()]
+    else:
+        pops = [f"pop{i}" for i in range(P)]
+    if len(pops) != P:
+        raise ValueError(f"read_f2: pops.txt has {len(pops)} labels but f2.bin has P={P}")
+    return (f2.reshape((P, P, nb), order="F"), vp.reshape((P, P, nb), order="F"), bs, pops)
+
+
+def read_f2(dir: str, device: int = 0) -> F2Handle:
+    """Load an f2 directory into a resident F2Handle (bindings/bind_f2handle.cpp:14-22)."""
+    f2, vp, bs, pops = read_f2_host(dir)
+    return upload_f2(f2, vp, bs, pops, device)
+
+
+def parse_precision(precision) -> int:
+    if precision not in _PRECISIONS:
+        raise ValueError("precision must be one of 'split_bf16'/'emu', 'fp64'/'native' "
+                         f"(got '{precision}')")
+    return _PRECISIONS[precision]
+
+
+def run_extract_f2(packed, n_snp: int, pop_offsets, block_id, n_block: int, pops=None,
+                   out: str = "", device: int = 0, ploidy="auto", precision=None,
+                   sample_ploidy=None):
+    """Genotype tile -> f2 (the compute core of bindings/bind_fstats.cpp:131-215).
+
+    The reference's entry reads GENO files by prefix; this one takes the decoded
+    front-end's product directly: a packed individual-major tile, population
+    offsets and the jackknife block partition.  ploidy: "auto" (detect), 1 or 2,
+    or "sample" with sample_ploidy.  Returns an F2Handle, or the directory path
+    when ``out`` is set."""
+    n_pop = len(pop_offsets) - 1
+    pops = list(pops) if pops is not None else [f"pop{i}" for i in range(n_pop)]
+    if sample_ploidy is not None:
+        tile, keep = make_tile(packed, n_snp, pop_offsets, sample_ploidy)
+    elif ploidy == "auto":
+        tile, keep = make_tile(packed, n_snp, pop_offsets, None, detect=True)
+    elif ploidy in (1, 2, "1", "2"):
+        tile, keep = make_tile(packed, n_snp, pop_offsets, None, ploidy=int(ploidy))
+    else:
+        raise ValueError(f"ploidy must be 'auto', 1 or 2 (got {ploidy!r})")
+    prec = parse_precision(precision)
+    bid = np.ascontiguousarray(block_id, dtype=np.int32)
+    if bid.shape != (int(n_snp),):
+        raise ValueError("block_id must have one entry per SNP")
+    ctx = Context.get(device)
+    raw = C.c_void_p()
+    check(lib().stp_extract_f2(ctx.h, C.byref(tile), _ptr(bid), int(n_block), prec, C.byref(raw)),
+          "run_extract_f2")
+    del keep
+    h = F2Handle(raw, pops, device)
+    if out:
+        write_f2_dir(out, h._f2_numpy(), h._vpair_numpy(), h.block_sizes, h.pops,
+                     {"precision_tag": PRECISION_NAMES[prec], "n_snp_total": int(n_snp),
+                      "n_snp_kept": int(n_snp)})
+        return out
+    return h

@@ -1,0 +1,322 @@
+// steppe_amd/csrc/capi.cpp — the extern "C" boundary (include/steppe_hip.h).
+//
+// Every entry point runs inside guard(): exceptions never cross the ABI; they are
+// mapped to status codes and the message is kept per thread for stp_last_error().
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "engine.hpp"
+#include "steppe_hip.h"
+
+struct stp_context {
+    std::unique_ptr<stp::HipBackend> be;
+};
+
+struct stp_f2_handle {
+    stp_context* ctx = nullptr;
+    std::unique_ptr<stp::F2Blocks> blocks;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct FormatError : std::runtime_error {
+    explicit FormatError(const std::string& m) : std::runtime_error(m) {}
+};
+struct IoError : std::runtime_error {
+    explicit IoError(const std::string& m) : std::runtime_error(m) {}
+};
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return STP_OK;
+    } catch (const stp::OomError& e) {
+        g_last_error = e.what();
+        return STP_ERR_OOM;
+    } catch (const stp::DeviceError& e) {
+        g_last_error = e.what();
+        return STP_ERR_DEVICE;
+    } catch (const stp::ArgError& e) {
+        g_last_error = e.what();
+        return STP_ERR_INVALID;
+    } catch (const FormatError& e) {
+        g_last_error = e.what();
+        return STP_ERR_FORMAT;
+    } catch (const IoError& e) {
+        g_last_error = e.what();
+        return STP_ERR_IO;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return STP_ERR_OOM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return STP_ERR_INTERNAL;
+    }
+}
+
+stp::HipBackend& backend(stp_context* ctx) {
+    STP_REQUIRE(ctx && ctx->be, "null context");
+    return *ctx->be;
+}
+
+const stp::F2Blocks& blocks_of(const stp_f2_handle* h) {
+    STP_REQUIRE(h && h->blocks, "null f2 handle");
+    return *h->blocks;
+}
+
+// STPF2BK1 header (src/device/f2_disk_format.hpp:17-50): 64 bytes, little-endian.
+struct F2BinHeader {
+    char magic[8];
+    std::uint32_t version;
+    std::uint32_t dtype;
+    std::int32_t P;
+    std::int32_t n_block;
+    std::uint64_t f2_offset;
+    std::uint64_t vpair_offset;
+    std::uint64_t block_sizes_offset;
+    std::uint8_t reserved[16];
+};
+static_assert(sizeof(F2BinHeader) == 64, "STPF2BK1 header is 64 bytes");
+constexpr char kMagic[8] = {'S', 'T', 'P', 'F', '2', 'B', 'K', '1'};
+
+F2BinHeader read_header(std::ifstream& f, const std::string& path) {
+    F2BinHeader h{};
+    f.read(reinterpret_cast<char*>(&h), sizeof(h));
+    if (!f) throw FormatError("f2.bin: truncated header: " + path);
+    if (std::memcmp(h.magic, kMagic, 8) != 0) throw FormatError("f2.bin: bad magic (not STPF2BK1): " + path);
+    if (h.version != 1u) throw FormatError("f2.bin: unsupported version " + std::to_string(h.version));
+    if (h.dtype != 1u) throw FormatError("f2.bin: unsupported dtype " + std::to_string(h.dtype));
+    if (h.P <= 0 || h.n_block <= 0) throw FormatError("f2.bin: degenerate shape");
+    const std::uint64_t slab = static_cast<std::uint64_t>(h.P) * h.P * h.n_block * sizeof(double);
+    if (h.f2_offset != 64 || h.vpair_offset != 64 + slab || h.block_sizes_offset != 64 + 2 * slab)
+        throw FormatError("f2.bin: section offsets do not match P and n_block");
+    f.seekg(0, std::ios::end);
+    const std::uint64_t size = static_cast<std::uint64_t>(f.tellg());
+    if (size < h.block_sizes_offset + 4ull * h.n_block) throw FormatError("f2.bin: truncated file: " + path);
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* stp_last_error(void) { return g_last_error.c_str(); }
+
+int stp_open(int device, stp_context** out) {
+    return guard([&] {
+        STP_REQUIRE(out != nullptr, "stp_open: out is null");
+        auto c = std::make_unique<stp_context>();
+        c->be = std::make_unique<stp::HipBackend>(device);
+        *out = c.release();
+    });
+}
+
+int stp_close(stp_context* ctx) {
+    return guard([&] { delete ctx; });
+}
+
+int stp_device_info(stp_context* ctx, char* name, int name_len, int32_t* n_cu) {
+    return guard([&] {
+        auto& be = backend(ctx);
+        if (name && name_len > 0) {
+            std::strncpy(name, be.name().c_str(), static_cast<std::size_t>(name_len - 1));
+            name[name_len - 1] = '\0';
+        }
+        if (n_cu) *n_cu = be.n_cu();
+    });
+}
+
+int stp_detect_ploidy(stp_context* ctx, const stp_tile* tile, int32_t* out_ploidy) {
+    return guard([&] {
+        STP_REQUIRE(tile && out_ploidy, "stp_detect_ploidy: null argument");
+        const auto p = backend(ctx).detect_ploidy(*tile);
+        for (std::size_t g = 0; g < p.size(); ++g) out_ploidy[g] = p[g];
+    });
+}
+
+int stp_decode_counts(stp_context* ctx, const stp_tile* tile, uint16_t* d_a2, uint16_t* d_n, int64_t ld) {
+    return guard([&] {
+        STP_REQUIRE(tile && d_a2 && d_n, "stp_decode_counts: null argument");
+        backend(ctx).decode_counts(*tile, d_a2, d_n, ld);
+    });
+}
+
+int stp_decode_af(stp_context* ctx, const stp_tile* tile, double* q, double* v, double* n) {
+    return guard([&] {
+        STP_REQUIRE(tile && q && v && n, "stp_decode_af: null argument");
+        backend(ctx).decode_af(*tile, q, v, n);
+    });
+}
+
+int stp_f2_blocks_from_counts(stp_context* ctx, const uint16_t* d_a2, const uint16_t* d_n, int64_t ld,
+                              int32_t P, int64_t M, const int32_t* block_id, int32_t n_block,
+                              int32_t precision, double* d_f2, double* d_vpair) {
+    return guard([&] {
+        STP_REQUIRE(d_a2 && d_n && d_f2 && d_vpair, "stp_f2_blocks_from_counts: null argument");
+        backend(ctx).f2_from_counts(d_a2, d_n, ld, P, M, block_id, n_block, precision, d_f2, d_vpair);
+    });
+}
+
+int stp_compute_f2_blocks(stp_context* ctx, const double* Q, const double* V, const double* N, int32_t P,
+                          int64_t M, const int32_t* block_id, int32_t n_block, int32_t precision, double* f2,
+                          double* vpair, int32_t* block_sizes) {
+    return guard([&] {
+        STP_REQUIRE(f2 && vpair && block_sizes, "stp_compute_f2_blocks: null output");
+        STP_REQUIRE(M == 0 || (Q && V && N), "stp_compute_f2_blocks: null Q/V/N");
+        backend(ctx).f2_from_qvn(Q, V, N, P, M, block_id, n_block, precision, f2, vpair, block_sizes);
+    });
+}
+
+int stp_extract_f2(stp_context* ctx, const stp_tile* tile, const int32_t* block_id, int32_t n_block,
+                   int32_t precision, stp_f2_handle** out) {
+    return guard([&] {
+        STP_REQUIRE(tile && out, "stp_extract_f2: null argument");
+        auto h = std::make_unique<stp_f2_handle>();
+        h->ctx = ctx;
+        h->blocks.reset(backend(ctx).extract_f2(*tile, block_id, n_block, precision));
+        *out = h.release();
+    });
+}
+
+int stp_f2_handle_upload(stp_context* ctx, const double* f2, const double* vpair, const int32_t* block_sizes,
+                         int32_t P, int32_t n_block, stp_f2_handle** out) {
+    return guard([&] {
+        STP_REQUIRE(out, "stp_f2_handle_upload: null out");
+        auto h = std::make_unique<stp_f2_handle>();
+        h->ctx = ctx;
+        h->blocks.reset(backend(ctx).upload(f2, vpair, block_sizes, P, n_block));
+        *out = h.release();
+    });
+}
+
+int stp_f2_handle_shape(const stp_f2_handle* h, int32_t* P, int32_t* n_block) {
+    return guard([&] {
+        const auto& b = blocks_of(h);
+        if (P) *P = b.P;
+        if (n_block) *n_block = b.n_block;
+    });
+}
+
+int stp_f2_handle_block_sizes(const stp_f2_handle* h, int32_t* block_sizes) {
+    return guard([&] {
+        const auto& b = blocks_of(h);
+        STP_REQUIRE(block_sizes, "null output");
+        std::memcpy(block_sizes, b.block_sizes.data(), b.block_sizes.size() * sizeof(int32_t));
+    });
+}
+
+const double* stp_f2_handle_f2_device(const stp_f2_handle* h) {
+    return (h && h->blocks) ? h->blocks->f2.get() : nullptr;
+}
+
+const double* stp_f2_handle_vpair_device(const stp_f2_handle* h) {
+    return (h && h->blocks) ? h->blocks->vpair.get() : nullptr;
+}
+
+int stp_f2_handle_download(const stp_f2_handle* h, double* f2, double* vpair) {
+    return guard([&] {
+        const auto& b = blocks_of(h);
+        auto& be = backend(h->ctx);
+        be.activate();
+        const std::size_t tot = static_cast<std::size_t>(b.P) * b.P * b.n_block;
+        if (f2) STP_HIP(hipMemcpyAsync(f2, b.f2.get(), tot * sizeof(double), hipMemcpyDeviceToHost, be.stream()));
+        if (vpair)
+            STP_HIP(hipMemcpyAsync(vpair, b.vpair.get(), tot * sizeof(double), hipMemcpyDeviceToHost, be.stream()));
+        be.sync();
+    });
+}
+
+int stp_f2_handle_free(stp_f2_handle* h) {
+    return guard([&] {
+        if (h && h->ctx && h->ctx->be) h->ctx->be->activate();
+        delete h;
+    });
+}
+
+int stp_f2bin_write(const char* path, const double* f2, const double* vpair, const int32_t* block_sizes,
+                    int32_t P, int32_t n_block) {
+    return guard([&] {
+        STP_REQUIRE(path && f2 && vpair && block_sizes, "stp_f2bin_write: null argument");
+        STP_REQUIRE(P > 0 && n_block > 0, "stp_f2bin_write: degenerate f2 tensor");
+        const std::uint64_t slab = static_cast<std::uint64_t>(P) * P * n_block * sizeof(double);
+        F2BinHeader h{};
+        std::memcpy(h.magic, kMagic, 8);
+        h.version = 1;
+        h.dtype = 1;
+        h.P = P;
+        h.n_block = n_block;
+        h.f2_offset = 64;
+        h.vpair_offset = 64 + slab;
+        h.block_sizes_offset = 64 + 2 * slab;
+        std::ofstream o(path, std::ios::binary | std::ios::trunc);
+        if (!o) throw IoError(std::string("cannot open for write: ") + path);
+        o.write(reinterpret_cast<const char*>(&h), sizeof(h));
+        o.write(reinterpret_cast<const char*>(f2), static_cast<std::streamsize>(slab));
+        o.write(reinterpret_cast<const char*>(vpair), static_cast<std::streamsize>(slab));
+        o.write(reinterpret_cast<const char*>(block_sizes), static_cast<std::streamsize>(4 * n_block));
+        o.flush();
+        if (!o) throw IoError(std::string("f2.bin write failed (disk full?): ") + path);
+    });
+}
+
+int stp_f2bin_read_header(const char* path, int32_t* P, int32_t* n_block) {
+    return guard([&] {
+        STP_REQUIRE(path, "null path");
+        std::ifstream f(path, std::ios::binary);
+        if (!f) throw IoError(std::string("cannot open: ") + path);
+        const F2BinHeader h = read_header(f, path);
+        if (P) *P = h.P;
+        if (n_block) *n_block = h.n_block;
+    });
+}
+
+int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_sizes) {
+    return guard([&] {
+        STP_REQUIRE(path, "null path");
+        std::ifstream f(path, std::ios::binary);
+        if (!f) throw IoError(std::string("cannot open: ") + path);
+        const F2BinHeader h = read_header(f, path);
+        const std::uint64_t slab = static_cast<std::uint64_t>(h.P) * h.P * h.n_block * sizeof(double);
+        if (f2) {
+            f.seekg(static_cast<std::streamoff>(h.f2_offset));
+            f.read(reinterpret_cast<char*>(f2), static_cast<std::streamsize>(slab));
+        }
+        if (vpair) {
+            f.seekg(static_cast<std::streamoff>(h.vpair_offset));
+            f.read(reinterpret_cast<char*>(vpair), static_cast<std::streamsize>(slab));
+        }
+        if (block_sizes) {
+            f.seekg(static_cast<std::streamoff>(h.block_sizes_offset));
+            f.read(reinterpret_cast<char*>(block_sizes), static_cast<std::streamsize>(4 * h.n_block));
+        }
+        if (!f) throw FormatError(std::string("f2.bin: read failed: ") + path);
+    });
+}
+
+int stpx_timing(stp_context* ctx, int32_t enable) {
+    return guard([&] { backend(ctx).set_timing(enable != 0); });
+}
+
+int stpx_timing_read(stp_context* ctx, double* ms, int64_t* count, int32_t reset) {
+    return guard([&] { backend(ctx).read_timing(ms, count, reset != 0); });
+}
+
+int stpx_synth_panel(stp_context* ctx, uint64_t seed, int32_t P, int64_t M, int64_t n_ind,
+                     int64_t bytes_per_record, const float* fst, const float* pop_missing,
+                     const int32_t* ind_pop, const int8_t* ind_haploid, uint8_t* d_packed) {
+    return guard([&] {
+        STP_REQUIRE(fst && pop_missing && ind_pop && ind_haploid && d_packed, "stpx_synth_panel: null argument");
+        backend(ctx).synth_panel(seed, P, M, n_ind, bytes_per_record, fst, pop_missing, ind_pop, ind_haploid,
+                                 d_packed);
+    });
+}
+
+}  // extern "C"

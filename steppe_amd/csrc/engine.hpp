@@ -1,0 +1,181 @@
+// steppe_amd/csrc/engine.hpp — the host-side engine behind the C ABI.
+//
+// HipBackend mirrors the part of the reference's ComputeBackend seam
+// (src/device/backend.hpp:673-1656) that the f-statistics hot path uses:
+// decode_af / detect_sample_ploidy_device / compute_f2_blocks(_device), and owns
+// the per-device resources the reference's CudaBackend owns (one stream, pooled
+// device workspaces; src/device/cuda/cuda_backend.cuh:576-602).  F2Blocks mirrors
+// DeviceF2Blocks (src/device/device_f2_blocks.hpp:18-53): a move-only owner of the
+// resident [P x P x n_block] f2 and vpair tensors.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "f2_layout.hpp"
+
+struct stp_tile;
+
+namespace stp {
+
+// ---- launchers (defined in the .hip translation units) -----------------------------
+void launch_detect_ploidy(const std::uint8_t* packed, std::int64_t bpr, std::int64_t n_snp,
+                          std::int64_t n_ind, std::int8_t* out, hipStream_t st);
+void launch_fill_ploidy(std::int8_t* out, std::int64_t n, int value, hipStream_t st);
+void launch_decode_counts(const std::uint8_t* packed, std::int64_t bpr, std::int64_t n_snp,
+                          const std::uint64_t* pop_offsets, const std::int8_t* ploidy, int n_pop,
+                          std::uint16_t* a2, std::uint16_t* n, std::int64_t ld, hipStream_t st);
+void launch_counts_to_qvn(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P,
+                          std::int64_t M, double* q, double* v, double* nout, hipStream_t st);
+void launch_zero_u32(unsigned* p, int n, hipStream_t st);
+void launch_prep_fp64_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P,
+                             int G, BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks,
+                             std::int64_t KS, double* pA, double* pV, double* pQ, void* pV8,
+                             hipStream_t st);
+void launch_prep_fp64_qvn(const double* q, const double* v, const double* n, int P, int G, BlockTable bt,
+                          const std::int32_t* chunk_block, std::int64_t n_chunks, std::int64_t KS,
+                          double* pA, double* pV, double* pQ, void* pV8, hipStream_t st);
+void launch_f2_fp64(const double* pA, const double* pV, const double* pQ, const void* pV8, BlockTable bt,
+                    const F2Item* items, int n_items, unsigned* counter, int P, int G, std::int64_t KS,
+                    double* f2, double* vpair, int n_workgroups, hipStream_t st);
+void launch_prep_split_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P, int G32,
+                              int G16, BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks,
+                              std::int64_t KS16, std::uint8_t* planes, void* pV8, hipStream_t st);
+void launch_prep_split_qvn(const double* q, const double* v, const double* n, int P, int G32, int G16,
+                           BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks, std::int64_t KS16,
+                           std::uint8_t* planes, void* pV8, hipStream_t st);
+void launch_count(const void* pV8, BlockTable bt, const F2Item* items, int n_items, unsigned* counter, int P,
+                  int G16, std::int64_t KS64, double* vpair, int n_workgroups, hipStream_t st);
+void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, int n_items, unsigned* counter,
+                     int P, int G32, std::int64_t KS16, const double* vpair, double* f2, int n_workgroups,
+                     hipStream_t st);
+void launch_synth_panel(std::uint64_t seed, int P, std::int64_t M, std::int64_t n_ind, std::int64_t bpr,
+                        const float* fst, const float* pop_missing, const std::int32_t* ind_pop,
+                        const std::int8_t* ind_haploid, float* freq_scratch, std::uint8_t* packed,
+                        hipStream_t st);
+
+// ---- block plan ----------------------------------------------------------------------
+// Host description of one f2 launch: block ranges, padded K layout, LPT work items.
+struct F2Plan {
+    int P = 0, G = 0, T = 0, n_block = 0;
+    std::int64_t M = 0, K_total = 0;
+    std::vector<std::int64_t> s0, k0;
+    std::vector<std::int32_t> len, kpad, chunk_block;
+    std::vector<F2Item> items;     // 64-row tile pairs (FP64 contraction, pair counts)
+    std::vector<F2Item> items128;  // 128-row tile pairs (split contraction)
+    int G32 = 0, T128 = 0;
+};
+
+// Validates block_id (contiguous, non-decreasing, in [0, n_block); the reference's
+// core::block_ranges throws on the same conditions, block_partition_rule.hpp:54-96).
+F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_block);
+
+class F2Blocks {
+public:
+    int P = 0, n_block = 0;
+    DevBuf<double> f2, vpair;
+    std::vector<std::int32_t> block_sizes;
+};
+
+class HipBackend {
+public:
+    explicit HipBackend(int device);
+    ~HipBackend();
+    HipBackend(const HipBackend&) = delete;
+    HipBackend& operator=(const HipBackend&) = delete;
+
+    int device() const { return device_; }
+    int n_cu() const { return n_cu_; }
+    const std::string& name() const { return name_; }
+    hipStream_t stream() const { return stream_; }
+    void sync();
+    void activate() const;
+
+    std::vector<int> detect_ploidy(const stp_tile& t);
+    // Decode into caller-owned device buffers.
+    void decode_counts(const stp_tile& t, std::uint16_t* d_a2, std::uint16_t* d_n, std::int64_t ld);
+    void decode_af(const stp_tile& t, double* q, double* v, double* n);
+    // f2 over device counts into caller-owned device outputs.
+    void f2_from_counts(const std::uint16_t* d_a2, const std::uint16_t* d_n, std::int64_t ld, int P,
+                        std::int64_t M, const std::int32_t* block_id, int n_block, int precision,
+                        double* d_f2, double* d_vpair);
+    // f2 over host Q/V/N into host outputs (the reference's compute_f2_blocks seam).
+    void f2_from_qvn(const double* Q, const double* V, const double* N, int P, std::int64_t M,
+                     const std::int32_t* block_id, int n_block, int precision, double* f2,
+                     double* vpair, std::int32_t* block_sizes);
+    F2Blocks* extract_f2(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision);
+    F2Blocks* upload(const double* f2, const double* vpair, const std::int32_t* block_sizes, int P,
+                     int n_block);
+
+    // Per-stage device timing with HIP events on this backend's stream (opt-in).
+    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kStages = 4 };
+    void set_timing(bool on);
+    void read_timing(double* ms, std::int64_t* count, bool reset);
+
+    void synth_panel(std::uint64_t seed, int P, std::int64_t M, std::int64_t n_ind, std::int64_t bpr,
+                     const float* fst, const float* pop_missing, const std::int32_t* ind_pop,
+                     const std::int8_t* ind_haploid, std::uint8_t* d_packed);
+
+private:
+    struct PlanDev {
+        DevBuf<std::int64_t> s0, k0;
+        DevBuf<std::int32_t> len, kpad, chunk_block;
+        DevBuf<F2Item> items, items128;
+        BlockTable table() const;
+    };
+    void upload_plan(const F2Plan& plan, PlanDev& pd);
+    // The plan for (P, M, block_id, n_block), rebuilt and re-uploaded only when the
+    // partition differs from the previous call's (compared element by element).
+    const F2Plan& plan_for(int P, std::int64_t M, const std::int32_t* block_id, int n_block);
+    F2Plan plan_;
+    std::vector<std::int32_t> plan_bid_;
+    bool plan_valid_ = false;
+    void prepare_tile(const stp_tile& t);
+    // Where the f2 prep reads frequencies from: device counts (a2, n, ld) or device
+    // FP64 Q/V/N [P x M].
+    struct F2Source {
+        const std::uint16_t* a2 = nullptr;
+        const std::uint16_t* n = nullptr;
+        std::int64_t ld = 0;
+        const double* q = nullptr;
+        const double* v = nullptr;
+        const double* nq = nullptr;
+    };
+    void run_f2(const F2Plan& plan, int precision, const F2Source& src, double* d_f2, double* d_vpair);
+
+    int device_ = 0;
+    int n_cu_ = 0;
+    std::string name_;
+    hipStream_t stream_ = nullptr;
+
+    // Pooled workspaces (grow-only).
+    DevBuf<std::uint8_t> packed_stage_;
+    DevBuf<std::uint64_t> pop_offsets_;
+    DevBuf<std::int8_t> ploidy_;
+    DevBuf<std::uint16_t> a2_, n_;
+    DevBuf<double> planeA_, planeV_, planeQ_;
+    DevBuf<std::uint8_t> planeV8_;
+    DevBuf<std::uint8_t> planeSplit_;
+    DevBuf<unsigned> counter_;
+    DevBuf<double> qvn_;
+    DevBuf<float> synth_freq_;
+    PlanDev plan_dev_;
+
+    // Stage timing: events bracket each stage launch; elapsed times are folded
+    // into the totals at the next sync.
+    void mark_begin(Stage s);
+    void mark_end(Stage s);
+    void collect_timing();
+    bool timing_ = false;
+    hipEvent_t ev_begin_[kStages] = {}, ev_end_[kStages] = {};
+    bool pending_[kStages] = {};
+    double total_ms_[kStages] = {};
+    std::int64_t count_[kStages] = {};
+
+    // The tile currently staged on the device.
+    const std::uint8_t* d_packed_ = nullptr;
+};
+
+}  // namespace stp

@@ -1,0 +1,45 @@
+// steppe_amd/csrc/f2_layout.hpp — the block table and operand-plane layout shared by
+// the f2 prep and contraction kernels.
+//
+// Per jackknife block b (contiguous SNP range [s0_b, s0_b + len_b)), the SNP axis
+// is padded to kpad_b = round_up(len_b, 64) and blocks are concatenated: block b
+// occupies padded columns [k0_b, k0_b + kpad_b) of a K axis of length K_total.
+// Padding columns carry V = 0 (and zero operands), so they contribute nothing.
+//
+// Populations are grouped in rows of 16 (G = ceil(P/16) groups; rows >= P are
+// zero).  Every operand plane is stored in MFMA fragment order so that one
+// wavefront load of 64 consecutive elements is exactly one MFMA operand:
+//
+//   FP64 planes (v_mfma_f64_16x16x4f64, 4-SNP K-steps, KS = K_total/4 steps):
+//     plane[(g*KS + t)*64 + lane],  lane -> row 16g + lane%16, SNP 4t + lane/16
+//   int8 V plane (v_mfma_i32_16x16x64_i8, 64-SNP K-steps, KS64 = K_total/64):
+//     v8[(g*KS64 + u)*64 + lane] is 16 bytes: row 16g + lane%16,
+//     SNPs 64u + 16*(lane/16) + 0..15
+//
+// The same fragment serves as the A operand (rows of the left population tile)
+// and the B operand (rows of the right tile), because both MFMA operand layouts
+// put the output index on lane%16 and the K index on lane/16.
+#pragma once
+
+#include <cstdint>
+
+namespace stp {
+
+constexpr int kRowGroup = 16;
+constexpr int kKPad = 64;   // padded block granularity (SNPs)
+constexpr int kTileRows = 64;  // rows per wavefront output tile (4 row groups)
+
+struct BlockTable {
+    std::int64_t* s0;    // first SNP of block b
+    std::int32_t* len;   // SNPs in block b
+    std::int64_t* k0;    // first padded column of block b
+    std::int32_t* kpad;  // padded columns of block b
+};
+
+struct F2Item {
+    std::int32_t b;   // block
+    std::int16_t ti;  // left 64-row tile
+    std::int16_t tj;  // right 64-row tile (ti <= tj)
+};
+
+}  // namespace stp

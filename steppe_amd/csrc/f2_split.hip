@@ -1,0 +1,483 @@
+// steppe_amd/csrc/f2_split.hip — per-block f2 from fixed-point BF16 slices on BF16 MFMA.
+//
+// The reference's default precision is cuBLAS's closed FP64 emulation on integer
+// units (src/device/cuda/f2_block_kernel.cu:121-144).  This is an independent
+// construction from the published Ozaki-scheme idea (Ozaki, Ogita, Oishi & Rump,
+// Numer. Algorithms 59:95-118, 2012; Ootomo, Ozaki & Yokota, IJHPCA 2024): split
+// every operand into fixed-point digits whose pairwise products and partial sums
+// are exact in the MFMA's FP32 accumulator, and recombine the exact partial sums
+// in FP64.  It computes the same estimator as the FP64 path (f2_fp64.hip):
+//
+//   num_ij = sum_s v_i v_j [(q_i - q_j)^2 - h_i - h_j],  f2 = num / vpair
+//
+// Operands.  With the per-SNP centre c_s, q' = q - c (|q'| < 1) and w = -(q'^2 - h)/2
+// (|w| <= 1/2), each is written as five balanced 8-bit digits (|digit| <= 128,
+// exact in BF16):
+//     q~ = 2^-7 sum_k d_k 2^-8k,     w~ = 2^-7 sum_m c_m 2^-8m,     k, m = 0..4
+// w is computed from q~ (not q'), so the device evaluates the estimator exactly for
+// the perturbed frequencies q~ (|q~ - q'| <= 2^-40).  Then
+//     -num_ij / 2 = sum_s [ q~_i q~_j + w~_i v_j + v_i w~_j ]
+//                 = sum_t U_t * L_t,   U_t = 2^(-14-8t),  t = 0..4
+//     L_t = sum_s [ sum_{k+l=t} d_k(i) d_l(j) + e_t(i) v(j) + v(i) e_t(j) ],  e_t = 2^7 c_t
+// Products with k + l >= 5 are dropped (<= 2^-38 per SNP).  Per SNP, |L_t| grows by
+// at most (t+3) 2^14, so 128 SNPs keep every L_t below 2^24: the FP32 MFMA sums are
+// exact.  Every 128 SNPs each level carries its multiple of 2^8 into the level
+// above (exact FP32 operations) and L_0 carries multiples of 2^12 into an FP32
+// high word, so a block of any length accumulates exactly in six FP32 words per
+// output and FP64 appears only in the per-block epilogue.
+//
+// MFMA work per (pair, SNP): 25 BF16 products (15 digit-by-digit, 10 digit-by-V),
+// plus the pair count V V^T, computed exactly by a separate int8 MFMA kernel.
+//
+// Kernel shape: persistent workgroups of 8 wavefronts (2 per SIMD) own a 128x128
+// output tile of one jackknife block; each wavefront owns 64x32 (two 32x32 MFMA
+// tiles, six FP32 accumulators per output).  Each 16-SNP K-step is staged
+// through LDS in two phases (the 5 q-digit
+// planes, then the 5 w-digit planes + V) by direct global->LDS loads into a
+// three-slot ring, so two phases of loads are in flight while one is computed.
+
+#include "common.hpp"
+#include "f2_layout.hpp"
+
+namespace stp {
+
+void launch_zero_u32(unsigned* p, int n, hipStream_t st);
+
+namespace {
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int kPlanes = 11;            // d0..d4, e0..e4, V
+constexpr int kChunk = 1024;           // bytes per (32-row group, 16-SNP step, plane)
+constexpr int kGroups = 8;             // 32-row groups per workgroup tile: 4 left + 4 right
+constexpr int kSlotBytes = kGroups * 6 * kChunk;   // 48 KB (largest phase: 6 planes)
+constexpr int kSlots = 3;
+constexpr int kDPlanes = 5, kEPlanes = 6;
+constexpr int kWaves = 8;
+constexpr double kCentreScale = 256.0;
+
+// ---------------------------------------------------------------------------------
+// prep: frequencies -> 11 BF16 digit planes (32x32x16 MFMA fragment order) + the
+// int8 V plane (16x16x64 fragment order, for the count kernel).
+//   plane chunk (g32, t16, p) holds rows 32 g32 + lane%32, SNPs 16 t16 + 8 (lane/32) + j
+//   at byte offset ((g32 * KS16 + t16) * 11 + p) * 1024 + lane * 16 + 2 j.
+// ---------------------------------------------------------------------------------
+struct CountsSrc {
+    const std::uint16_t* a2;
+    const std::uint16_t* n;
+    std::int64_t ld;
+    __device__ bool get(int r, std::int64_t s, double& q, double& nd) const {
+        const std::int64_t o = static_cast<std::int64_t>(r) * ld + s;
+        const unsigned c = n[o];
+        if (!c) return false;
+        nd = static_cast<double>(c);
+        q = static_cast<double>(a2[o]) / (2.0 * nd);
+        return true;
+    }
+};
+
+struct QvnSrc {
+    const double* q;
+    const double* v;
+    const double* n;
+    int P;
+    __device__ bool get(int r, std::int64_t s, double& qq, double& nd) const {
+        const std::int64_t o = r + static_cast<std::int64_t>(P) * s;
+        if (v[o] == 0.0) return false;
+        qq = q[o];
+        nd = n[o];
+        return true;
+    }
+};
+
+__device__ inline unsigned bf16_bits(double integer_digit) {
+    // The digit is an integer of at most 8 significant bits: float and bf16 exact.
+    return __float_as_uint(static_cast<float>(integer_digit)) >> 16;
+}
+
+template <class Src>
+__global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32, int G16, BlockTable bt,
+                                                         const std::int32_t* __restrict__ chunk_block,
+                                                         std::int64_t KS16, std::uint8_t* __restrict__ planes,
+                                                         i4* __restrict__ pV8) {
+    __shared__ double red_sum[4][64];
+    __shared__ int red_cnt[4][64];
+    __shared__ double centre[64];
+
+    const int tid = threadIdx.x;
+    const std::int64_t c = blockIdx.x;
+    const int b = chunk_block[c];
+    const std::int64_t local0 = 64 * c - bt.k0[b];
+    const std::int64_t snp0 = bt.s0[b] + local0;
+    const int live = static_cast<int>(max<std::int64_t>(0, min<std::int64_t>(64, bt.len[b] - local0)));
+    {
+        const int kk = tid & 63, pg = tid >> 6;
+        double s = 0.0;
+        int cnt = 0;
+        if (kk < live)
+            for (int p = pg; p < P; p += 4) {
+                double q, nd;
+                if (src.get(p, snp0 + kk, q, nd)) { s += q; ++cnt; }
+            }
+        red_sum[pg][kk] = s;
+        red_cnt[pg][kk] = cnt;
+        __syncthreads();
+        if (tid < 64) {
+            const double ts = red_sum[0][tid] + red_sum[1][tid] + red_sum[2][tid] + red_sum[3][tid];
+            const int tc = red_cnt[0][tid] + red_cnt[1][tid] + red_cnt[2][tid] + red_cnt[3][tid];
+            centre[tid] = tc ? rint(ts / tc * kCentreScale) / kCentreScale : 0.0;
+        }
+        __syncthreads();
+    }
+
+    // BF16 digit planes: thread -> (16-SNP step t4 of the chunk, fragment lane).
+    const int t4 = tid >> 6, lane = tid & 63;
+    const std::int64_t t16 = 4 * c + t4;
+    for (int g = 0; g < G32; ++g) {
+        const int r = 32 * g + (lane & 31);
+        unsigned w[kPlanes][4];
+#pragma unroll
+        for (int p = 0; p < kPlanes; ++p)
+#pragma unroll
+            for (int h = 0; h < 4; ++h) w[p][h] = 0u;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int kk = 16 * t4 + 8 * (lane >> 5) + j;
+            double q, nd;
+            if (r < P && kk < live && src.get(r, snp0 + kk, q, nd)) {
+                const double nm1 = nd - 1.0;
+                const double h = q * (1.0 - q) / (nm1 > 1.0 ? nm1 : 1.0);
+                double x = (q - centre[kk]) * 128.0;
+                double d[5];
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    d[k] = rint(x);
+                    x = (x - d[k]) * 256.0;
+                }
+                const double qt = (d[0] + (d[1] + (d[2] + (d[3] + d[4] / 256.0) / 256.0) / 256.0) / 256.0) / 128.0;
+                double y = -0.5 * (qt * qt - h) * 128.0;
+                double e[5];
+#pragma unroll
+                for (int m = 0; m < 5; ++m) {
+                    e[m] = rint(y);
+                    y = (y - e[m]) * 256.0;
+                }
+                const int sh = 16 * (j & 1);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) w[k][j >> 1] |= bf16_bits(d[k]) << sh;
+#pragma unroll
+                for (int m = 0; m < 5; ++m) w[5 + m][j >> 1] |= bf16_bits(e[m] * 128.0) << sh;
+                w[10][j >> 1] |= bf16_bits(1.0) << sh;
+            }
+        }
+        std::uint8_t* base = planes + ((static_cast<std::int64_t>(g) * KS16 + t16) * kPlanes) * kChunk + lane * 16;
+#pragma unroll
+        for (int p = 0; p < kPlanes; ++p)
+            *reinterpret_cast<uint4*>(base + p * kChunk) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
+    }
+    // int8 V plane for the pair-count kernel (16x16x64 layout).
+    if (tid < 64) {
+        for (int g = 0; g < G16; ++g) {
+            const int r = 16 * g + (tid & 15);
+            const int kb = 16 * (tid >> 4);
+            unsigned w[4] = {0, 0, 0, 0};
+            if (r < P) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int kk = kb + j;
+                    double q, nd;
+                    if (kk < live && src.get(r, snp0 + kk, q, nd)) w[j >> 2] |= 1u << (8 * (j & 3));
+                }
+            }
+            i4 v;
+            v[0] = static_cast<int>(w[0]);
+            v[1] = static_cast<int>(w[1]);
+            v[2] = static_cast<int>(w[2]);
+            v[3] = static_cast<int>(w[3]);
+            pV8[(static_cast<std::int64_t>(g) * (KS16 / 4) + c) * 64 + tid] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// pair counts: vpair_ij = sum_s v_i v_j by int8 MFMA (exact int32), written for both
+// (i,j) and (j,i).  Persistent wavefronts over (block, 64x64 tile pair) items.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void count_kernel(const i4* __restrict__ pV8, BlockTable bt,
+                                                    const F2Item* __restrict__ items, int n_items,
+                                                    unsigned* __restrict__ counter, int P, int G16,
+                                                    std::int64_t KS64, double* __restrict__ vpair) {
+    const int lane = threadIdx.x & 63;
+    const std::int64_t slab = static_cast<std::int64_t>(P) * P;
+    for (;;) {
+        int it = 0;
+        if (lane == 0) it = static_cast<int>(atomicAdd(counter, 1u));
+        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+        if (it >= n_items) break;
+        const F2Item wi = items[it];
+        const int b = __builtin_amdgcn_readfirstlane(wi.b);
+        const int gI0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(wi.ti));
+        const int gJ0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(wi.tj));
+        const std::int64_t u0 = bt.k0[b] / 64;
+        const int nu = bt.kpad[b] / 64;
+        bool rowI[4], rowJ[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) { rowI[x] = gI0 + x < G16; rowJ[x] = gJ0 + x < G16; }
+        i4 cnt[4][4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) cnt[x][y] = i4{0, 0, 0, 0};
+        for (int u = 0; u < nu; ++u) {
+            i4 cI[4], cJ[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                cI[x] = rowI[x] ? pV8[((gI0 + x) * KS64 + u0 + u) * 64 + lane] : i4{0, 0, 0, 0};
+                cJ[x] = rowJ[x] ? pV8[((gJ0 + x) * KS64 + u0 + u) * 64 + lane] : i4{0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y)
+                    if (rowI[x] && rowJ[y] && gI0 + x <= gJ0 + y)
+                        cnt[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cI[x], cJ[y], cnt[x][y], 0, 0, 0);
+        }
+        double* vpb = vpair + slab * b;
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                const int gI = gI0 + x, gJ = gJ0 + y;
+                if (!(rowI[x] && rowJ[y] && gI <= gJ)) continue;
+                const int col = 16 * gJ + (lane & 15);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int row = 16 * gI + 4 * (lane >> 4) + e;
+                    if (row >= P || col >= P || (gI == gJ && row > col)) continue;
+                    const double vp = static_cast<double>(cnt[x][y][e]);
+                    vpb[row + static_cast<std::int64_t>(P) * col] = vp;
+                    vpb[col + static_cast<std::int64_t>(P) * row] = vp;
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// The split contraction.
+// ---------------------------------------------------------------------------------
+__device__ inline bf8 lds_frag(const std::uint8_t* slot, int group, int plane, int nplanes, int lane) {
+    return *reinterpret_cast<const bf8*>(slot + (group * nplanes + plane) * kChunk + lane * 16);
+}
+
+__device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, int phase_planes,
+                                   int first_plane, std::int64_t t16, std::int64_t KS16, int gI0, int gJ0, int G32,
+                                   int wave, int lane) {
+    const int nchunks = kGroups * phase_planes;
+    for (int ci = wave; ci < nchunks; ci += kWaves) {
+        const int lg = ci / phase_planes, pp = ci - lg * phase_planes;
+        int gg = lg < 4 ? gI0 + lg : gJ0 + lg - 4;
+        if (gg >= G32) gg = 0;  // rows beyond P: any valid address; never read back
+        const std::uint8_t* src =
+            planes + ((static_cast<std::int64_t>(gg) * KS16 + t16) * kPlanes + first_plane + pp) * kChunk + lane * 16;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
+                                         (__attribute__((address_space(3))) void*)(slot + ci * kChunk), 16, 0, 0);
+    }
+}
+
+__global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __restrict__ planes, BlockTable bt,
+                                                       const F2Item* __restrict__ items, int n_items,
+                                                       unsigned* __restrict__ counter, int P, int G32,
+                                                       std::int64_t KS16, const double* __restrict__ vpair,
+                                                       double* __restrict__ f2) {
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds[kSlots * kSlotBytes];
+    __shared__ int s_item;
+    const int lane = threadIdx.x & 63;
+    // Wavefront-uniform values are moved to scalar registers so that every branch
+    // on them is uniform and the MFMA accumulators never sit in divergent flow.
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int wa = wave & 1;   // left sub-tile: groups 2wa, 2wa+1 of the left tile
+    const int wc = wave >> 1;  // right sub-tile: group wc of the right tile
+    const std::int64_t slab = static_cast<std::int64_t>(P) * P;
+
+    for (;;) {
+        if (threadIdx.x == 0) s_item = static_cast<int>(atomicAdd(counter, 1u));
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= n_items) break;
+        const F2Item wi = items[it];
+        const int b = __builtin_amdgcn_readfirstlane(wi.b);
+        const int gI0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(wi.ti));
+        const int gJ0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(wi.tj));
+        const std::int64_t t0 = bt.k0[b] / 16;
+        const int nsteps = __builtin_amdgcn_readfirstlane(bt.kpad[b] / 16);
+
+        bool act[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const int li = 2 * wa + x;
+            act[x] = (gI0 + li < G32) && (gJ0 + wc < G32) && (gI0 < gJ0 || li <= wc);
+        }
+        const bool any = act[0] || act[1];
+
+        f16v L[2][5], H[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+#pragma unroll
+            for (int t = 0; t < 5; ++t) L[x][t] = f16v{};
+            H[x] = f16v{};
+        }
+
+        // Prologue: the first K-step's two phases in flight (every block has
+        // nsteps >= 1 except empty blocks, whose loop body never runs).
+        if (nsteps > 0) {
+            stage_phase(planes, lds, kDPlanes, 0, t0, KS16, gI0, gJ0, G32, wave, lane);
+            stage_phase(planes, lds + kSlotBytes, kEPlanes, kDPlanes, t0, KS16, gI0, gJ0, G32, wave, lane);
+        }
+
+        for (int step = 0; step < nsteps; ++step) {
+            const bool more = step + 1 < nsteps;
+            // ---- phase 2*step: q digits.  Outstanding: this phase (5 loads) and the
+            // next w phase (6 loads), so vmcnt(6) means this phase has landed.
+            asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (more)
+                stage_phase(planes, lds + ((2 * step + 2) % kSlots) * kSlotBytes, kDPlanes, 0, t0 + step + 1, KS16,
+                            gI0, gJ0, G32, wave, lane);
+            if (any) {
+                const std::uint8_t* slot = lds + ((2 * step) % kSlots) * kSlotBytes;
+                bf8 dj[5];
+#pragma unroll
+                for (int k = 0; k < 5; ++k) dj[k] = lds_frag(slot, 4 + wc, k, kDPlanes, lane);
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    if (!act[x]) continue;
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) {
+                        const bf8 di = lds_frag(slot, 2 * wa + x, k, kDPlanes, lane);
+#pragma unroll
+                        for (int t = k; t < 5; ++t)
+                            L[x][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(di, dj[t - k], L[x][t], 0, 0, 0);
+                    }
+                }
+            }
+            // ---- phase 2*step+1: w digits and V.  Outstanding: this phase and, when
+            // issued, the next q phase (5 loads).
+            if (more) asm volatile("s_waitcnt vmcnt(5)" ::: "memory");
+            else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (more)
+                stage_phase(planes, lds + ((2 * step + 3) % kSlots) * kSlotBytes, kEPlanes, kDPlanes, t0 + step + 1,
+                            KS16, gI0, gJ0, G32, wave, lane);
+            if (any) {
+                const std::uint8_t* slot = lds + ((2 * step + 1) % kSlots) * kSlotBytes;
+                bf8 ej[5];
+#pragma unroll
+                for (int m = 0; m < 5; ++m) ej[m] = lds_frag(slot, 4 + wc, m, kEPlanes, lane);
+                const bf8 vj = lds_frag(slot, 4 + wc, 5, kEPlanes, lane);
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    if (!act[x]) continue;
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) {
+                        const bf8 ei = lds_frag(slot, 2 * wa + x, m, kEPlanes, lane);
+                        L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][m], 0, 0, 0);
+                    }
+                    const bf8 vi = lds_frag(slot, 2 * wa + x, 5, kEPlanes, lane);
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vi, ej[m], L[x][m], 0, 0, 0);
+                }
+                // Every 8 K-steps (128 SNPs) and at the block end: exact carries.
+                if ((step & 7) == 7 || !more) {
+#pragma unroll
+                    for (int x = 0; x < 2; ++x) {
+                        if (!act[x]) continue;
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) {
+#pragma unroll
+                            for (int t = 4; t >= 1; --t) {
+                                const float cr = rintf(L[x][t][e] * (1.0f / 256.0f));
+                                L[x][t][e] = fmaf(cr, -256.0f, L[x][t][e]);
+                                L[x][t - 1][e] += cr;
+                            }
+                            const float c0 = rintf(L[x][0][e] * (1.0f / 4096.0f)) * 4096.0f;
+                            L[x][0][e] -= c0;
+                            H[x][e] += c0;
+                        }
+                    }
+                }
+            }
+        }
+
+        // Epilogue: num = -2 sum_t U_t L_t in FP64; f2 = num / vpair.
+        double* f2b = f2 + slab * b;
+        const double* vpb = vpair + slab * b;
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            if (!act[x]) continue;
+            const int gI = gI0 + 2 * wa + x, gJ = gJ0 + wc;
+            const int col = 32 * gJ + (lane & 31);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int row = 32 * gI + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                if (row >= P || col >= P || (gI == gJ && row > col)) continue;
+                const double s = (static_cast<double>(H[x][e]) + static_cast<double>(L[x][0][e])) * 0x1p-14 +
+                                 static_cast<double>(L[x][1][e]) * 0x1p-22 +
+                                 static_cast<double>(L[x][2][e]) * 0x1p-30 +
+                                 static_cast<double>(L[x][3][e]) * 0x1p-38 +
+                                 static_cast<double>(L[x][4][e]) * 0x1p-46;
+                const double num = -2.0 * s;
+                const std::int64_t ij = row + static_cast<std::int64_t>(P) * col;
+                const double vp = vpb[ij];
+                const double val = vp > 0.0 ? num / vp : 0.0;
+                f2b[ij] = val;
+                f2b[col + static_cast<std::int64_t>(P) * row] = val;
+            }
+        }
+    }
+}
+}  // namespace
+
+void launch_prep_split_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P, int G32,
+                              int G16, BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks,
+                              std::int64_t KS16, std::uint8_t* planes, void* pV8, hipStream_t st) {
+    if (n_chunks <= 0) return;
+    prep_split_kernel<CountsSrc><<<static_cast<unsigned>(n_chunks), 256, 0, st>>>(
+        CountsSrc{a2, n, ld}, P, G32, G16, bt, chunk_block, KS16, planes, static_cast<i4*>(pV8));
+    STP_LAUNCH_CHECK();
+}
+
+void launch_prep_split_qvn(const double* q, const double* v, const double* n, int P, int G32, int G16,
+                           BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks, std::int64_t KS16,
+                           std::uint8_t* planes, void* pV8, hipStream_t st) {
+    if (n_chunks <= 0) return;
+    prep_split_kernel<QvnSrc><<<static_cast<unsigned>(n_chunks), 256, 0, st>>>(
+        QvnSrc{q, v, n, P}, P, G32, G16, bt, chunk_block, KS16, planes, static_cast<i4*>(pV8));
+    STP_LAUNCH_CHECK();
+}
+
+void launch_count(const void* pV8, BlockTable bt, const F2Item* items, int n_items, unsigned* counter, int P,
+                  int G16, std::int64_t KS64, double* vpair, int n_workgroups, hipStream_t st) {
+    if (n_items <= 0) return;
+    launch_zero_u32(counter, 1, st);
+    count_kernel<<<static_cast<unsigned>(n_workgroups), 256, 0, st>>>(static_cast<const i4*>(pV8), bt, items,
+                                                                      n_items, counter, P, G16, KS64, vpair);
+    STP_LAUNCH_CHECK();
+}
+
+void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, int n_items, unsigned* counter,
+                     int P, int G32, std::int64_t KS16, const double* vpair, double* f2, int n_workgroups,
+                     hipStream_t st) {
+    if (n_items <= 0) return;
+    launch_zero_u32(counter, 1, st);
+    f2_split_kernel<<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, n_items, counter, P, G32,
+                                                                        KS16, vpair, f2);
+    STP_LAUNCH_CHECK();
+}
+
+}  // namespace stp

@@ -1,0 +1,14 @@
+"""Readers for the reference's committed test fixtures (data only)."""
+import numpy as np
+
+
+def read_raw_fixture(path):
+    """Raw f2 fixture (tests/reference/test_qpadm_parity.cu:143-167):
+    int32 P, int32 n_block, int32[n_block] block_sizes, float64[P*P*n_block] at
+    i + P*j + P*P*b.  Returns (f2 as (P, P, nb) F-order, block_sizes)."""
+    raw = open(path, "rb").read()
+    P, nb = np.frombuffer(raw, np.int32, 2)
+    bs = np.frombuffer(raw, np.int32, nb, offset=8).copy()
+    off = 8 + 4 * int(nb)
+    f2 = np.frombuffer(raw, np.float64, int(P) * int(P) * int(nb), offset=off)
+    return f2.reshape((int(P), int(P), int(nb)), order="F").copy(), bs

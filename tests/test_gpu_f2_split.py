@@ -1,0 +1,89 @@
+"""GPU parity of the split-BF16 f2 form (fixed-point BF16 slices on BF16 MFMA).
+
+Bar: vpair bit-exact; f2 within 1e-9 relative of the FP64 oracle with an absolute
+floor of 1e-6 f2 units on |r| (|c - r| <= 1e-9 * max(|r|, 1e-6)) — the tolerance
+the split form is specified to; the measured error is printed and is typically
+orders of magnitude below it.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch  # noqa: F401  (keep the first, slow import out of test timeouts)
+
+from steppe_amd import _core, synth
+from test_gpu_decode_f2 import F2_FLOOR_TINY_BLOCKS, PANELS, _f2_qvn_gpu, _rel_err
+
+pytestmark = pytest.mark.gpu
+
+SPLIT_RTOL = 1e-9
+FLOOR = 1e-6
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", params=sorted(PANELS))
+def panel(request):
+    _say(f"[panel {request.param}]")
+    return request.param, PANELS[request.param]()
+
+
+def test_split_matches_oracle(engine, cref, panel):
+    name, p = panel
+    q, v, n = cref.decode_af(p, "sample")
+    _say(f"f2 split {name} P={p.n_pop} M={p.n_snp} nb={p.n_block}")
+    f2, vp, bs = _f2_qvn_gpu(engine, q, v, n, p.block_id, p.n_block, _core.STP_PREC_SPLIT_BF16)
+    fr, vr, br = cref.f2_blocks(q, v, n, p.block_id, p.n_block, n_threads=8)
+    assert np.array_equal(bs, br)
+    assert np.array_equal(vp, vr)
+    err = _rel_err(f2, fr)
+    _say(f"  split vs oracle: max rel err (floor {FLOOR}) = {err:.3e}")
+    assert err <= SPLIT_RTOL
+    assert np.array_equal(f2, f2.transpose(1, 0, 2))
+    f64, _, _ = _f2_qvn_gpu(engine, q, v, n, p.block_id, p.n_block, _core.STP_PREC_FP64)
+    _say(f"  split vs fp64 gpu: {_rel_err(f2, f64):.3e}")
+    assert _rel_err(f2, f64) <= SPLIT_RTOL
+
+
+@pytest.mark.parametrize("P", [1, 2, 31, 32, 33, 127, 128, 129, 260])
+def test_split_population_counts_edge_cases(engine, cref, P):
+    p = synth.scaled_config(P=P, M=900, seed=200 + P, block_lo=1, block_hi=300)
+    q, v, n = cref.decode_af(p, "sample")
+    f2, vp, bs = _f2_qvn_gpu(engine, q, v, n, p.block_id, p.n_block, _core.STP_PREC_SPLIT_BF16)
+    fr, vr, br = cref.f2_blocks(q, v, n, p.block_id, p.n_block)
+    assert np.array_equal(vp, vr) and np.array_equal(bs, br)
+    assert _rel_err(f2, fr, F2_FLOOR_TINY_BLOCKS) <= SPLIT_RTOL
+
+
+def test_split_long_blocks_and_extreme_frequencies(engine, cref):
+    # 8000-SNP blocks (the largest the workloads allow) with frequencies pinned at
+    # 0 and 1 in half the populations: the carry chain and the level bounds at
+    # their worst case.
+    rng = np.random.default_rng(5)
+    P, M = 40, 16000
+    q = rng.random((P, M))
+    q[: P // 2] = (q[: P // 2] > 0.5).astype(np.float64)
+    n = rng.integers(1, 120, (P, M)).astype(np.float64)
+    q = np.round(q * 2 * n) / (2 * n)          # frequencies of the form a2 / (2n)
+    v = (rng.random((P, M)) > 0.1).astype(np.float64)
+    q, n = q * v, n * v
+    bid = np.repeat(np.arange(2, dtype=np.int32), 8000)
+    f2, vp, bs = _f2_qvn_gpu(engine, q, v, n, bid, 2, _core.STP_PREC_SPLIT_BF16)
+    fr, vr, br = cref.f2_blocks(q, v, n, bid, 2, n_threads=8)
+    assert np.array_equal(vp, vr)
+    err = _rel_err(f2, fr)
+    _say(f"  extreme: max rel err = {err:.3e}")
+    assert err <= SPLIT_RTOL
+
+
+def test_split_extract_end_to_end(engine, cref):
+    p = synth.config1()
+    h = _core.run_extract_f2(p.packed, p.n_snp, p.pop_offsets, p.block_id, p.n_block,
+                             pops=p.pop_names, ploidy="auto", precision="split_bf16")
+    q, v, n = cref.decode_af(p, "detect")
+    fr, vr, br = cref.f2_blocks(q, v, n, p.block_id, p.n_block)
+    assert np.array_equal(h._vpair_numpy(), vr)
+    assert _rel_err(h._f2_numpy(), fr) <= SPLIT_RTOL
