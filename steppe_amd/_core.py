@@ -49,6 +49,57 @@ class Tile(C.Structure):
     ]
 
 
+QPADM_MAX_NL = 8
+QPADM_MAX_RANK = QPADM_MAX_NL
+QPADM_MAX_POPDROP = QPADM_MAX_NL + 1
+STATUS_NAMES = {0: "ok", 1: "device_oom", 2: "rank_deficient", 3: "non_spd_covariance",
+                4: "chisq_undefined", 5: "invalid_config"}
+
+
+class QpAdmResult(C.Structure):
+    """stp_qpadm_result (include/stp_qpadm_result.h)."""
+    _fields_ = [
+        ("status", C.c_int32), ("model_index", C.c_int32), ("nl", C.c_int32), ("dof", C.c_int32),
+        ("est_rank", C.c_int32), ("f4rank", C.c_int32), ("n_rank", C.c_int32), ("n_popdrop", C.c_int32),
+        ("p", C.c_double), ("chisq", C.c_double),
+        ("weight", C.c_double * QPADM_MAX_NL), ("se", C.c_double * QPADM_MAX_NL),
+        ("z", C.c_double * QPADM_MAX_NL),
+        ("rank_chisq", C.c_double * QPADM_MAX_RANK), ("rank_dof", C.c_int32 * QPADM_MAX_RANK),
+        ("rankdrop_f4rank", C.c_int32 * QPADM_MAX_RANK), ("rankdrop_dof", C.c_int32 * QPADM_MAX_RANK),
+        ("rankdrop_dofdiff", C.c_int32 * QPADM_MAX_RANK),
+        ("rankdrop_chisq", C.c_double * QPADM_MAX_RANK), ("rankdrop_p", C.c_double * QPADM_MAX_RANK),
+        ("rankdrop_chisqdiff", C.c_double * QPADM_MAX_RANK),
+        ("rankdrop_p_nested", C.c_double * QPADM_MAX_RANK),
+        ("popdrop_pat", (C.c_char * (QPADM_MAX_NL + 1)) * QPADM_MAX_POPDROP),
+        ("popdrop_wt", C.c_int32 * QPADM_MAX_POPDROP), ("popdrop_dof", C.c_int32 * QPADM_MAX_POPDROP),
+        ("popdrop_f4rank", C.c_int32 * QPADM_MAX_POPDROP),
+        ("popdrop_feasible", C.c_int32 * QPADM_MAX_POPDROP),
+        ("popdrop_chisq", C.c_double * QPADM_MAX_POPDROP), ("popdrop_p", C.c_double * QPADM_MAX_POPDROP),
+    ]
+
+    def to_dict(self, precision: str = "fp64") -> dict:
+        """The reference's result dict (bindings/internal/bind_common.hpp:179-212)."""
+        nl, nk, npd = self.nl, self.n_rank, self.n_popdrop
+        d = {
+            "weight": list(self.weight[:nl]), "se": list(self.se[:nl]), "z": list(self.z[:nl]),
+            "p": self.p, "chisq": self.chisq, "dof": self.dof, "est_rank": self.est_rank,
+            "f4rank": self.f4rank,
+            "rankdrop_f4rank": list(self.rankdrop_f4rank[:nk]), "rankdrop_dof": list(self.rankdrop_dof[:nk]),
+            "rankdrop_dofdiff": list(self.rankdrop_dofdiff[:nk]),
+            "rankdrop_chisq": list(self.rankdrop_chisq[:nk]), "rankdrop_p": list(self.rankdrop_p[:nk]),
+            "rankdrop_chisqdiff": list(self.rankdrop_chisqdiff[:nk]),
+            "rankdrop_p_nested": list(self.rankdrop_p_nested[:nk]),
+            "popdrop_pat": [self.popdrop_pat[k].value.decode() for k in range(npd)],
+            "popdrop_wt": list(self.popdrop_wt[:npd]), "popdrop_dof": list(self.popdrop_dof[:npd]),
+            "popdrop_f4rank": list(self.popdrop_f4rank[:npd]),
+            "popdrop_chisq": list(self.popdrop_chisq[:npd]), "popdrop_p": list(self.popdrop_p[:npd]),
+            "popdrop_feasible": [bool(v) for v in self.popdrop_feasible[:npd]],
+            "status": STATUS_NAMES.get(self.status, str(self.status)), "precision": precision,
+            "model_index": self.model_index,
+        }
+        return d
+
+
 _lib = None
 
 

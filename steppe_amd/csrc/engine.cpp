@@ -320,9 +320,9 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
         mark_end(kF2);
         return;
     }
-    // Split-BF16: 11 digit planes of 2 bytes per (row, SNP), rows padded to 32.
+    // Split-BF16: 13 digit planes of 2 bytes per (row, SNP), rows padded to 32.
     const std::int64_t KS16 = plan.K_total / 16;
-    const std::size_t split_bytes = static_cast<std::size_t>(plan.G32) * 32 * plan.K_total * 22;
+    const std::size_t split_bytes = static_cast<std::size_t>(plan.G32) * 32 * plan.K_total * 26;
     planeSplit_.ensure(std::max<std::size_t>(split_bytes, 16));
     mark_begin(kPrep);
     if (src.a2)

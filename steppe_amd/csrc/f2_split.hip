@@ -11,30 +11,33 @@
 //   num_ij = sum_s v_i v_j [(q_i - q_j)^2 - h_i - h_j],  f2 = num / vpair
 //
 // Operands.  With the per-SNP centre c_s, q' = q - c (|q'| < 1) and w = -(q'^2 - h)/2
-// (|w| <= 1/2), each is written as five balanced 8-bit digits (|digit| <= 128,
-// exact in BF16):
-//     q~ = 2^-7 sum_k d_k 2^-8k,     w~ = 2^-7 sum_m c_m 2^-8m,     k, m = 0..4
-// w is computed from q~ (not q'), so the device evaluates the estimator exactly for
-// the perturbed frequencies q~ (|q~ - q'| <= 2^-40).  Then
+// (|w| <= 1/2), each is written as six balanced 8-bit digits (|digit| <= 128, exact
+// in BF16):
+//     q~ = 2^-7 sum_k d_k 2^-8k,     w~ = 2^-7 sum_m c_m 2^-8m,     k, m = 0..5
+// w is computed from q~ (not q'), so the device evaluates the estimator for the
+// perturbed frequencies q~ (|q~ - q'| <= 2^-48).  Then
 //     -num_ij / 2 = sum_s [ q~_i q~_j + w~_i v_j + v_i w~_j ]
-//                 = sum_t U_t * L_t,   U_t = 2^(-14-8t),  t = 0..4
+//                 = sum_t U_t * L_t,   U_t = 2^(-14-8t)
 //     L_t = sum_s [ sum_{k+l=t} d_k(i) d_l(j) + e_t(i) v(j) + v(i) e_t(j) ],  e_t = 2^7 c_t
-// Products with k + l >= 5 are dropped (<= 2^-38 per SNP).  Per SNP, |L_t| grows by
-// at most (t+3) 2^14, so 128 SNPs keep every L_t below 2^24: the FP32 MFMA sums are
-// exact.  Every 128 SNPs each level carries its multiple of 2^8 into the level
-// above (exact FP32 operations) and L_0 carries multiples of 2^12 into an FP32
-// high word, so a block of any length accumulates exactly in six FP32 words per
-// output and FP64 appears only in the per-block epilogue.
+// Levels t = 0..4 each have an FP32 accumulator.  Per SNP, |L_t| grows by at most
+// (t+3) 2^14, so 128 SNPs keep every L_t below 2^24 and the MFMA sums are exact.
+// Level 5 (d_k d_{5-k} for k = 0..5, and c_5) is folded into L_4 through operands
+// scaled by 2^-8 (exact in BF16): those sums are rounded at 2^-47 per MFMA, far
+// below the result's precision.  Products with k + l >= 6 are dropped
+// (<= 2^-45 per SNP).  Every 128 SNPs each level carries its multiple of 2^8 into
+// the level above (exact FP32 operations) and L_0 carries multiples of 2^12 into an
+// FP32 high word, so a block of any length accumulates in six FP32 words per output
+// and FP64 appears only in the per-block epilogue.
 //
-// MFMA work per (pair, SNP): 25 BF16 products (15 digit-by-digit, 10 digit-by-V),
+// MFMA work per (pair, SNP): 33 BF16 products (21 digit-by-digit, 12 digit-by-V),
 // plus the pair count V V^T, computed exactly by a separate int8 MFMA kernel.
 //
 // Kernel shape: persistent workgroups of 8 wavefronts (2 per SIMD) own a 128x128
 // output tile of one jackknife block; each wavefront owns 64x32 (two 32x32 MFMA
-// tiles, six FP32 accumulators per output).  Each 16-SNP K-step is staged
-// through LDS in two phases (the 5 q-digit
-// planes, then the 5 w-digit planes + V) by direct global->LDS loads into a
-// three-slot ring, so two phases of loads are in flight while one is computed.
+// tiles, six FP32 accumulators per output).  Each 16-SNP K-step is staged through
+// LDS in three phases (the 6 q-digit planes; e0..e2 + V; e3..e5 + V) by direct
+// global->LDS loads into a three-slot ring, so two phases of loads are in flight
+// while one is computed.
 
 #include "common.hpp"
 #include "f2_layout.hpp"
@@ -49,20 +52,22 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 typedef int i4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-constexpr int kPlanes = 11;            // d0..d4, e0..e4, V
+constexpr int kPlanes = 13;            // d0..d5 (0-5), e0..e5 (6-11), V (12)
+constexpr int kVPlane = 12;
 constexpr int kChunk = 1024;           // bytes per (32-row group, 16-SNP step, plane)
 constexpr int kGroups = 8;             // 32-row groups per workgroup tile: 4 left + 4 right
 constexpr int kSlotBytes = kGroups * 6 * kChunk;   // 48 KB (largest phase: 6 planes)
 constexpr int kSlots = 3;
-constexpr int kDPlanes = 5, kEPlanes = 6;
+constexpr int kDPlanes = 6, kEPlanes = 4;  // planes per phase: q digits; 3 w digits + V
 constexpr int kWaves = 8;
 constexpr double kCentreScale = 256.0;
 
 // ---------------------------------------------------------------------------------
-// prep: frequencies -> 11 BF16 digit planes (32x32x16 MFMA fragment order) + the
+// prep: frequencies -> 13 BF16 digit planes (32x32x16 MFMA fragment order) + the
 // int8 V plane (16x16x64 fragment order, for the count kernel).
 //   plane chunk (g32, t16, p) holds rows 32 g32 + lane%32, SNPs 16 t16 + 8 (lane/32) + j
-//   at byte offset ((g32 * KS16 + t16) * 11 + p) * 1024 + lane * 16 + 2 j.
+//   at byte offset ((g32 * KS16 + t16) * 13 + p) * 1024 + lane * 16 + 2 j.
+//   Planes 5 (d5) and 11 (e5) are stored scaled by 2^-8 (they only enter level 5).
 // ---------------------------------------------------------------------------------
 struct CountsSrc {
     const std::uint16_t* a2;
@@ -93,7 +98,8 @@ struct QvnSrc {
 };
 
 __device__ inline unsigned bf16_bits(double integer_digit) {
-    // The digit is an integer of at most 8 significant bits: float and bf16 exact.
+    // The digit has at most 8 significant bits (an integer, possibly times a power
+    // of two): the float and bf16 conversions are exact.
     return __float_as_uint(static_cast<float>(integer_digit)) >> 16;
 }
 
@@ -150,26 +156,29 @@ __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32
                 const double nm1 = nd - 1.0;
                 const double h = q * (1.0 - q) / (nm1 > 1.0 ? nm1 : 1.0);
                 double x = (q - centre[kk]) * 128.0;
-                double d[5];
+                double d[6];
 #pragma unroll
-                for (int k = 0; k < 5; ++k) {
+                for (int k = 0; k < 6; ++k) {
                     d[k] = rint(x);
                     x = (x - d[k]) * 256.0;
                 }
-                const double qt = (d[0] + (d[1] + (d[2] + (d[3] + d[4] / 256.0) / 256.0) / 256.0) / 256.0) / 128.0;
+                const double qt =
+                    (d[0] + (d[1] + (d[2] + (d[3] + (d[4] + d[5] / 256.0) / 256.0) / 256.0) / 256.0) / 256.0) / 128.0;
                 double y = -0.5 * (qt * qt - h) * 128.0;
-                double e[5];
+                double e[6];
 #pragma unroll
-                for (int m = 0; m < 5; ++m) {
+                for (int m = 0; m < 6; ++m) {
                     e[m] = rint(y);
                     y = (y - e[m]) * 256.0;
                 }
                 const int sh = 16 * (j & 1);
 #pragma unroll
                 for (int k = 0; k < 5; ++k) w[k][j >> 1] |= bf16_bits(d[k]) << sh;
+                w[5][j >> 1] |= bf16_bits(d[5] / 256.0) << sh;
 #pragma unroll
-                for (int m = 0; m < 5; ++m) w[5 + m][j >> 1] |= bf16_bits(e[m] * 128.0) << sh;
-                w[10][j >> 1] |= bf16_bits(1.0) << sh;
+                for (int m = 0; m < 5; ++m) w[6 + m][j >> 1] |= bf16_bits(e[m] * 128.0) << sh;
+                w[11][j >> 1] |= bf16_bits(e[5] * 0.5) << sh;
+                w[kVPlane][j >> 1] |= bf16_bits(1.0) << sh;
             }
         }
         std::uint8_t* base = planes + ((static_cast<std::int64_t>(g) * KS16 + t16) * kPlanes) * kChunk + lane * 16;
@@ -271,16 +280,40 @@ __device__ inline bf8 lds_frag(const std::uint8_t* slot, int group, int plane, i
     return *reinterpret_cast<const bf8*>(slot + (group * nplanes + plane) * kChunk + lane * 16);
 }
 
-__device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, int phase_planes,
-                                   int first_plane, std::int64_t t16, std::int64_t KS16, int gI0, int gJ0, int G32,
-                                   int wave, int lane) {
-    const int nchunks = kGroups * phase_planes;
+// Multiply 8 BF16 values by 2^-8 (exact: the values are small integers or zero).
+__device__ inline bf8 scale_2m8(bf8 a) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    u4 x = __builtin_bit_cast(u4, a);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float lo = __uint_as_float(x[i] << 16) * 0.00390625f;
+        const float hi = __uint_as_float(x[i] & 0xFFFF0000u) * 0.00390625f;
+        x[i] = (__float_as_uint(lo) >> 16) | (__float_as_uint(hi) & 0xFFFF0000u);
+    }
+    return __builtin_bit_cast(bf8, x);
+}
+
+// Phase kinds of a K-step: 0 = q digits (planes 0-5), 1 = e0..e2 + V, 2 = e3..e5 + V.
+__device__ inline int phase_plane(int kind, int pp) {
+    if (kind == 0) return pp;
+    if (pp == 3) return kVPlane;
+    return (kind == 1 ? 6 : 9) + pp;
+}
+
+// Issue this wavefront's share of one phase's 1 KB chunks (direct global->LDS).
+// Every wavefront issues exactly (8 groups * planes) / 8 loads per phase.
+__device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, int kind,
+                                   std::int64_t t16, std::int64_t KS16, int gI0, int gJ0, int G32, int wave,
+                                   int lane) {
+    const int np = kind == 0 ? kDPlanes : kEPlanes;
+    const int nchunks = kGroups * np;
     for (int ci = wave; ci < nchunks; ci += kWaves) {
-        const int lg = ci / phase_planes, pp = ci - lg * phase_planes;
+        const int lg = ci / np, pp = ci - lg * np;
         int gg = lg < 4 ? gI0 + lg : gJ0 + lg - 4;
         if (gg >= G32) gg = 0;  // rows beyond P: any valid address; never read back
         const std::uint8_t* src =
-            planes + ((static_cast<std::int64_t>(gg) * KS16 + t16) * kPlanes + first_plane + pp) * kChunk + lane * 16;
+            planes + ((static_cast<std::int64_t>(gg) * KS16 + t16) * kPlanes + phase_plane(kind, pp)) * kChunk +
+            lane * 16;
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
                                          (__attribute__((address_space(3))) void*)(slot + ci * kChunk), 16, 0, 0);
     }
@@ -300,6 +333,9 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
     const int wa = wave & 1;   // left sub-tile: groups 2wa, 2wa+1 of the left tile
     const int wc = wave >> 1;  // right sub-tile: group wc of the right tile
     const std::int64_t slab = static_cast<std::int64_t>(P) * P;
+    std::uint8_t* const slot_d = lds;
+    std::uint8_t* const slot_e1 = lds + kSlotBytes;
+    std::uint8_t* const slot_e2 = lds + 2 * kSlotBytes;
 
     for (;;) {
         if (threadIdx.x == 0) s_item = static_cast<int>(atomicAdd(counter, 1u));
@@ -330,67 +366,90 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
             H[x] = f16v{};
         }
 
-        // Prologue: the first K-step's two phases in flight (every block has
-        // nsteps >= 1 except empty blocks, whose loop body never runs).
+        // Prologue: the first K-step's first two phases in flight (the slot of a
+        // phase is its kind, since a K-step has three phases and the ring three slots).
         if (nsteps > 0) {
-            stage_phase(planes, lds, kDPlanes, 0, t0, KS16, gI0, gJ0, G32, wave, lane);
-            stage_phase(planes, lds + kSlotBytes, kEPlanes, kDPlanes, t0, KS16, gI0, gJ0, G32, wave, lane);
+            stage_phase(planes, slot_d, 0, t0, KS16, gI0, gJ0, G32, wave, lane);
+            stage_phase(planes, slot_e1, 1, t0, KS16, gI0, gJ0, G32, wave, lane);
         }
 
         for (int step = 0; step < nsteps; ++step) {
             const bool more = step + 1 < nsteps;
-            // ---- phase 2*step: q digits.  Outstanding: this phase (5 loads) and the
-            // next w phase (6 loads), so vmcnt(6) means this phase has landed.
-            asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+            // ---- q digits.  Outstanding: this phase (6 loads), the e1 phase (4).
+            asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            if (more)
-                stage_phase(planes, lds + ((2 * step + 2) % kSlots) * kSlotBytes, kDPlanes, 0, t0 + step + 1, KS16,
-                            gI0, gJ0, G32, wave, lane);
+            stage_phase(planes, slot_e2, 2, t0 + step, KS16, gI0, gJ0, G32, wave, lane);
             if (any) {
-                const std::uint8_t* slot = lds + ((2 * step) % kSlots) * kSlotBytes;
-                bf8 dj[5];
+                bf8 dj[6];
 #pragma unroll
-                for (int k = 0; k < 5; ++k) dj[k] = lds_frag(slot, 4 + wc, k, kDPlanes, lane);
+                for (int k = 0; k < 6; ++k) dj[k] = lds_frag(slot_d, 4 + wc, k, kDPlanes, lane);
 #pragma unroll
                 for (int x = 0; x < 2; ++x) {
                     if (!act[x]) continue;
 #pragma unroll
-                    for (int k = 0; k < 5; ++k) {
-                        const bf8 di = lds_frag(slot, 2 * wa + x, k, kDPlanes, lane);
+                    for (int k = 0; k < 6; ++k) {
+                        const bf8 di = lds_frag(slot_d, 2 * wa + x, k, kDPlanes, lane);
 #pragma unroll
                         for (int t = k; t < 5; ++t)
                             L[x][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(di, dj[t - k], L[x][t], 0, 0, 0);
+                        // Level 5 into L4: d5 is stored pre-scaled; d1..d4 are scaled here.
+                        const bf8 d5l = (k == 0 || k == 5) ? di : scale_2m8(di);
+                        L[x][4] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(d5l, dj[5 - k], L[x][4], 0, 0, 0);
                     }
                 }
             }
-            // ---- phase 2*step+1: w digits and V.  Outstanding: this phase and, when
-            // issued, the next q phase (5 loads).
-            if (more) asm volatile("s_waitcnt vmcnt(5)" ::: "memory");
-            else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            // ---- e0..e2 + V.  Outstanding: this phase (4), the e2 phase (4).
+            asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            if (more)
-                stage_phase(planes, lds + ((2 * step + 3) % kSlots) * kSlotBytes, kEPlanes, kDPlanes, t0 + step + 1,
-                            KS16, gI0, gJ0, G32, wave, lane);
+            if (more) stage_phase(planes, slot_d, 0, t0 + step + 1, KS16, gI0, gJ0, G32, wave, lane);
             if (any) {
-                const std::uint8_t* slot = lds + ((2 * step + 1) % kSlots) * kSlotBytes;
-                bf8 ej[5];
+                bf8 ej[3];
 #pragma unroll
-                for (int m = 0; m < 5; ++m) ej[m] = lds_frag(slot, 4 + wc, m, kEPlanes, lane);
-                const bf8 vj = lds_frag(slot, 4 + wc, 5, kEPlanes, lane);
+                for (int m = 0; m < 3; ++m) ej[m] = lds_frag(slot_e1, 4 + wc, m, kEPlanes, lane);
+                const bf8 vj = lds_frag(slot_e1, 4 + wc, 3, kEPlanes, lane);
 #pragma unroll
                 for (int x = 0; x < 2; ++x) {
                     if (!act[x]) continue;
 #pragma unroll
-                    for (int m = 0; m < 5; ++m) {
-                        const bf8 ei = lds_frag(slot, 2 * wa + x, m, kEPlanes, lane);
+                    for (int m = 0; m < 3; ++m) {
+                        const bf8 ei = lds_frag(slot_e1, 2 * wa + x, m, kEPlanes, lane);
                         L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][m], 0, 0, 0);
                     }
-                    const bf8 vi = lds_frag(slot, 2 * wa + x, 5, kEPlanes, lane);
+                    const bf8 vi = lds_frag(slot_e1, 2 * wa + x, 3, kEPlanes, lane);
 #pragma unroll
-                    for (int m = 0; m < 5; ++m)
+                    for (int m = 0; m < 3; ++m)
                         L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vi, ej[m], L[x][m], 0, 0, 0);
+                }
+            }
+            // ---- e3..e5 + V (e5 pre-scaled into level 4).  Outstanding: this phase
+            // and, when issued, the next q phase (6).
+            if (more) asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+            else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (more) stage_phase(planes, slot_e1, 1, t0 + step + 1, KS16, gI0, gJ0, G32, wave, lane);
+            if (any) {
+                bf8 ej[3];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) ej[m] = lds_frag(slot_e2, 4 + wc, m, kEPlanes, lane);
+                const bf8 vj = lds_frag(slot_e2, 4 + wc, 3, kEPlanes, lane);
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    if (!act[x]) continue;
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        const bf8 ei = lds_frag(slot_e2, 2 * wa + x, m, kEPlanes, lane);
+                        const int t = m == 2 ? 4 : 3 + m;
+                        L[x][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][t], 0, 0, 0);
+                    }
+                    const bf8 vi = lds_frag(slot_e2, 2 * wa + x, 3, kEPlanes, lane);
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        const int t = m == 2 ? 4 : 3 + m;
+                        L[x][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vi, ej[m], L[x][t], 0, 0, 0);
+                    }
                 }
                 // Every 8 K-steps (128 SNPs) and at the block end: exact carries.
                 if ((step & 7) == 7 || !more) {
@@ -441,6 +500,7 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
         }
     }
 }
+
 }  // namespace
 
 void launch_prep_split_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P, int G32,

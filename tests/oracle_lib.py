@@ -123,6 +123,30 @@ class _Lib:
                 bs)
 
 
+    # ---- qpAdm ----------------------------------------------------------------
+    def qpadm(self, f2, vpair, block_sizes, target, left, right, fudge=1e-4, als_iterations=20,
+              rank=-1, rank_alpha=0.05, jackknife=2):
+        """One model on a (P, P, nb) tensor; returns the reference's result dict."""
+        from steppe_amd._core import QpAdmResult
+        f2 = np.asarray(f2, np.float64)
+        P, _, nb = f2.shape
+        fl = np.asfortranarray(f2).ravel(order="F")
+        vp = None if vpair is None else np.asfortranarray(np.asarray(vpair, np.float64)).ravel(order="F")
+        bs = np.ascontiguousarray(block_sizes, np.int32)
+        le = np.ascontiguousarray(left, np.int32)
+        ri = np.ascontiguousarray(right, np.int32)
+        out = QpAdmResult()
+        f = self.fn("qpadm")
+        f.restype = C.c_int
+        rc = f(fl.ctypes.data_as(C.c_void_p), None if vp is None else vp.ctypes.data_as(C.c_void_p),
+               bs.ctypes.data_as(C.c_void_p), C.c_int(P), C.c_int(nb), C.c_int(int(target)),
+               le.ctypes.data_as(C.c_void_p), C.c_int(len(le)), ri.ctypes.data_as(C.c_void_p),
+               C.c_int(len(ri)), C.c_double(fudge), C.c_int(als_iterations), C.c_int(rank),
+               C.c_double(rank_alpha), C.c_int(jackknife), C.byref(out))
+        assert rc == 0
+        return out.to_dict()
+
+
 _cache = {}
 
 
