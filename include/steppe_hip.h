@@ -130,6 +130,38 @@ const double* stp_f2_handle_vpair_device(const stp_f2_handle* h);
 int stp_f2_handle_download(const stp_f2_handle* h, double* f2, double* vpair);
 int stp_f2_handle_free(stp_f2_handle* h);
 
+/* ---- qpAdm on a resident f2 handle ------------------------------------------------ */
+
+/* Fit options (reference: steppe::QpAdmOptions, include/steppe/qpadm.hpp:35-44). */
+#define STP_JACKKNIFE_NONE 0
+#define STP_JACKKNIFE_FEASIBLE 1  /* SEs only when every weight is in [0, 1] */
+#define STP_JACKKNIFE_ALL 2
+typedef struct stp_qpadm_options {
+    double fudge;            /* ridge factor on tr(Q) and on the GLS normal matrices (1e-4) */
+    int32_t als_iterations;  /* fixed ALS iterations per fit (20) */
+    int32_t rank;            /* fitted rank; -1 = number of sources - 1 */
+    double rank_alpha;       /* rank-test significance level (0.05) */
+    int32_t jackknife;       /* STP_JACKKNIFE_*: when leave-one-block-out SEs are computed */
+    int32_t se_require_p;    /* with STP_JACKKNIFE_FEASIBLE: also require p >= p_se_threshold */
+    double p_se_threshold;   /* (0.05) */
+} stp_qpadm_options;
+
+/* Batched qpAdm on the device-resident tensor (reference: run_qpadm_search and
+ * ComputeBackend::fit_models_batched, src/core/qpadm/model_search.cpp:30-218,
+ * src/device/backend.hpp:1636-1647; a single model is run_qpadm,
+ * src/core/qpadm/qpadm_fit.cpp:205-226).  Model i has target[i], sources
+ * left[left_offsets[i] .. left_offsets[i+1]) and right set
+ * right[right_offsets[i] .. right_offsets[i+1]) (right[...][0] is the base outgroup),
+ * all population indices of the handle.  Results go to out[i] (host array of
+ * n_models records), in input order, with out[i].model_index = i.  Limits: at most
+ * STP_QPADM_MAX_NL sources, at least as many right populations beyond the base as
+ * sources, (sources x (right - 1)) <= 64 and right - 1 <= 32; violations return
+ * STP_ERR_INVALID.  Per-model domain outcomes (rank deficient, non-SPD covariance,
+ * undefined chi-square) are reported in out[i].status, as in the reference. */
+int stp_qpadm_batch(stp_context* ctx, const stp_f2_handle* f2, int32_t n_models, const int32_t* target,
+                    const int32_t* left_offsets, const int32_t* left, const int32_t* right_offsets,
+                    const int32_t* right, const stp_qpadm_options* options, stp_qpadm_result* out);
+
 /* ---- STPF2BK1 f2.bin (reference: src/device/f2_disk_format.hpp:17-50,
  *      src/app/f2_dir_writer.cpp:306-440, src/app/f2_dir_io.cpp:52-158) ------------- */
 
@@ -143,9 +175,9 @@ int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_s
 /* Enable per-stage device timing: HIP events recorded on the context's own stream
  * around the decode, prep and f2 contraction launches. */
 int stpx_timing(stp_context* ctx, int32_t enable);
-/* Accumulated milliseconds and launch counts per stage [decode, prep, f2, count]
- * (f2 = the contraction kernel, count = the pair-count kernel); reset
- * clears them. */
+/* Accumulated milliseconds and launch counts per stage [decode, prep, f2, count, qpadm]
+ * (f2 = the contraction kernel, count = the pair-count kernel, qpadm = the batched
+ * fit kernels of one stp_qpadm_batch call); reset clears them. */
 int stpx_timing_read(stp_context* ctx, double* ms, int64_t* count, int32_t reset);
 
 /* ---- synthetic panels (bench/test input; no counterpart in the reference) ------------ */

@@ -181,14 +181,14 @@ class Context:
             cls._by_device[int(device)] = ctx
         return ctx
 
-    STAGES = ("decode", "prep", "f2", "count")
+    STAGES = ("decode", "prep", "f2", "count", "qpadm")
 
     def timing(self, enable: bool = True):
         check(lib().stpx_timing(self.h, 1 if enable else 0))
 
     def read_timing(self, reset: bool = False) -> dict:
-        ms = np.zeros(4, np.float64)
-        cnt = np.zeros(4, np.int64)
+        ms = np.zeros(len(self.STAGES), np.float64)
+        cnt = np.zeros(len(self.STAGES), np.int64)
         check(lib().stpx_timing_read(self.h, _ptr(ms), _ptr(cnt), 1 if reset else 0))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.STAGES)}
 

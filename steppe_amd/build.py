@@ -22,7 +22,11 @@ OBJDIR = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(LIBDIR, "libsteppe_hip.so")
 
 ARCH = os.environ.get("STEPPE_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["decode.hip", "f2_fp64.hip", "f2_split.hip", "synth.hip", "engine.cpp", "capi.cpp"]
+SOURCES = ["decode.hip", "f2_fp64.hip", "f2_split.hip", "synth.hip", "qpadm.hip", "engine.cpp", "qpadm_host.cpp",
+           "capi.cpp"]
+# Built without FP contraction: each product is rounded before it is added, as in the
+# reference's fit chain (plain double sums in a fixed order).
+NO_CONTRACT = {"qpadm.hip"}
 
 
 def hipcc() -> str:
@@ -41,6 +45,8 @@ def common_flags() -> list:
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
     cmd = [hipcc(), *common_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+    if src in NO_CONTRACT:
+        cmd.insert(-4, "-ffp-contract=off")
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -241,6 +241,18 @@ int stp_f2_handle_free(stp_f2_handle* h) {
     });
 }
 
+int stp_qpadm_batch(stp_context* ctx, const stp_f2_handle* f2, int32_t n_models, const int32_t* target,
+                    const int32_t* left_offsets, const int32_t* left, const int32_t* right_offsets,
+                    const int32_t* right, const stp_qpadm_options* options, stp_qpadm_result* out) {
+    return guard([&] {
+        STP_REQUIRE(f2 && f2->blocks, "stp_qpadm_batch: null f2 handle");
+        STP_REQUIRE(options, "stp_qpadm_batch: null options");
+        STP_REQUIRE(f2->ctx == ctx, "stp_qpadm_batch: the f2 handle belongs to another context");
+        backend(ctx).qpadm_batch(*f2->blocks, n_models, target, left_offsets, left, right_offsets, right, *options,
+                                 out);
+    });
+}
+
 int stp_f2bin_write(const char* path, const double* f2, const double* vpair, const int32_t* block_sizes,
                     int32_t P, int32_t n_block) {
     return guard([&] {

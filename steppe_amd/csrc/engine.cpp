@@ -65,24 +65,38 @@ F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_b
     for (int b = 0; b < n_block; ++b)
         for (std::int64_t c = pl.k0[b] / kKPad; c < (pl.k0[b] + pl.kpad[b]) / kKPad; ++c)
             pl.chunk_block[static_cast<std::size_t>(c)] = b;
-    // Work items: block-major (all tile pairs of a block adjacent, so concurrent
-    // wavefronts share the block's operand slab in cache), blocks by decreasing size
-    // (longest-processing-time first balances the persistent queue's tail).
+    // Work items.  Every block goes to one die's queue (greedy longest-first onto the
+    // least-loaded die, so the dies finish together), and each queue is block-major:
+    // all tile pairs of a block sit together, so the workgroups of one die run a
+    // block's tiles concurrently and share its operand panels in that die's L2.
     std::vector<int> order(n_block);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pl.len[a] > pl.len[b]; });
-    pl.items.reserve(static_cast<std::size_t>(n_block) * pl.T * (pl.T + 1) / 2);
-    for (int b : order)
-        for (int ti = 0; ti < pl.T; ++ti)
-            for (int tj = ti; tj < pl.T; ++tj)
-                pl.items.push_back(F2Item{b, static_cast<std::int16_t>(ti), static_cast<std::int16_t>(tj)});
+    std::vector<std::vector<int>> die_blocks(kXcds);
+    {
+        std::vector<std::int64_t> load(kXcds, 0);
+        for (int b : order) {
+            const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            load[x] += pl.kpad[b];
+            die_blocks[x].push_back(b);
+        }
+    }
+    auto build = [&](int T, std::vector<F2Item>& items, ItemQueues& q) {
+        items.clear();
+        items.reserve(static_cast<std::size_t>(n_block) * T * (T + 1) / 2);
+        for (int x = 0; x < kXcds; ++x) {
+            q.off[x] = static_cast<std::int32_t>(items.size());
+            for (int b : die_blocks[x])
+                for (int ti = 0; ti < T; ++ti)
+                    for (int tj = ti; tj < T; ++tj)
+                        items.push_back(F2Item{b, static_cast<std::int16_t>(ti), static_cast<std::int16_t>(tj)});
+        }
+        q.off[kXcds] = static_cast<std::int32_t>(items.size());
+    };
+    build(pl.T, pl.items, pl.q64);
     pl.G32 = static_cast<int>(ceil_div(P, 32));
     pl.T128 = static_cast<int>(ceil_div(pl.G32, 4));
-    pl.items128.reserve(static_cast<std::size_t>(n_block) * pl.T128 * (pl.T128 + 1) / 2);
-    for (int b : order)
-        for (int ti = 0; ti < pl.T128; ++ti)
-            for (int tj = ti; tj < pl.T128; ++tj)
-                pl.items128.push_back(F2Item{b, static_cast<std::int16_t>(ti), static_cast<std::int16_t>(tj)});
+    build(pl.T128, pl.items128, pl.q128);
     return pl;
 }
 
@@ -299,7 +313,10 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
     const std::int64_t n_chunks = plan.K_total / kKPad;
     const BlockTable bt = plan_dev_.table();
     const int n_items = static_cast<int>(plan.items.size());
-    const int n_wg = std::max(1, std::min<int>(2 * n_cu_, static_cast<int>(ceil_div(n_items, 4))));
+    // Persistent grids are whole multiples of the die count so every die queue has
+    // its own workers.
+    const int n_wg = static_cast<int>(
+        round_up(std::max(1, std::min<int>(2 * n_cu_, static_cast<int>(ceil_div(n_items, 4)))), kXcds));
     planeV8_.ensure(std::max<std::size_t>(plane, 16));
     if (precision == STP_PREC_FP64) {
         planeA_.ensure(std::max<std::size_t>(plane, 1));
@@ -315,8 +332,8 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
                                  planeA_.get(), planeV_.get(), planeQ_.get(), planeV8_.get(), stream_);
         mark_end(kPrep);
         mark_begin(kF2);
-        launch_f2_fp64(planeA_.get(), planeV_.get(), planeQ_.get(), planeV8_.get(), bt, plan_dev_.items.get(), n_items,
-                       counter_.get(), plan.P, plan.G, KS, d_f2, d_vpair, n_wg, stream_);
+        launch_f2_fp64(planeA_.get(), planeV_.get(), planeQ_.get(), planeV8_.get(), bt, plan_dev_.items.get(),
+                       plan.q64, counter_.get(), plan.P, plan.G, KS, d_f2, d_vpair, n_wg, stream_);
         mark_end(kF2);
         return;
     }
@@ -333,13 +350,14 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
                               KS16, planeSplit_.get(), planeV8_.get(), stream_);
     mark_end(kPrep);
     mark_begin(kCount);
-    launch_count(planeV8_.get(), bt, plan_dev_.items.get(), n_items, counter_.get(), plan.P, plan.G, plan.K_total / 64,
-                 d_vpair, n_wg, stream_);
+    launch_count(planeV8_.get(), bt, plan_dev_.items.get(), plan.q64, counter_.get(), plan.P, plan.G,
+                 plan.K_total / 64, d_vpair, n_wg, stream_);
     mark_end(kCount);
     const int n_items128 = static_cast<int>(plan.items128.size());
     mark_begin(kF2);
-    launch_f2_split(planeSplit_.get(), bt, plan_dev_.items128.get(), n_items128, counter_.get() + 1, plan.P, plan.G32,
-                    KS16, d_vpair, d_f2, std::max(1, std::min(n_cu_, n_items128)), stream_);
+    launch_f2_split(planeSplit_.get(), bt, plan_dev_.items128.get(), plan.q128, counter_.get() + kXcds, plan.P,
+                    plan.G32, KS16, d_vpair, d_f2,
+                    static_cast<int>(round_up(std::max(1, std::min(n_cu_, n_items128)), kXcds)), stream_);
     mark_end(kF2);
 }
 

@@ -15,8 +15,7 @@
 
 #include "common.hpp"
 #include "f2_layout.hpp"
-
-struct stp_tile;
+#include "steppe_hip.h"
 
 namespace stp {
 
@@ -38,19 +37,20 @@ void launch_prep_fp64_qvn(const double* q, const double* v, const double* n, int
                           const std::int32_t* chunk_block, std::int64_t n_chunks, std::int64_t KS,
                           double* pA, double* pV, double* pQ, void* pV8, hipStream_t st);
 void launch_f2_fp64(const double* pA, const double* pV, const double* pQ, const void* pV8, BlockTable bt,
-                    const F2Item* items, int n_items, unsigned* counter, int P, int G, std::int64_t KS,
-                    double* f2, double* vpair, int n_workgroups, hipStream_t st);
+                    const F2Item* items, const ItemQueues& queues, unsigned* counters, int P, int G,
+                    std::int64_t KS, double* f2, double* vpair, int n_workgroups, hipStream_t st);
 void launch_prep_split_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P, int G32,
                               int G16, BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks,
                               std::int64_t KS16, std::uint8_t* planes, void* pV8, hipStream_t st);
 void launch_prep_split_qvn(const double* q, const double* v, const double* n, int P, int G32, int G16,
                            BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks, std::int64_t KS16,
                            std::uint8_t* planes, void* pV8, hipStream_t st);
-void launch_count(const void* pV8, BlockTable bt, const F2Item* items, int n_items, unsigned* counter, int P,
-                  int G16, std::int64_t KS64, double* vpair, int n_workgroups, hipStream_t st);
-void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, int n_items, unsigned* counter,
-                     int P, int G32, std::int64_t KS16, const double* vpair, double* f2, int n_workgroups,
-                     hipStream_t st);
+void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const ItemQueues& queues,
+                  unsigned* counters, int P, int G16, std::int64_t KS64, double* vpair, int n_workgroups,
+                  hipStream_t st);
+void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
+                     unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
+                     int n_workgroups, hipStream_t st);
 void launch_synth_panel(std::uint64_t seed, int P, std::int64_t M, std::int64_t n_ind, std::int64_t bpr,
                         const float* fst, const float* pop_missing, const std::int32_t* ind_pop,
                         const std::int8_t* ind_haploid, float* freq_scratch, std::uint8_t* packed,
@@ -65,6 +65,7 @@ struct F2Plan {
     std::vector<std::int32_t> len, kpad, chunk_block;
     std::vector<F2Item> items;     // 64-row tile pairs (FP64 contraction, pair counts)
     std::vector<F2Item> items128;  // 128-row tile pairs (split contraction)
+    ItemQueues q64{}, q128{};      // per-die segments of items / items128
     int G32 = 0, T128 = 0;
 };
 
@@ -108,9 +109,13 @@ public:
     F2Blocks* extract_f2(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision);
     F2Blocks* upload(const double* f2, const double* vpair, const std::int32_t* block_sizes, int P,
                      int n_block);
+    // Batched qpAdm over models on a resident tensor; results to host (qpadm_host.cpp).
+    void qpadm_batch(const F2Blocks& h, int n_models, const std::int32_t* target, const std::int32_t* left_off,
+                     const std::int32_t* left, const std::int32_t* right_off, const std::int32_t* right,
+                     const stp_qpadm_options& opt, stp_qpadm_result* out);
 
     // Per-stage device timing with HIP events on this backend's stream (opt-in).
-    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kStages = 4 };
+    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kStages = 5 };
     void set_timing(bool on);
     void read_timing(double* ms, std::int64_t* count, bool reset);
 
@@ -169,6 +174,11 @@ private:
     void mark_end(Stage s);
     void collect_timing();
     bool timing_ = false;
+    // qpAdm working set (grow-only): survivor-transposed f2, survivor sizes and list,
+    // the model arena, per-workgroup scratch and the device result records.
+    DevBuf<double> qpT_, qp_scratch_;
+    DevBuf<std::int32_t> qp_i32_, qp_keep_;
+    DevBuf<stp_qpadm_result> qp_out_;
     hipEvent_t ev_begin_[kStages] = {}, ev_end_[kStages] = {};
     bool pending_[kStages] = {};
     double total_ms_[kStages] = {};

@@ -157,8 +157,8 @@ __global__ __launch_bounds__(256) void prep_fp64_kernel(
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void f2_fp64_kernel(
     const double* __restrict__ pA, const double* __restrict__ pV, const double* __restrict__ pQ,
-    const i4* __restrict__ pV8, BlockTable bt, const F2Item* __restrict__ items, int n_items,
-    unsigned* __restrict__ counter, int P, int G, std::int64_t KS, double* __restrict__ f2,
+    const i4* __restrict__ pV8, BlockTable bt, const F2Item* __restrict__ items, ItemQueues queues,
+    unsigned* __restrict__ counters, int P, int G, std::int64_t KS, double* __restrict__ f2,
     double* __restrict__ vpair) {
     const int lane = threadIdx.x & 63;
     // v_mfma_f64_16x16x4f64 returns D[(l/16) + 4e][l%16] in lane l, item e, while the
@@ -169,12 +169,14 @@ __global__ __launch_bounds__(256) void f2_fp64_kernel(
     const std::int64_t KS64 = KS / 16;
     const std::int64_t slab = static_cast<std::int64_t>(P) * P;
 
+    const int home = static_cast<int>(blockIdx.x % kXcds);
+    int xq = 0;
     for (;;) {
         int it = 0;
-        if (lane == 0) it = static_cast<int>(atomicAdd(counter, 1u));
+        if (lane == 0) it = claim_item(counters, queues, home, xq);
         // Uniform values in scalar registers keep every branch below uniform.
         it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-        if (it >= n_items) break;
+        if (it < 0) break;
         const F2Item w = items[it];
         const int b = __builtin_amdgcn_readfirstlane(w.b);
         const int gI0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(w.ti));
@@ -298,12 +300,12 @@ void launch_prep_fp64_qvn(const double* q, const double* v, const double* n, int
 }
 
 void launch_f2_fp64(const double* pA, const double* pV, const double* pQ, const void* pV8, BlockTable bt,
-                    const F2Item* items, int n_items, unsigned* counter, int P, int G, std::int64_t KS,
-                    double* f2, double* vpair, int n_workgroups, hipStream_t st) {
-    if (n_items <= 0) return;
-    launch_zero_u32(counter, 1, st);
+                    const F2Item* items, const ItemQueues& queues, unsigned* counters, int P, int G,
+                    std::int64_t KS, double* f2, double* vpair, int n_workgroups, hipStream_t st) {
+    if (queues.off[kXcds] <= 0) return;
+    launch_zero_u32(counters, kXcds, st);
     f2_fp64_kernel<<<static_cast<unsigned>(n_workgroups), 256, 0, st>>>(
-        pA, pV, pQ, static_cast<const i4*>(pV8), bt, items, n_items, counter, P, G, KS, f2, vpair);
+        pA, pV, pQ, static_cast<const i4*>(pV8), bt, items, queues, counters, P, G, KS, f2, vpair);
     STP_LAUNCH_CHECK();
 }
 

@@ -42,4 +42,34 @@ struct F2Item {
     std::int16_t tj;  // right 64-row tile (ti <= tj)
 };
 
+// MI355X has 8 accelerator dies (XCDs), each with its own L2, and the dispatcher
+// hands workgroup i of a launch to die i % 8.  The persistent f2 kernels therefore
+// keep one work queue per die: every block (all its tile pairs) is assigned to one
+// die, so the tiles that share a block's operand panels run under the same L2.
+constexpr int kXcds = 8;
+
+// Items of die x are items[off[x] .. off[x+1]); counters[x] is that queue's cursor.
+struct ItemQueues {
+    std::int32_t off[kXcds + 1];
+};
+
+#ifdef __HIPCC__
+// Claims the next item for a worker homed on die `home`: its own queue first, then
+// the other dies' queues in turn once it is drained (`xq` counts the queues already
+// exhausted by this worker and starts at 0).  Returns -1 when every queue is empty.
+// Called by one lane; at most kXcds atomics per call.
+__device__ inline int claim_item(unsigned* counters, const ItemQueues& q, int home, int& xq) {
+    while (xq < kXcds) {
+        const int x = (home + xq) % kXcds;
+        const int n = q.off[x + 1] - q.off[x];
+        if (n > 0) {
+            const unsigned v = atomicAdd(&counters[x], 1u);
+            if (static_cast<int>(v) < n) return q.off[x] + static_cast<int>(v);
+        }
+        ++xq;
+    }
+    return -1;
+}
+#endif
+
 }  // namespace stp

@@ -10,8 +10,9 @@
 //
 //   num_ij = sum_s v_i v_j [(q_i - q_j)^2 - h_i - h_j],  f2 = num / vpair
 //
-// Operands.  With the per-SNP centre c_s, q' = q - c (|q'| < 1) and w = -(q'^2 - h)/2
-// (|w| <= 1/2), each is written as six balanced 8-bit digits (|digit| <= 128, exact
+// Operands.  The estimator only sees differences q_i - q_j, so any per-SNP shift of q
+// leaves it unchanged; with q' = q - 1/2 (|q'| <= 1/2) and w = -(q'^2 - h)/2
+// (|w| <= 1/8), each is written as six balanced 8-bit digits (|digit| <= 128, exact
 // in BF16):
 //     q~ = 2^-7 sum_k d_k 2^-8k,     w~ = 2^-7 sum_m c_m 2^-8m,     k, m = 0..5
 // w is computed from q~ (not q'), so the device evaluates the estimator for the
@@ -60,26 +61,48 @@ constexpr int kSlotBytes = kGroups * 6 * kChunk;   // 48 KB (largest phase: 6 pl
 constexpr int kSlots = 3;
 constexpr int kDPlanes = 6, kEPlanes = 4;  // planes per phase: q digits; 3 w digits + V
 constexpr int kWaves = 8;
-constexpr double kCentreScale = 256.0;
 
 // ---------------------------------------------------------------------------------
 // prep: frequencies -> 13 BF16 digit planes (32x32x16 MFMA fragment order) + the
-// int8 V plane (16x16x64 fragment order, for the count kernel).
+// int8 V plane (16x16x64 fragment order, for the count kernel), in one pass.
 //   plane chunk (g32, t16, p) holds rows 32 g32 + lane%32, SNPs 16 t16 + 8 (lane/32) + j
 //   at byte offset ((g32 * KS16 + t16) * 13 + p) * 1024 + lane * 16 + 2 j.
 //   Planes 5 (d5) and 11 (e5) are stored scaled by 2^-8 (they only enter level 5).
+// A workgroup converts one (32-row group, 64-SNP chunk) unit at a time: it stages the
+// unit's frequencies and heterozygosity terms in LDS with coalesced row reads, then
+// every thread expands one 8-SNP fragment into its digits.
 // ---------------------------------------------------------------------------------
+constexpr int kStageLd = 65;  // LDS row stride (doubles) of the staged unit
+
+// Staging reads of the two frequency sources.  Each thread stages 8 elements of the
+// 32 x 64 unit; (q - 1/2, h) goes to sq/sh, with h = -1 marking an invalid entry.
+// h = q(1-q)/max(n-1, 1) as in the reference's het_correction
+// (src/core/internal/f2_estimator.hpp:22-29).
 struct CountsSrc {
     const std::uint16_t* a2;
     const std::uint16_t* n;
     std::int64_t ld;
-    __device__ bool get(int r, std::int64_t s, double& q, double& nd) const {
-        const std::int64_t o = static_cast<std::int64_t>(r) * ld + s;
-        const unsigned c = n[o];
-        if (!c) return false;
-        nd = static_cast<double>(c);
-        q = static_cast<double>(a2[o]) / (2.0 * nd);
-        return true;
+    // Lanes run along the SNP axis: each wavefront reads 128 contiguous bytes per row.
+    __device__ void stage(int r0, int P, std::int64_t snp0, int live, int tid, double* sq, double* sh) const {
+        const int kk = tid & 63;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row = (tid >> 6) + 4 * i;
+            double qp = 0.0, h = -1.0;
+            if (r0 + row < P && kk < live) {
+                const std::int64_t o = static_cast<std::int64_t>(r0 + row) * ld + snp0 + kk;
+                const unsigned c = n[o];
+                if (c) {
+                    const double nd = static_cast<double>(c);
+                    const double q = static_cast<double>(a2[o]) / (2.0 * nd);
+                    const double nm1 = nd - 1.0;
+                    h = q * (1.0 - q) / (nm1 > 1.0 ? nm1 : 1.0);
+                    qp = q - 0.5;
+                }
+            }
+            sq[row * kStageLd + kk] = qp;
+            sh[row * kStageLd + kk] = h;
+        }
     }
 };
 
@@ -87,18 +110,31 @@ struct QvnSrc {
     const double* q;
     const double* v;
     const double* n;
-    int P;
-    __device__ bool get(int r, std::int64_t s, double& qq, double& nd) const {
-        const std::int64_t o = r + static_cast<std::int64_t>(P) * s;
-        if (v[o] == 0.0) return false;
-        qq = q[o];
-        nd = n[o];
-        return true;
+    // Q/V/N are population-fastest (i + P*s): lanes run along the population axis.
+    __device__ void stage(int r0, int P, std::int64_t snp0, int live, int tid, double* sq, double* sh) const {
+        const int row = tid & 31;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int kk = (tid >> 5) + 8 * i;
+            double qp = 0.0, h = -1.0;
+            if (r0 + row < P && kk < live) {
+                const std::int64_t o = r0 + row + static_cast<std::int64_t>(P) * (snp0 + kk);
+                if (v[o] != 0.0) {
+                    const double qq = q[o];
+                    const double nm1 = n[o] - 1.0;
+                    h = qq * (1.0 - qq) / (nm1 > 1.0 ? nm1 : 1.0);
+                    qp = qq - 0.5;
+                }
+            }
+            sq[row * kStageLd + kk] = qp;
+            sh[row * kStageLd + kk] = h;
+        }
     }
 };
 
 __device__ inline unsigned bf16_bits(double integer_digit) {
-    // The digit has at most 8 significant bits (an integer, possibly times a power
+    // Integer digits |d| <= 2^14 and their 2^-8 multiples have <= 8 significant
+    // bits (a power of two times an integer below 2^8 in magnitude, or a power
     // of two): the float and bf16 conversions are exact.
     return __float_as_uint(static_cast<float>(integer_digit)) >> 16;
 }
@@ -106,56 +142,40 @@ __device__ inline unsigned bf16_bits(double integer_digit) {
 template <class Src>
 __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32, int G16, BlockTable bt,
                                                          const std::int32_t* __restrict__ chunk_block,
-                                                         std::int64_t KS16, std::uint8_t* __restrict__ planes,
+                                                         std::int64_t n_chunks, std::int64_t KS16,
+                                                         std::uint8_t* __restrict__ planes,
                                                          i4* __restrict__ pV8) {
-    __shared__ double red_sum[4][64];
-    __shared__ int red_cnt[4][64];
-    __shared__ double centre[64];
-
+    __shared__ double sq[32 * kStageLd];
+    __shared__ double sh[32 * kStageLd];
     const int tid = threadIdx.x;
-    const std::int64_t c = blockIdx.x;
-    const int b = chunk_block[c];
-    const std::int64_t local0 = 64 * c - bt.k0[b];
-    const std::int64_t snp0 = bt.s0[b] + local0;
-    const int live = static_cast<int>(max<std::int64_t>(0, min<std::int64_t>(64, bt.len[b] - local0)));
-    {
-        const int kk = tid & 63, pg = tid >> 6;
-        double s = 0.0;
-        int cnt = 0;
-        if (kk < live)
-            for (int p = pg; p < P; p += 4) {
-                double q, nd;
-                if (src.get(p, snp0 + kk, q, nd)) { s += q; ++cnt; }
-            }
-        red_sum[pg][kk] = s;
-        red_cnt[pg][kk] = cnt;
-        __syncthreads();
-        if (tid < 64) {
-            const double ts = red_sum[0][tid] + red_sum[1][tid] + red_sum[2][tid] + red_sum[3][tid];
-            const int tc = red_cnt[0][tid] + red_cnt[1][tid] + red_cnt[2][tid] + red_cnt[3][tid];
-            centre[tid] = tc ? rint(ts / tc * kCentreScale) / kCentreScale : 0.0;
-        }
-        __syncthreads();
-    }
-
-    // BF16 digit planes: thread -> (16-SNP step t4 of the chunk, fragment lane).
     const int t4 = tid >> 6, lane = tid & 63;
-    const std::int64_t t16 = 4 * c + t4;
-    for (int g = 0; g < G32; ++g) {
-        const int r = 32 * g + (lane & 31);
+    const std::int64_t n_units = static_cast<std::int64_t>(G32) * n_chunks;
+    // Units run chunk-fastest so consecutive workgroups read adjacent row segments and
+    // write adjacent plane chunks.
+    for (std::int64_t u = blockIdx.x; u < n_units; u += gridDim.x) {
+        const int g = static_cast<int>(u / n_chunks);
+        const std::int64_t c = u - static_cast<std::int64_t>(g) * n_chunks;
+        const int b = chunk_block[c];
+        const std::int64_t local0 = 64 * c - bt.k0[b];
+        const std::int64_t snp0 = bt.s0[b] + local0;
+        const int live = static_cast<int>(max<std::int64_t>(0, min<std::int64_t>(64, bt.len[b] - local0)));
+        __syncthreads();  // the previous unit's readers are done with the stage
+        src.stage(32 * g, P, snp0, live, tid, sq, sh);
+        __syncthreads();
+
+        // BF16 digit planes: thread -> (16-SNP step t4 of the chunk, fragment lane).
+        const int r = lane & 31;
         unsigned w[kPlanes][4];
 #pragma unroll
         for (int p = 0; p < kPlanes; ++p)
 #pragma unroll
-            for (int h = 0; h < 4; ++h) w[p][h] = 0u;
+            for (int hh = 0; hh < 4; ++hh) w[p][hh] = 0u;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int kk = 16 * t4 + 8 * (lane >> 5) + j;
-            double q, nd;
-            if (r < P && kk < live && src.get(r, snp0 + kk, q, nd)) {
-                const double nm1 = nd - 1.0;
-                const double h = q * (1.0 - q) / (nm1 > 1.0 ? nm1 : 1.0);
-                double x = (q - centre[kk]) * 128.0;
+            const double h = sh[r * kStageLd + kk];
+            if (h >= 0.0) {
+                double x = sq[r * kStageLd + kk] * 128.0;
                 double d[6];
 #pragma unroll
                 for (int k = 0; k < 6; ++k) {
@@ -171,41 +191,40 @@ __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32
                     e[m] = rint(y);
                     y = (y - e[m]) * 256.0;
                 }
-                const int sh = 16 * (j & 1);
+                const int sft = 16 * (j & 1);
 #pragma unroll
-                for (int k = 0; k < 5; ++k) w[k][j >> 1] |= bf16_bits(d[k]) << sh;
-                w[5][j >> 1] |= bf16_bits(d[5] / 256.0) << sh;
+                for (int k = 0; k < 5; ++k) w[k][j >> 1] |= bf16_bits(d[k]) << sft;
+                w[5][j >> 1] |= bf16_bits(d[5] / 256.0) << sft;
 #pragma unroll
-                for (int m = 0; m < 5; ++m) w[6 + m][j >> 1] |= bf16_bits(e[m] * 128.0) << sh;
-                w[11][j >> 1] |= bf16_bits(e[5] * 0.5) << sh;
-                w[kVPlane][j >> 1] |= bf16_bits(1.0) << sh;
+                for (int m = 0; m < 5; ++m) w[6 + m][j >> 1] |= bf16_bits(e[m] * 128.0) << sft;
+                w[11][j >> 1] |= bf16_bits(e[5] * 0.5) << sft;
+                w[kVPlane][j >> 1] |= bf16_bits(1.0) << sft;
             }
         }
+        const std::int64_t t16 = 4 * c + t4;
         std::uint8_t* base = planes + ((static_cast<std::int64_t>(g) * KS16 + t16) * kPlanes) * kChunk + lane * 16;
 #pragma unroll
         for (int p = 0; p < kPlanes; ++p)
             *reinterpret_cast<uint4*>(base + p * kChunk) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
-    }
-    // int8 V plane for the pair-count kernel (16x16x64 layout).
-    if (tid < 64) {
-        for (int g = 0; g < G16; ++g) {
-            const int r = 16 * g + (tid & 15);
-            const int kb = 16 * (tid >> 4);
-            unsigned w[4] = {0, 0, 0, 0};
-            if (r < P) {
+
+        // int8 V plane for the pair-count kernel (16x16x64 layout): the unit's two
+        // 16-row groups, one 16-byte fragment per thread of the first two wavefronts.
+        if (tid < 128) {
+            const int g16 = 2 * g + (tid >> 6);
+            if (g16 < G16) {
+                const int rr = 16 * (tid >> 6) + (lane & 15);
+                const int kb = 16 * (lane >> 4);
+                unsigned v4[4] = {0, 0, 0, 0};
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int kk = kb + j;
-                    double q, nd;
-                    if (kk < live && src.get(r, snp0 + kk, q, nd)) w[j >> 2] |= 1u << (8 * (j & 3));
-                }
+                for (int j = 0; j < 16; ++j)
+                    if (sh[rr * kStageLd + kb + j] >= 0.0) v4[j >> 2] |= 1u << (8 * (j & 3));
+                i4 v;
+                v[0] = static_cast<int>(v4[0]);
+                v[1] = static_cast<int>(v4[1]);
+                v[2] = static_cast<int>(v4[2]);
+                v[3] = static_cast<int>(v4[3]);
+                pV8[(static_cast<std::int64_t>(g16) * (KS16 / 4) + c) * 64 + lane] = v;
             }
-            i4 v;
-            v[0] = static_cast<int>(w[0]);
-            v[1] = static_cast<int>(w[1]);
-            v[2] = static_cast<int>(w[2]);
-            v[3] = static_cast<int>(w[3]);
-            pV8[(static_cast<std::int64_t>(g) * (KS16 / 4) + c) * 64 + tid] = v;
         }
     }
 }
@@ -215,16 +234,18 @@ __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32
 // (i,j) and (j,i).  Persistent wavefronts over (block, 64x64 tile pair) items.
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void count_kernel(const i4* __restrict__ pV8, BlockTable bt,
-                                                    const F2Item* __restrict__ items, int n_items,
-                                                    unsigned* __restrict__ counter, int P, int G16,
+                                                    const F2Item* __restrict__ items, ItemQueues queues,
+                                                    unsigned* __restrict__ counters, int P, int G16,
                                                     std::int64_t KS64, double* __restrict__ vpair) {
     const int lane = threadIdx.x & 63;
     const std::int64_t slab = static_cast<std::int64_t>(P) * P;
+    const int home = static_cast<int>(blockIdx.x % kXcds);
+    int xq = 0;
     for (;;) {
         int it = 0;
-        if (lane == 0) it = static_cast<int>(atomicAdd(counter, 1u));
+        if (lane == 0) it = claim_item(counters, queues, home, xq);
         it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-        if (it >= n_items) break;
+        if (it < 0) break;
         const F2Item wi = items[it];
         const int b = __builtin_amdgcn_readfirstlane(wi.b);
         const int gI0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(wi.ti));
@@ -320,8 +341,8 @@ __device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std:
 }
 
 __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __restrict__ planes, BlockTable bt,
-                                                       const F2Item* __restrict__ items, int n_items,
-                                                       unsigned* __restrict__ counter, int P, int G32,
+                                                       const F2Item* __restrict__ items, ItemQueues queues,
+                                                       unsigned* __restrict__ counters, int P, int G32,
                                                        std::int64_t KS16, const double* __restrict__ vpair,
                                                        double* __restrict__ f2) {
     __shared__ __attribute__((aligned(16))) std::uint8_t lds[kSlots * kSlotBytes];
@@ -337,12 +358,14 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
     std::uint8_t* const slot_e1 = lds + kSlotBytes;
     std::uint8_t* const slot_e2 = lds + 2 * kSlotBytes;
 
+    const int home = static_cast<int>(blockIdx.x % kXcds);
+    int xq = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_item = static_cast<int>(atomicAdd(counter, 1u));
+        if (threadIdx.x == 0) s_item = claim_item(counters, queues, home, xq);
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
-        if (it >= n_items) break;
+        if (it < 0) break;
         const F2Item wi = items[it];
         const int b = __builtin_amdgcn_readfirstlane(wi.b);
         const int gI0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(wi.ti));
@@ -503,40 +526,47 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
 
 }  // namespace
 
+// Persistent-style grid for the prep: enough workgroups to fill every CU several
+// times over, each looping over (32-row group, chunk) units.
+static unsigned prep_grid(std::int64_t n_units) {
+    return static_cast<unsigned>(std::max<std::int64_t>(1, std::min<std::int64_t>(n_units, 8192)));
+}
+
 void launch_prep_split_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P, int G32,
                               int G16, BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks,
                               std::int64_t KS16, std::uint8_t* planes, void* pV8, hipStream_t st) {
-    if (n_chunks <= 0) return;
-    prep_split_kernel<CountsSrc><<<static_cast<unsigned>(n_chunks), 256, 0, st>>>(
-        CountsSrc{a2, n, ld}, P, G32, G16, bt, chunk_block, KS16, planes, static_cast<i4*>(pV8));
+    if (n_chunks <= 0 || G32 <= 0) return;
+    prep_split_kernel<CountsSrc><<<prep_grid(G32 * n_chunks), 256, 0, st>>>(
+        CountsSrc{a2, n, ld}, P, G32, G16, bt, chunk_block, n_chunks, KS16, planes, static_cast<i4*>(pV8));
     STP_LAUNCH_CHECK();
 }
 
 void launch_prep_split_qvn(const double* q, const double* v, const double* n, int P, int G32, int G16,
                            BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks, std::int64_t KS16,
                            std::uint8_t* planes, void* pV8, hipStream_t st) {
-    if (n_chunks <= 0) return;
-    prep_split_kernel<QvnSrc><<<static_cast<unsigned>(n_chunks), 256, 0, st>>>(
-        QvnSrc{q, v, n, P}, P, G32, G16, bt, chunk_block, KS16, planes, static_cast<i4*>(pV8));
+    if (n_chunks <= 0 || G32 <= 0) return;
+    prep_split_kernel<QvnSrc><<<prep_grid(G32 * n_chunks), 256, 0, st>>>(
+        QvnSrc{q, v, n}, P, G32, G16, bt, chunk_block, n_chunks, KS16, planes, static_cast<i4*>(pV8));
     STP_LAUNCH_CHECK();
 }
 
-void launch_count(const void* pV8, BlockTable bt, const F2Item* items, int n_items, unsigned* counter, int P,
-                  int G16, std::int64_t KS64, double* vpair, int n_workgroups, hipStream_t st) {
-    if (n_items <= 0) return;
-    launch_zero_u32(counter, 1, st);
+void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const ItemQueues& queues,
+                  unsigned* counters, int P, int G16, std::int64_t KS64, double* vpair, int n_workgroups,
+                  hipStream_t st) {
+    if (queues.off[kXcds] <= 0) return;
+    launch_zero_u32(counters, kXcds, st);
     count_kernel<<<static_cast<unsigned>(n_workgroups), 256, 0, st>>>(static_cast<const i4*>(pV8), bt, items,
-                                                                      n_items, counter, P, G16, KS64, vpair);
+                                                                      queues, counters, P, G16, KS64, vpair);
     STP_LAUNCH_CHECK();
 }
 
-void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, int n_items, unsigned* counter,
-                     int P, int G32, std::int64_t KS16, const double* vpair, double* f2, int n_workgroups,
-                     hipStream_t st) {
-    if (n_items <= 0) return;
-    launch_zero_u32(counter, 1, st);
-    f2_split_kernel<<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, n_items, counter, P, G32,
-                                                                        KS16, vpair, f2);
+void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
+                     unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
+                     int n_workgroups, hipStream_t st) {
+    if (queues.off[kXcds] <= 0) return;
+    launch_zero_u32(counters, kXcds, st);
+    f2_split_kernel<<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues, counters, P,
+                                                                        G32, KS16, vpair, f2);
     STP_LAUNCH_CHECK();
 }
 

@@ -1,0 +1,1071 @@
+// steppe_amd/csrc/qpadm.hip — batched qpAdm fits on the device-resident f2 tensor.
+//
+// One workgroup (4 wavefronts) fits one model at a time and loops over the models of
+// its launch (a shared atomic cursor hands them out in decreasing-cost order).  Per
+// model it runs the reference's fit chain (src/core/qpadm/qpadm_fit.cpp:49-132):
+//
+//   f4 basis        X[k][b] = (f2(Li,R0) + f2(L0,Rj) - f2(L0,R0) - f2(Li,Rj)) / 2,
+//                   k = j + nr*i, over the surviving blocks
+//                   (src/device/cpu/cpu_backend.cpp:125-150, 2065-2113)
+//   LOO / total     src/device/cpu/cpu_backend.cpp:2674-2723
+//   covariance      Q = sum_b tau tau^T / nb, Qf = Q + fudge tr(Q) I, Qinv = Qf^-1 by LU
+//                   with partial pivoting (cpu_backend.cpp:2219-2271, 2758-2764;
+//                   src/core/internal/small_linalg.hpp:55-135)
+//   ALS fit         one-sided Jacobi SVD seed, fixed ALS iterations of ridge-GLS
+//                   A- and B-steps, weights from [A, 1], residual chi^2
+//                   (cpu_backend.cpp:2727-2935; small_linalg.hpp:137-218)
+//   rank sweep      ranks 0..min(nl, nr)-1 on the full model (cpu_backend.cpp:2533-2610)
+//   popdrop         the full model and each source dropped (src/core/qpadm/ranktest.cpp:25-140)
+//   LOO SE          nb leave-one-block-out refits, scaled by (nb-1)/sqrt(nb)
+//                   (cpu_backend.cpp:98-122, 2625-2668)
+//
+// Where the reference accumulates in long double, the device accumulates in
+// double-double (TwoSum / FMA TwoProd), which is at least as precise.  Plain double
+// sums keep the reference's order; this file is built with FP contraction off so each
+// product is rounded before it is added, as in the reference.  The GLS right-hand side
+// is evaluated as L^T (Qinv^T x) rather than (Qinv L)^T x (the same quantity, another
+// rounding order) and back substitution runs column-wise.
+//
+// Work per wavefront.  A fit's normal matrices (t x t, t = nl*r or r*nr <= T) are held
+// one row per lane in registers; the LU runs the reference's pivot rule (first row of
+// largest magnitude) with logical row positions instead of row moves, and pivot rows
+// are broadcast by lane reads.  Qinv lives in LDS and is shared by the workgroup; the
+// per-wave vectors (x, Qinv^T x, A, B, one W column, the SVD work arrays) live in a
+// per-wave LDS slice.
+
+#include <climits>
+#include <cmath>
+#include <type_traits>
+#include <utility>
+
+#include "common.hpp"
+#include "qpadm.hpp"
+#include "stp_qpadm_result.h"
+
+namespace stp {
+
+namespace {
+
+constexpr int kQWaves = 4;
+constexpr int kQThreads = 64 * kQWaves;
+constexpr int kMaxNl = STP_QPADM_MAX_NL;
+constexpr int kMaxRows = kMaxNl + 1;   // popdrop rows
+constexpr int kMaxRanks = kMaxNl + 1;  // fits per row: ranks 0..8
+
+// ---------------------------------------------------------------------------------
+// wavefront helpers
+// ---------------------------------------------------------------------------------
+__device__ inline int lane_id() { return static_cast<int>(threadIdx.x & 63); }
+
+__device__ inline double rl(double x, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ inline int rli(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+
+// f(std::integral_constant<int, I>) for I = 0 .. N-1, as straight-line code.
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+        (f(std::integral_constant<int, I>{}), ...);
+    }(std::make_integer_sequence<int, N>{});
+}
+
+// A value every lane computed identically, moved to scalar registers so the compiler
+// treats branches on it as uniform.
+__device__ inline double uni(double x) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+
+__device__ inline int first_lane(bool pred) {
+    const unsigned long long b = __ballot(pred);
+    return b ? __builtin_ctzll(b) : -1;
+}
+
+__device__ inline int popc(bool pred) { return __builtin_popcountll(__ballot(pred)); }
+
+// Full-wavefront reductions (all 64 lanes active) over DPP row operations and the
+// row broadcasts; the result is read from lane 63.
+#define STP_DPP_STEP(op, ident, ctrl, rmask) v = op(v, __builtin_amdgcn_update_dpp(ident, v, ctrl, rmask, 0xF, false))
+__device__ inline int wave_max_i32(int v) {
+    STP_DPP_STEP(max, INT_MIN, 0xB1, 0xF);   // quad_perm [1,0,3,2]
+    STP_DPP_STEP(max, INT_MIN, 0x4E, 0xF);   // quad_perm [2,3,0,1]
+    STP_DPP_STEP(max, INT_MIN, 0x141, 0xF);  // row_half_mirror
+    STP_DPP_STEP(max, INT_MIN, 0x140, 0xF);  // row_mirror
+    STP_DPP_STEP(max, INT_MIN, 0x142, 0xA);  // row_bcast:15
+    STP_DPP_STEP(max, INT_MIN, 0x143, 0xC);  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ inline int wave_min_i32(int v) {
+    STP_DPP_STEP(min, INT_MAX, 0xB1, 0xF);
+    STP_DPP_STEP(min, INT_MAX, 0x4E, 0xF);
+    STP_DPP_STEP(min, INT_MAX, 0x141, 0xF);
+    STP_DPP_STEP(min, INT_MAX, 0x140, 0xF);
+    STP_DPP_STEP(min, INT_MAX, 0x142, 0xA);
+    STP_DPP_STEP(min, INT_MAX, 0x143, 0xC);
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ inline unsigned umax_(unsigned a, unsigned b) { return a > b ? a : b; }
+__device__ inline unsigned wave_max_u32(unsigned v0) {
+    int v = static_cast<int>(v0);
+#define STP_UMAX(a, b) static_cast<int>(umax_(static_cast<unsigned>(a), static_cast<unsigned>(b)))
+    STP_DPP_STEP(STP_UMAX, 0, 0xB1, 0xF);
+    STP_DPP_STEP(STP_UMAX, 0, 0x4E, 0xF);
+    STP_DPP_STEP(STP_UMAX, 0, 0x141, 0xF);
+    STP_DPP_STEP(STP_UMAX, 0, 0x140, 0xF);
+    STP_DPP_STEP(STP_UMAX, 0, 0x142, 0xA);
+    STP_DPP_STEP(STP_UMAX, 0, 0x143, 0xC);
+#undef STP_UMAX
+    return static_cast<unsigned>(__builtin_amdgcn_readlane(v, 63));
+}
+#undef STP_DPP_STEP
+
+// The lane holding the pivot: largest |value| among candidate lanes (val >= 0;
+// non-candidates pass -1), ties broken by the smallest logical position, matching
+// the reference's scan with a strict '>' (small_linalg.hpp:62-70).
+__device__ inline int argmax_pivot(double val, int pos) {
+    const long long bits = __double_as_longlong(val);
+    const int hi = static_cast<int>(bits >> 32);
+    const int mhi = wave_max_i32(hi);
+    bool tie = hi == mhi;
+    if (popc(tie) > 1) {
+        const unsigned lo = tie ? static_cast<unsigned>(bits & 0xffffffffLL) : 0u;
+        const unsigned mlo = wave_max_u32(lo);
+        tie = tie && lo == mlo;
+        if (popc(tie) > 1) {
+            const int mp = wave_min_i32(tie ? pos : INT_MAX);
+            tie = tie && pos == mp;
+        }
+    }
+    return first_lane(tie);
+}
+
+// ---------------------------------------------------------------------------------
+// double-double accumulation (stands in for the reference's long double sums)
+// ---------------------------------------------------------------------------------
+struct dd {
+    double hi, lo;
+};
+
+__device__ inline dd dd_norm(double s, double e) {
+    const double h = s + e;
+    return {h, e - (h - s)};
+}
+__device__ inline dd dd_add(dd a, double b) {
+    const double s = a.hi + b;
+    const double bb = s - a.hi;
+    const double e = (a.hi - (s - bb)) + (b - bb);
+    return dd_norm(s, e + a.lo);
+}
+__device__ inline dd dd_add_dd(dd a, dd b) {
+    const double s = a.hi + b.hi;
+    const double bb = s - a.hi;
+    const double e = (a.hi - (s - bb)) + (b.hi - bb);
+    return dd_norm(s, e + a.lo + b.lo);
+}
+// a + x*y with the product kept exactly.
+__device__ inline dd dd_fma(dd a, double x, double y) {
+    const double p = x * y;
+    const double pe = fma(x, y, -p);
+    dd r = dd_add(a, p);
+    return dd_norm(r.hi, r.lo + pe);
+}
+// a + x*y for a double-double y.
+__device__ inline dd dd_fma_dd(dd a, double x, dd y) {
+    const double p = x * y.hi;
+    const double pe = fma(x, y.hi, -p);
+    dd r = dd_add(a, p);
+    return dd_norm(r.hi, r.lo + (pe + x * y.lo));
+}
+// (double)(a / d)
+__device__ inline double dd_div_d(dd a, double d) {
+    const double q = a.hi / d;
+    const double p = q * d;
+    const double pe = fma(q, d, -p);
+    const double r = ((a.hi - p) - pe) + a.lo;
+    return q + r / d;
+}
+// (double)(a / b)
+__device__ inline double dd_div_dd(dd a, dd b) {
+    const double q = a.hi / b.hi;
+    const double p = q * b.hi;
+    const double pe = fma(q, b.hi, -p);
+    const double r = (((a.hi - p) - pe) + a.lo) - q * b.lo;
+    return q + r / b.hi;
+}
+
+// ---------------------------------------------------------------------------------
+// chi-square upper tail: regularised incomplete gamma Q(dof/2, x/2), series below
+// a + 1 and Lentz's continued fraction above (the reference's pchisq_upper,
+// src/core/internal/pchisq.hpp:16-70, follows the same textbook construction).
+// ---------------------------------------------------------------------------------
+__device__ double gamma_pref(double a, double x) { return exp(-x + a * log(x) - lgamma(a)); }
+
+__device__ double chisq_upper(double x, int dof) {
+    if (dof <= 0) return __builtin_nan("");
+    if (x <= 0.0) return 1.0;
+    const double a = 0.5 * dof, xx = 0.5 * x;
+    if (xx < a + 1.0) {
+        double ap = a, sum = 1.0 / a, del = sum;
+        for (int n = 0; n < 1000; ++n) {
+            ap += 1.0;
+            del *= xx / ap;
+            sum += del;
+            if (fabs(del) < fabs(sum) * 1e-15) break;
+        }
+        return 1.0 - sum * gamma_pref(a, xx);
+    }
+    const double tiny = 1e-300;
+    double b = xx + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
+    for (int i = 1; i <= 1000; ++i) {
+        const double an = -static_cast<double>(i) * (static_cast<double>(i) - a);
+        b += 2.0;
+        d = an * d + b;
+        if (fabs(d) < tiny) d = tiny;
+        c = b + an / c;
+        if (fabs(c) < tiny) c = tiny;
+        d = 1.0 / d;
+        const double del = d * c;
+        h *= del;
+        if (fabs(del - 1.0) < 1e-15) break;
+    }
+    return gamma_pref(a, xx) * h;
+}
+
+__device__ inline int dof_of(int nl, int nr, int r) { return (nl - r) * (nr - r); }
+
+// ---------------------------------------------------------------------------------
+// LU solve of an n x n system held one row per lane (lane a < n: row[0..n) of row a,
+// right-hand side b).  Partial pivoting with the reference's pivot choice; rows keep
+// their lanes and carry a logical position.  Returns false for an exactly zero pivot
+// (the reference's singular test); otherwise x[a] is returned in lane a.
+// ---------------------------------------------------------------------------------
+template <int T>
+__device__ __forceinline__ bool lu_solve_rows(double (&row)[T], double b, int n, double& x_out, double* prow) {
+    const int lane = lane_id();
+    const bool act = lane < n;
+    int pos = act ? lane : INT_MAX;
+    bool ok = true;
+    // Straight-line code over compile-time column indices keeps row[] in registers.
+    static_for<T>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if (k < n && ok) {
+            const double cand = (act && pos >= k) ? fabs(row[k]) : -1.0;
+            const int pl = argmax_pivot(cand, pos);
+            const double pv = rl(row[k], pl);
+            if (pv == 0.0) {
+                ok = false;
+            } else {
+                const int lk = first_lane(pos == k);
+                const int ppl = rli(pos, pl);
+                if (lane == pl) pos = k;
+                else if (lane == lk) pos = ppl;
+                const double rp = 1.0 / pv;
+                // the pivot row goes through LDS: one writer lane, broadcast reads
+                if (lane == pl) {
+#pragma unroll
+                    for (int j = k + 1; j < T; ++j)
+                        if (j < n) prow[j] = row[j];
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (act && pos > k) {
+                    const double f = row[k] * rp;
+                    row[k] = f;
+#pragma unroll
+                    for (int j = k + 1; j < T; ++j)
+                        if (j < n) row[j] -= f * prow[j];
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    });
+    if (!ok) return false;
+    // forward substitution (unit lower), the right-hand side moves with its row
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+        if (j < n) {
+            const double yj = rl(b, first_lane(pos == j));
+            if (act && pos > j) b -= row[j] * yj;
+        }
+    }
+    // back substitution, column by column
+    double x = 0.0;
+#pragma unroll
+    for (int ii = T - 1; ii >= 0; --ii) {
+        if (ii < n) {
+            const int li = first_lane(pos == ii);
+            const double xi = rl(b, li) / rl(row[ii], li);
+            if (act && pos < ii) b -= row[ii] * xi;
+            if (lane == ii) x = xi;
+        }
+    }
+    x_out = x;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------
+// per-wave LDS slice and one ALS fit
+// ---------------------------------------------------------------------------------
+struct WaveSlice {
+    double* x;    // xvec, k = i*nr + j of the fit's (reduced) model
+    double* y;    // Qsub^T x
+    double* a;    // A[i][p] at i*r + p
+    double* b;    // B[p][j] at p*nr + j
+    double* w;    // one W column / the residual
+    double* ws;   // Jacobi work matrix, (i, j) at i + nl*j
+    double* sig;  // singular values (nr)
+    double* prow; // LU pivot row broadcast (64)
+    double* v;    // Jacobi right vectors, (i, j) at i + nr*j
+    int* ord;     // singular value order (nr)
+};
+
+struct FitShape {
+    int nl, nr, r, drop;  // drop: full-model source removed (-1: none)
+};
+
+// Row of the full Qinv for reduced source index i.
+__device__ inline int src_of(int i, int drop) { return (drop >= 0 && i >= drop) ? i + 1 : i; }
+
+// Residual chi^2 = e^T Qsub e, e = x - (A B) (cpu_backend.cpp:2840-2860 ordering:
+// row sums over b, then the sum over a, both in extended precision).
+__device__ double resid_chisq(const double* Q, int ldq, FitShape s, const WaveSlice& W) {
+    const int lane = lane_id();
+    const int m = s.nl * s.nr;
+    if (lane < m) {
+        const int i = lane / s.nr, j = lane - (lane / s.nr) * s.nr;
+        double ab = 0.0;
+        for (int p = 0; p < s.r; ++p) ab += W.a[i * s.r + p] * W.b[p * s.nr + j];
+        W.w[lane] = W.x[lane] - ab;
+    }
+    __builtin_amdgcn_wave_barrier();
+    dd row{0.0, 0.0};
+    if (lane < m) {
+        const int ia = lane / s.nr, ja = lane - ia * s.nr;
+        const double* qrow = Q + (src_of(ia, s.drop) * s.nr + ja) * ldq;
+        for (int i = 0; i < s.nl; ++i) {
+            const double* qq = qrow + src_of(i, s.drop) * s.nr;
+            for (int j = 0; j < s.nr; ++j) row = dd_fma(row, qq[j], W.w[i * s.nr + j]);
+        }
+    }
+    const double e_own = lane < m ? W.w[lane] : 0.0;
+    dd acc{0.0, 0.0};
+    for (int a = 0; a < m; ++a) {
+        const double ea = rl(e_own, a);
+        const dd ra{rl(row.hi, a), rl(row.lo, a)};
+        acc = dd_fma_dd(acc, ea, ra);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return uni(acc.hi + acc.lo);
+}
+
+// The ALS fit of one (reduced) model at rank r on W.x (cpu_backend.cpp:2727-2935).
+// Returns ok; writes the weights (nl) to w_out (LDS) and the chi^2.
+template <int T>
+__device__ __noinline__ bool als_fit(const double* Q, int ldq, FitShape s, int iters, double fudge, const WaveSlice& W,
+                        double* w_out, double& chisq) {
+    const int lane = lane_id();
+    const int nl = s.nl, nr = s.nr, r = s.r, m = nl * nr;
+    chisq = 0.0;
+    if (r == 0) {
+        if (lane < nl) w_out[lane] = 1.0;
+        chisq = resid_chisq(Q, ldq, s, W);
+        return true;
+    }
+    // y = Qsub^T x (column kc of Qsub against x, k ascending)
+    if (lane < m) {
+        const int ic = lane / nr, jc = lane - (lane / nr) * nr;
+        const int col = src_of(ic, s.drop) * nr + jc;
+        double acc = 0.0;
+        for (int i = 0; i < nl; ++i) {
+            const int rb = src_of(i, s.drop) * nr;
+            for (int j = 0; j < nr; ++j) acc += Q[(rb + j) * ldq + col] * W.x[i * nr + j];
+        }
+        W.y[lane] = acc;
+    }
+    // ---- seed: one-sided Jacobi SVD of xm (nl x nr) on its columns ----
+    if (lane < m) {
+        const int i = lane / nr, j = lane - (lane / nr) * nr;
+        W.ws[i + nl * j] = W.x[lane];
+    }
+    for (int e = lane; e < nr * nr; e += 64) W.v[e] = (e % nr == e / nr) ? 1.0 : 0.0;
+    __builtin_amdgcn_wave_barrier();
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p + 1 < nr; ++p)
+            for (int q = p + 1; q < nr; ++q) {
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int i = 0; i < nl; ++i) {
+                    const double wp = W.ws[i + nl * p], wq = W.ws[i + nl * q];
+                    al += wp * wp;
+                    be += wq * wq;
+                    ga += wp * wq;
+                }
+                al = uni(al);
+                be = uni(be);
+                ga = uni(ga);
+                off += ga * ga;
+                if (fabs(ga) <= 1e-15 * sqrt(al * be) || ga == 0.0) continue;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t);
+                const double sn = c * t;
+                if (lane < nl) {
+                    const double x0 = W.ws[lane + nl * p], y0 = W.ws[lane + nl * q];
+                    W.ws[lane + nl * p] = c * x0 - sn * y0;
+                    W.ws[lane + nl * q] = sn * x0 + c * y0;
+                }
+                if (lane < nr) {
+                    const double x0 = W.v[lane + nr * p], y0 = W.v[lane + nr * q];
+                    W.v[lane + nr * p] = c * x0 - sn * y0;
+                    W.v[lane + nr * q] = sn * x0 + c * y0;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        if (uni(off) < 1e-30) break;
+    }
+    if (lane < nr) {
+        double ss = 0.0;
+        for (int i = 0; i < nl; ++i) ss += W.ws[i + nl * lane] * W.ws[i + nl * lane];
+        W.sig[lane] = sqrt(ss);
+        W.ord[lane] = lane;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // The reference's exchange sort on descending sigma; positions 0..r-1 are final
+    // after r outer passes.  Every lane runs it on the same LDS values.
+    for (int i = 0; i < r; ++i)
+        for (int j = i + 1; j < nr; ++j) {
+            const int oi = W.ord[i], oj = W.ord[j];
+            if (W.sig[oj] > W.sig[oi]) {
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) {
+                    W.ord[i] = oj;
+                    W.ord[j] = oi;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane; k < r * nr; k += 64) {
+        const int p = k / nr, j = k - (k / nr) * nr;
+        W.b[k] = W.v[j + nr * W.ord[p]];
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < nl * r) {
+        const int i = lane / r, p = lane - (lane / r) * r;
+        double acc = 0.0;
+        for (int j = 0; j < nr; ++j) acc += W.x[i * nr + j] * W.b[p * nr + j];
+        W.a[lane] = acc;
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // lane -> its row of Qsub (full-matrix row index) for the W columns
+    int qrow = 0;
+    if (lane < m) {
+        const int i = lane / nr, j = lane - (lane / nr) * nr;
+        qrow = src_of(i, s.drop) * nr + j;
+    }
+    const double* Qr = Q + qrow * ldq;
+
+    for (int it = 0; it < iters; ++it) {
+        // step 0, the A-step: unknowns A[i][p] at a = i*r + p (L(k=(i',j), a) = [i'==i] B[p][j]);
+        // step 1, the B-step: unknowns B[p][j] at a = p*nr + j (L(k=(i,j'), a) = [j'==j] A[i][p]).
+        for (int step = 0; step < 2; ++step) {
+            const bool bstep = step == 1;
+            const int t = bstep ? r * nr : nl * r;
+            // lane a's (outer, inner) unknown indices: A-step (i, p), B-step (p, j)
+            const int div = bstep ? nr : r;
+            const int ua = lane / div, va = lane - (lane / div) * div;
+            double row[T];
+            double diag = 0.0;
+            int uc = 0, vc = 0;
+            static_for<T>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                row[c] = 0.0;
+                if (c < t) {
+                    // W[:, c] = Qsub L[:, c]
+                    if (lane < m) {
+                        double acc = 0.0;
+                        if (!bstep) {
+                            const double* qq = Qr + src_of(uc, s.drop) * nr;
+                            const double* bb = W.b + vc * nr;
+                            for (int j = 0; j < nr; ++j) acc += qq[j] * bb[j];
+                        } else {
+                            for (int i = 0; i < nl; ++i) acc += Qr[src_of(i, s.drop) * nr + vc] * W.a[i * r + uc];
+                        }
+                        W.w[lane] = acc;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    // C[a][c] = L[:, a]^T W[:, c]
+                    double cv = 0.0;
+                    if (lane < t) {
+                        if (!bstep) {
+                            const double* bb = W.b + va * nr;
+                            const double* ww = W.w + ua * nr;
+                            for (int j = 0; j < nr; ++j) cv += bb[j] * ww[j];
+                        } else {
+                            for (int i = 0; i < nl; ++i) cv += W.a[i * r + ua] * W.w[i * nr + va];
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    row[c] = cv;
+                    if (lane == c) diag = cv;
+                    if (++vc == div) { vc = 0; ++uc; }
+                }
+            });
+            double tr = 0.0;
+            for (int a = 0; a < t; ++a) tr += rl(diag, a);
+            const double ft = fudge * tr;
+            static_for<T>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                if (c < t && lane == c) row[c] += ft;
+            });
+            double rhs = 0.0;
+            if (lane < t) {
+                if (!bstep) {
+                    const double* bb = W.b + va * nr;
+                    const double* yy = W.y + ua * nr;
+                    for (int j = 0; j < nr; ++j) rhs += bb[j] * yy[j];
+                } else {
+                    for (int i = 0; i < nl; ++i) rhs += W.a[i * r + ua] * W.y[i * nr + va];
+                }
+            }
+            double sol = 0.0;
+            const bool ok = lu_solve_rows<T>(row, rhs, t, sol, W.prow);
+            double* dst = bstep ? W.b : W.a;
+            if (lane < t) dst[lane] = ok ? sol : 0.0;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    // ---- weights: (Xa Xa^T) w = Xa[:, r], Xa = [A, 1], normalised to sum 1 ----
+    {
+        double row[kMaxNl];
+#pragma unroll
+        for (int i2 = 0; i2 < kMaxNl; ++i2) {
+            double acc = 0.0;
+            if (lane < nl && i2 < nl) {
+                for (int p = 0; p < r; ++p) acc += W.a[lane * r + p] * W.a[i2 * r + p];
+                acc += 1.0 * 1.0;
+            }
+            row[i2] = acc;
+        }
+        double wv = 0.0;
+        if (!lu_solve_rows<kMaxNl>(row, 1.0, nl, wv, W.prow)) return false;
+        double sum = 0.0;
+        for (int i = 0; i < nl; ++i) sum += rl(wv, i);
+        if (lane < nl) w_out[lane] = wv / sum;
+    }
+    __builtin_amdgcn_wave_barrier();
+    chisq = resid_chisq(Q, ldq, s, W);
+    return true;
+}
+
+// ---------------------------------------------------------------------------------
+// LU factorisation of the m x m matrix in LDS (row i at lu + i*ld) by one wavefront,
+// lane = row; pos[lane] and lane_of[pos] record the pivoting.  Returns false on an
+// exactly zero pivot.
+// ---------------------------------------------------------------------------------
+__device__ bool lds_lu(double* lu, int ld, int m, int* pos_of_lane, int* lane_of_pos) {
+    const int lane = lane_id();
+    const bool act = lane < m;
+    int pos = act ? lane : INT_MAX;
+    double* my = lu + (act ? lane : 0) * ld;
+    for (int k = 0; k < m; ++k) {
+        const double cand = (act && pos >= k) ? fabs(my[k]) : -1.0;
+        const int pl = argmax_pivot(cand, pos);
+        const double pv = uni(lu[pl * ld + k]);
+        if (pv == 0.0) return false;
+        const int lk = first_lane(pos == k);
+        const int ppl = rli(pos, pl);
+        if (lane == pl) pos = k;
+        else if (lane == lk) pos = ppl;
+        const double rp = 1.0 / pv;
+        const double* prow = lu + pl * ld;
+        if (act && pos > k) {
+            const double f = my[k] * rp;
+            my[k] = f;
+            for (int j = k + 1; j < m; ++j) my[j] -= f * prow[j];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (act) {
+        pos_of_lane[lane] = pos;
+        lane_of_pos[pos] = lane;
+    }
+    return true;
+}
+
+// Column c of the inverse from the LDS LU factors: solves LU x = P e_c with the row
+// right-hand sides carried by their lanes; x[i] lands in lane i.
+__device__ double lds_inverse_column(const double* lu, int ld, int m, const int* pos_of_lane,
+                                     const int* lane_of_pos, int c) {
+    const int lane = lane_id();
+    const bool act = lane < m;
+    const int pos = act ? pos_of_lane[lane] : INT_MAX;
+    const double* my = lu + (act ? lane : 0) * ld;
+    double y = (lane == c) ? 1.0 : 0.0;
+    for (int j = 0; j < m; ++j) {
+        const double yj = rl(y, lane_of_pos[j]);
+        if (act && pos > j) y -= my[j] * yj;
+    }
+    double x = 0.0;
+    for (int i = m - 1; i >= 0; --i) {
+        const int li = lane_of_pos[i];
+        const double xi = rl(y, li) / lu[li * ld + i];
+        if (act && pos < i) y -= my[i] * xi;
+        if (lane == i) x = xi;
+    }
+    return x;
+}
+
+__device__ bool feasible_w(const double* w, int n) {
+    bool any = false;
+    for (int i = 0; i < n; ++i) {
+        const double v = w[i];
+        if (v != v) continue;
+        any = true;
+        if (v < 0.0 || v > 1.0) return false;
+    }
+    return any;
+}
+
+// ---------------------------------------------------------------------------------
+// the batched kernel
+// ---------------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(kQThreads) void qpadm_kernel(QpadmArgs g) {
+    extern __shared__ __attribute__((aligned(16))) double lds[];
+    __shared__ int s_model, s_status, s_next, s_nfit;
+    __shared__ int s_L[kMaxNl + 1];
+    __shared__ int s_R[kMaxQpadmNr + 1];
+    __shared__ int s_pos[kMaxQpadmM], s_lane_of[kMaxQpadmM];
+    __shared__ signed char s_fit_row[kMaxRows * kMaxRanks + 2], s_fit_rank[kMaxRows * kMaxRanks + 2];
+    __shared__ double s_chisq[kMaxRows][kMaxRanks];
+    __shared__ double s_wrf[kMaxRows][kMaxNl];  // weights of each popdrop row's fit at rank nlr-1
+    __shared__ double s_wmain[kMaxNl];          // weights of the main fit when its rank differs
+    __shared__ int s_ok[kMaxRows][kMaxRanks];
+    __shared__ double s_xtot[kMaxQpadmM], s_totline[kMaxQpadmM];
+    __shared__ double s_prank[kMaxRanks];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ldq = g.ldq;
+    double* const sQi = lds;                                   // Qinv, m x ldq
+    double* const sLU = lds + g.m_max * ldq;                   // Qf / its LU (setup only)
+    double* const wbase = sLU;                                 // per-wave slices (fit phase)
+    const int wstride = g.wave_slice;
+    WaveSlice W;
+    {
+        double* p = wbase + wave * wstride;
+        W.x = p;
+        W.y = p + 64;
+        W.a = p + 128;
+        W.b = p + 192;
+        W.w = p + 256;
+        W.ws = p + 320;
+        W.sig = p + 384;
+        W.prow = p + 416;
+        W.v = p + 480;
+        W.ord = reinterpret_cast<int*>(p + 480 + g.nr_max * g.nr_max);
+    }
+    double* const scr = g.scratch + static_cast<std::int64_t>(blockIdx.x) * g.scratch_stride;
+    const int nbs = g.nbs;
+    double* const sX = scr;                                    // X, then tau: [m][nbs]
+    double* const sLOO = scr + static_cast<std::int64_t>(g.m_max) * nbs;      // [m][nbs]
+    double* const sWM = sLOO + static_cast<std::int64_t>(g.m_max) * nbs;      // [nbs][nl]
+    const double n_tot = g.n_total;
+
+    for (;;) {
+        if (tid == 0) s_model = static_cast<int>(atomicAdd(g.counter, 1u));
+        __syncthreads();
+        const int slot = __builtin_amdgcn_readfirstlane(s_model);
+        __syncthreads();
+        if (slot >= g.n_order) break;
+        const int mi = g.order[slot];
+        const int l0 = g.left_off[mi], l1 = g.left_off[mi + 1];
+        const int r0 = g.right_off[mi], r1 = g.right_off[mi + 1];
+        const int nl = l1 - l0, nr = r1 - r0 - 1, m = nl * nr;
+        const int rank = g.rank < 0 ? nl - 1 : g.rank;
+        if (tid == 0) s_L[0] = g.target[mi];
+        if (tid < nl) s_L[tid + 1] = g.left[l0 + tid];
+        if (tid <= nr) s_R[tid] = g.right[r0 + tid];
+        __syncthreads();
+        stp_qpadm_result* out = g.out + mi;
+
+        // ---- f4 basis over the surviving blocks ----
+        {
+            const int P = g.P;
+            const std::int64_t ld2 = nbs;
+            const int L0 = s_L[0], R0 = s_R[0];
+            const double* f00 = g.f2T + (L0 + static_cast<std::int64_t>(P) * R0) * ld2;
+            for (int k = wave; k < m; k += kQWaves) {
+                const int i = k / nr, j = k - (k / nr) * nr;
+                const int Li = s_L[i + 1], Rj = s_R[j + 1];
+                const double* fi0 = g.f2T + (Li + static_cast<std::int64_t>(P) * R0) * ld2;
+                const double* f0j = g.f2T + (L0 + static_cast<std::int64_t>(P) * Rj) * ld2;
+                const double* fij = g.f2T + (Li + static_cast<std::int64_t>(P) * Rj) * ld2;
+                for (int b = lane; b < nbs; b += 64)
+                    sX[static_cast<std::int64_t>(k) * nbs + b] = 0.5 * (fi0[b] + f0j[b] - f00[b] - fij[b]);
+            }
+        }
+        __syncthreads();
+        // ---- LOO and total (one thread per f4 entry) ----
+        if (tid < m) {
+            const double* X = sX + static_cast<std::int64_t>(tid) * nbs;
+            double* LOO = sLOO + static_cast<std::int64_t>(tid) * nbs;
+            dd s{0.0, 0.0};
+            for (int b = 0; b < nbs; ++b) s = dd_fma(s, X[b], static_cast<double>(g.bsz[b]));
+            const double tot = dd_div_d(s, n_tot);
+            for (int b = 0; b < nbs; ++b) {
+                const double rel = static_cast<double>(g.bsz[b]) / n_tot;
+                LOO[b] = (tot - X[b] * rel) / (1.0 - rel);
+            }
+            dd wn{0.0, 0.0}, wd{0.0, 0.0};
+            for (int b = 0; b < nbs; ++b) {
+                const double w = 1.0 - static_cast<double>(g.bsz[b]) / n_tot;
+                wn = dd_fma(wn, LOO[b], w);
+                wd = dd_add(wd, w);
+            }
+            const double tl = dd_div_dd(wn, wd);
+            dd ds{0.0, 0.0};
+            for (int b = 0; b < nbs; ++b) ds = dd_add(dd_add(ds, tl), -LOO[b]);
+            const double term1 = dd_div_d(ds, static_cast<double>(nbs)) * static_cast<double>(nbs);
+            dd wb{0.0, 0.0};
+            for (int b = 0; b < nbs; ++b) wb = dd_fma(wb, LOO[b], static_cast<double>(g.bsz[b]));
+            s_totline[tid] = tl;
+            s_xtot[tid] = term1 + dd_div_d(wb, n_tot);
+        }
+        __syncthreads();
+        // ---- tau, overwriting X ----
+        for (int k = wave; k < m; k += kQWaves) {
+            const double xt = s_xtot[k], tl = s_totline[k];
+            for (int b = lane; b < nbs; b += 64) {
+                const double h = n_tot / static_cast<double>(g.bsz[b]);
+                const double sq = sqrt(h - 1.0);
+                const std::int64_t o = static_cast<std::int64_t>(k) * nbs + b;
+                sX[o] = (xt * h - sLOO[o] * (h - 1.0) - tl) / sq;
+            }
+        }
+        __syncthreads();
+        // ---- Q (upper triangle, mirrored) into sLU ----
+        for (int e = tid; e < m * m; e += kQThreads) {
+            const int a = e / m, c = e - (e / m) * m;
+            if (c < a) continue;
+            const double* ta = sX + static_cast<std::int64_t>(a) * nbs;
+            const double* tc = sX + static_cast<std::int64_t>(c) * nbs;
+            dd acc{0.0, 0.0};
+            for (int b = 0; b < nbs; ++b) acc = dd_fma(acc, ta[b], tc[b]);
+            const double v = dd_div_d(acc, static_cast<double>(nbs));
+            sLU[a * ldq + c] = v;
+            sLU[c * ldq + a] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double tr = 0.0;
+            for (int k = 0; k < m; ++k) tr += sLU[k * ldq + k];
+            for (int k = 0; k < m; ++k) sLU[k * ldq + k] += g.fudge * tr;
+            s_status = 0;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const bool ok = lds_lu(sLU, ldq, m, s_pos, s_lane_of);
+            if (lane == 0 && !ok) s_status = 3;
+        }
+        __syncthreads();
+        if (s_status != 0) {
+            if (tid == 0) {
+                out->status = 3;  // NonSpdCovariance
+                out->model_index = mi;
+                out->nl = 0;
+                out->est_rank = rank;
+                out->dof = dof_of(nl, nr, rank);
+                out->n_rank = 0;
+                out->n_popdrop = 0;
+            }
+            continue;
+        }
+        for (int c = wave; c < m; c += kQWaves) {
+            const double x = lds_inverse_column(sLU, ldq, m, s_pos, s_lane_of, c);
+            if (lane < m) sQi[lane * ldq + c] = x;
+        }
+        __syncthreads();
+
+        // ---- fit table: rank sweeps of the full model and of each popdrop row ----
+        const int nrows = nl >= 2 ? nl + 1 : 1;
+        if (tid == 0) {
+            int n = 0;
+            for (int k = 0; k < nrows; ++k) {
+                const int nlr = k == 0 ? nl : nl - 1;
+                const int rmax = min(nlr, nr) - 1, rf = nlr - 1;
+                for (int rr = 0; rr <= rmax; ++rr) {
+                    s_fit_row[n] = static_cast<signed char>(k);
+                    s_fit_rank[n++] = static_cast<signed char>(rr);
+                }
+                if (rf > rmax) {
+                    s_fit_row[n] = static_cast<signed char>(k);
+                    s_fit_rank[n++] = static_cast<signed char>(rf);
+                }
+                if (k == 0 && rank > rmax && rank != rf) {
+                    s_fit_row[n] = 0;
+                    s_fit_rank[n++] = static_cast<signed char>(rank);
+                }
+            }
+            s_nfit = n;
+            s_next = 0;
+        }
+        __syncthreads();
+        for (;;) {
+            int f = 0;
+            if (lane == 0) f = atomicAdd(&s_next, 1);
+            f = __builtin_amdgcn_readfirstlane(__shfl(f, 0));
+            if (f >= s_nfit) break;
+            const int k = s_fit_row[f], rr = s_fit_rank[f];
+            FitShape sh{k == 0 ? nl : nl - 1, nr, rr, k == 0 ? -1 : nl - k};
+            const int mr = sh.nl * nr;
+            if (lane < mr) {
+                const int i = lane / nr, j = lane - (lane / nr) * nr;
+                W.x[lane] = s_xtot[src_of(i, sh.drop) * nr + j];
+            }
+            __builtin_amdgcn_wave_barrier();
+            double chisq = 0.0;
+            const int rf = sh.nl - 1;
+            double* wout = rr == rf ? s_wrf[k] : (k == 0 && rr == rank ? s_wmain : W.sig);
+            const bool ok = als_fit<T>(sQi, ldq, sh, g.iters, g.fudge, W, wout, chisq);
+            if (lane == 0) {
+                s_chisq[k][rr] = ok ? chisq : 0.0;
+                s_ok[k][rr] = ok ? 1 : 0;
+            }
+        }
+        __syncthreads();
+        const bool main_ok = s_ok[0][rank] != 0;
+        const double* wmain = rank == nl - 1 ? s_wrf[0] : s_wmain;
+        const int dof = dof_of(nl, nr, rank);
+        bool do_se = false;
+        if (main_ok && nbs >= 2) {
+            if (g.jackknife == 2) do_se = true;
+            else if (g.jackknife == 1)
+                do_se = feasible_w(wmain, nl) &&
+                        (!g.se_require_p || chisq_upper(s_chisq[0][rank], dof) >= g.p_se_threshold);
+        }
+        if (do_se) {
+            if (tid == 0) s_next = 0;
+            __syncthreads();
+            for (;;) {
+                int b = 0;
+                if (lane == 0) b = atomicAdd(&s_next, 1);
+                b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
+                if (b >= nbs) break;
+                if (lane < m) W.x[lane] = sLOO[static_cast<std::int64_t>(lane) * nbs + b];
+                __builtin_amdgcn_wave_barrier();
+                FitShape sh{nl, nr, rank, -1};
+                double chisq = 0.0;
+                double* wrow = W.sig;  // nl weights; sig is free after the seed
+                const bool ok = als_fit<T>(sQi, ldq, sh, g.iters, g.fudge, W, wrow, chisq);
+                __builtin_amdgcn_wave_barrier();
+                if (lane < nl) sWM[static_cast<std::int64_t>(b) * nl + lane] = ok ? wrow[lane] : 0.0;
+            }
+            __syncthreads();
+        }
+
+        // ---- results (wave 0; lane 0 writes the record fields) ----
+        if (tid < 64) {
+            double se = 0.0;
+            if (do_se && lane < nl) {
+                const double scale = static_cast<double>(nbs - 1) / sqrt(static_cast<double>(nbs));
+                double mean = 0.0;
+                for (int b = 0; b < nbs; ++b) mean += sWM[static_cast<std::int64_t>(b) * nl + lane] * scale;
+                mean /= nbs;
+                dd acc{0.0, 0.0};
+                for (int b = 0; b < nbs; ++b) {
+                    const double d = sWM[static_cast<std::int64_t>(b) * nl + lane] * scale - mean;
+                    acc = dd_fma(acc, d, d);
+                }
+                se = sqrt(dd_div_d(acc, static_cast<double>(nbs - 1)));
+            }
+            const int rmax = min(nl, nr) - 1;
+            // p of each swept rank, one lane per rank
+            if (main_ok && lane <= rmax) s_prank[lane] = chisq_upper(s_chisq[0][lane], dof_of(nl, nr, lane));
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                stp_qpadm_result* o = out;
+                o->status = 0;
+                o->model_index = mi;
+                o->est_rank = rank;
+                o->dof = dof;
+                o->nl = 0;
+                o->n_rank = 0;
+                o->n_popdrop = 0;
+                o->f4rank = 0;
+                o->p = 0.0;
+                o->chisq = 0.0;
+                const bool lanes_write_se = main_ok && do_se;
+                for (int i = 0; i < kMaxNl; ++i) {
+                    o->weight[i] = 0.0;
+                    if (!lanes_write_se || i >= nl) {
+                        o->se[i] = 0.0;
+                        o->z[i] = 0.0;
+                    }
+                }
+                if (!main_ok) {
+                    o->status = 2;  // RankDeficient
+                } else {
+                    o->nl = nl;
+                    o->chisq = s_chisq[0][rank];
+                    o->p = chisq_upper(s_chisq[0][rank], dof);
+                    for (int i = 0; i < nl; ++i) o->weight[i] = wmain[i];
+                    // rank sweep of the full model
+                    o->n_rank = rmax + 1;
+                    int f4 = rmax;
+                    for (int rr = rmax; rr >= 0; --rr)
+                        if (s_prank[rr] > g.alpha) f4 = rr;
+                    o->f4rank = f4;
+                    for (int rr = 0; rr <= rmax; ++rr) {
+                        o->rank_chisq[rr] = s_chisq[0][rr];
+                        o->rank_dof[rr] = dof_of(nl, nr, rr);
+                    }
+                    for (int kk = 0; kk <= rmax; ++kk) {
+                        const int rr = rmax - kk;
+                        o->rankdrop_f4rank[kk] = rr;
+                        o->rankdrop_dof[kk] = dof_of(nl, nr, rr);
+                        o->rankdrop_chisq[kk] = s_chisq[0][rr];
+                        o->rankdrop_p[kk] = s_prank[rr];
+                        if (rr >= 1) {
+                            const int ddf = dof_of(nl, nr, rr - 1) - dof_of(nl, nr, rr);
+                            const double cd = s_chisq[0][rr - 1] - s_chisq[0][rr];
+                            o->rankdrop_dofdiff[kk] = ddf;
+                            o->rankdrop_chisqdiff[kk] = cd;
+                            o->rankdrop_p_nested[kk] = chisq_upper(cd, ddf);
+                        } else {
+                            o->rankdrop_dofdiff[kk] = INT_MIN;
+                            o->rankdrop_chisqdiff[kk] = __builtin_nan("");
+                            o->rankdrop_p_nested[kk] = __builtin_nan("");
+                        }
+                    }
+                    // popdrop rows: the full model, then each source dropped (last first)
+                    o->n_popdrop = nrows;
+                    for (int k = 0; k < nrows; ++k) {
+                        const int drop = k == 0 ? -1 : nl - k;
+                        const int nlr = k == 0 ? nl : nl - 1;
+                        const int rmx = min(nlr, nr) - 1, rf = nlr - 1;
+                        for (int c = 0; c < nl; ++c) o->popdrop_pat[k][c] = (c == drop) ? '1' : '0';
+                        o->popdrop_pat[k][nl] = '\0';
+                        o->popdrop_wt[k] = drop >= 0 ? 1 : 0;
+                        o->popdrop_f4rank[k] = rf;
+                        const int pdof = dof_of(nlr, nr, rf);
+                        o->popdrop_dof[k] = pdof;
+                        o->popdrop_chisq[k] = rf <= rmx ? s_chisq[k][rf] : 0.0;
+                        o->popdrop_p[k] = rf <= rmx ? chisq_upper(s_chisq[k][rf], pdof) : 0.0;
+                        // feasible: every weight of the row's fit in [0, 1] (the dropped
+                        // source is NaN and ignored); a failed fit is infeasible
+                        bool feas = s_ok[k][rf] != 0 && nlr > 0;
+                        if (feas)
+                            for (int ii = 0; ii < nlr; ++ii) {
+                                const double v = s_wrf[k][ii];
+                                if (!(v >= 0.0 && v <= 1.0)) feas = false;
+                            }
+                        o->popdrop_feasible[k] = feas ? 1 : 0;
+                    }
+                    o->status = dof <= 0 ? 4 : 0;  // ChisqUndefined
+                }
+            }
+            if (main_ok && do_se && lane < nl) {
+                // each lane owns its weight's SE and z (no other lane writes these fields)
+                out->se[lane] = se;
+                out->z[lane] = se > 0.0 ? wmain[lane] / se : 0.0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// survivors and the survivor-transposed tensor
+// ---------------------------------------------------------------------------------
+// keep[b] = !(some pair has vpair <= 0 and another has vpair > 0)
+// (the reference's survivor_blocks, cpu_backend.cpp:125-150).
+__global__ __launch_bounds__(256) void survivor_kernel(const double* __restrict__ vpair, std::int64_t slab,
+                                                       std::int32_t* __restrict__ keep) {
+    const std::int64_t b = blockIdx.x;
+    const double* v = vpair + b * slab;
+    int pres = 0, miss = 0;
+    for (std::int64_t e = threadIdx.x; e < slab; e += blockDim.x) {
+        if (v[e] > 0.0) pres = 1;
+        else miss = 1;
+    }
+    pres = __syncthreads_or(pres);
+    miss = __syncthreads_or(miss);
+    if (threadIdx.x == 0) keep[b] = (pres && miss) ? 0 : 1;
+}
+
+// f2T[pair * nbs + s] = f2[pair + slab * surv[s]] by 64 x 64 LDS tiles.
+__global__ __launch_bounds__(256) void transpose_survivors_kernel(const double* __restrict__ f2, std::int64_t slab,
+                                                                  const std::int32_t* __restrict__ surv, int nbs,
+                                                                  double* __restrict__ f2T) {
+    __shared__ double tile[64][65];
+    const std::int64_t p0 = static_cast<std::int64_t>(blockIdx.x) * 64;
+    const int s0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int sy = ty; sy < 64; sy += 4) {
+        const int s = s0 + sy;
+        const std::int64_t p = p0 + tx;
+        tile[sy][tx] = (s < nbs && p < slab) ? f2[p + slab * surv[s]] : 0.0;
+    }
+    __syncthreads();
+    for (int py = ty; py < 64; py += 4) {
+        const std::int64_t p = p0 + py;
+        const int s = s0 + tx;
+        if (p < slab && s < nbs) f2T[p * nbs + s] = tile[tx][py];
+    }
+}
+
+}  // namespace
+
+void launch_survivors(const double* vpair, std::int64_t slab, int n_block, std::int32_t* keep, hipStream_t st) {
+    if (n_block <= 0) return;
+    survivor_kernel<<<static_cast<unsigned>(n_block), 256, 0, st>>>(vpair, slab, keep);
+    STP_LAUNCH_CHECK();
+}
+
+void launch_transpose_survivors(const double* f2, std::int64_t slab, const std::int32_t* surv, int nbs, double* f2T,
+                                hipStream_t st) {
+    if (nbs <= 0 || slab <= 0) return;
+    const dim3 grid(static_cast<unsigned>(ceil_div(slab, 64)), static_cast<unsigned>(ceil_div(nbs, 64)));
+    transpose_survivors_kernel<<<grid, 256, 0, st>>>(f2, slab, surv, nbs, f2T);
+    STP_LAUNCH_CHECK();
+}
+
+std::size_t qpadm_lds_bytes(int m_max, int ldq, int wave_slice) {
+    const std::size_t a = static_cast<std::size_t>(m_max) * ldq;
+    const std::size_t b = std::max<std::size_t>(a, static_cast<std::size_t>(kQWaves) * wave_slice);
+    return (a + b) * sizeof(double);
+}
+
+int qpadm_wave_slice(int nr_max) { return 480 + nr_max * nr_max + (kMaxQpadmNr + 1) / 2 + 1; }
+
+void launch_qpadm(int T, const QpadmArgs& args, int n_workgroups, hipStream_t st) {
+    if (args.n_order <= 0) return;
+    const std::size_t lds = qpadm_lds_bytes(args.m_max, args.ldq, args.wave_slice);
+    const dim3 grid(static_cast<unsigned>(n_workgroups));
+    auto go = [&](auto kernel) {
+        if (lds > 64 * 1024)
+            STP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        kernel<<<grid, kQThreads, lds, st>>>(args);
+    };
+    switch (T) {
+        case 16: go(qpadm_kernel<16>); break;
+        case 32: go(qpadm_kernel<32>); break;
+        case 48: go(qpadm_kernel<48>); break;
+        case 64: go(qpadm_kernel<64>); break;
+        default: throw ArgError("qpadm: unsupported system size bucket " + std::to_string(T));
+    }
+    STP_LAUNCH_CHECK();
+}
+
+int qpadm_threads() { return kQThreads; }
+
+}  // namespace stp
