@@ -1,6 +1,8 @@
 """Benchmark: the f2 block-tensor extraction on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3] [--precision fp64|split_bf16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2] [--precision split_bf16|fp64]
+                    [--qpadm-models N] [--no-qpadm] [--no-cpu-baseline]
+    python bench.py --calibrate-copy      # a plain 1 GiB device copy, for counter calibration
 
 One step = one pass of the hot path over the synthetic panel already resident in
 HBM: decode the packed 2-bit genotypes to per-population counts, then build the
@@ -14,11 +16,21 @@ Binomial(2,p), 40% pseudo-haploid populations with Beta(2,2) missingness):
   c3: 1000 populations (sizes U{1..60}), 1.2M SNPs, blocks LogNormal(ln 1500, 0.6)
       clipped to [50, 8000]
 
+The default workload is c3 in the split-BF16 form (the metric's "split-BF16 vs
+FP64" configuration).
+
 Extra JSON fields: ``roofline`` for the dominant kernel (the f2 contraction),
-timed with HIP events on the engine's own stream; ``stages`` with every stage's
-average device time; ``cpu_baseline``: the reference CpuBackend (oracle/_ref,
-compiled from the reference sources) or the restated oracle, single-threaded, on
-a bounded sample of the same workload.
+timed with HIP events on the engine's own stream: achieved = k * F_alg / t, with
+F_alg = 8 W FLOP (4 MACs per pair-SNP) and k = 1 for the FP64 form, k = 8.25 for the
+split form (33 BF16 products per pair-SNP over 4 MACs); ``traffic`` is the HBM bytes
+per launch from the committed rocprofv3 counter summary of the same workload
+(profiles/), or null; ``stages`` with every stage's average device time;
+``cpu_baseline``: the reference CpuBackend (oracle/_ref, compiled from the reference
+sources) or the restated oracle, single-threaded, on a bounded sample of the same
+workload.  ``qpadm``: the batched qpAdm metric (fits/s) for 10,000 random models
+(1 target + 2-5 sources, 6-12 right populations, drawn without overlap) on the
+resident c2 tensor, with its own single-threaded reference CPU rate on a sample of
+the same models.
 """
 from __future__ import annotations
 
@@ -36,6 +48,9 @@ sys.path.insert(0, ROOT)
 
 SEED = 20261202
 MI355X_PEAK = {"hbm_GBs": 8000.0, "fp64_TFLOPs": 78.6, "bf16_TFLOPs": 2500.0}
+SPLIT_K = 33 * 2 / 8.0   # executed BF16 FLOP per algorithmic FLOP in the split form
+PROFILES = {("c3", "split_bf16"): ("profiles/f2_split_c3.md", "f2_split_kernel"),
+            ("c3", "fp64"): ("profiles/f2_fp64_c3.md", "f2_fp64_kernel")}
 METRIC = "f2 pop-pair·SNPs/sec at ≤1e-9 rel err (split-BF16 vs FP64); qpAdm fits/sec"
 
 
@@ -127,15 +142,120 @@ def cpu_baseline(packed, n_ind, bpr, offs, wp, target_s: float = 15.0):
                       f"Q/V/N, {dt:.1f} s single-threaded ({os.path.basename(lib.path)})"}
 
 
+def profiled_traffic(workload: str, precision: str):
+    """HBM bytes per launch of the f2 kernel from the committed counter summary
+    (the 'traffic GB' column of its row), or None."""
+    key = PROFILES.get((workload, precision))
+    if not key:
+        return None
+    path, kernel = key
+    try:
+        for line in open(os.path.join(ROOT, path)):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 6 and cells[0] == f"`{kernel}`":
+                return float(cells[5]) * 1e9
+    except OSError:
+        return None
+    return None
+
+
+def random_models(rng, P: int, count: int):
+    """1 target + 2-5 sources, 6-12 right populations, all distinct."""
+    models = []
+    for _ in range(count):
+        ns = int(rng.integers(2, 6))
+        nrt = int(rng.integers(6, 13))
+        pick = rng.choice(P, 1 + ns + nrt, replace=False)
+        models.append((int(pick[0]), [int(x) for x in pick[1:1 + ns]], [int(x) for x in pick[1 + ns:]]))
+    return models
+
+
+def qpadm_bench(ctx, n_models: int, cpu: bool):
+    """C4: n_models random models on the resident c2 tensor in one batched call."""
+    import torch
+    from steppe_amd import _core
+    wp = workload_params("c2")
+    P, M = wp["P"], wp["M"]
+    bs = wp["block_sizes"]
+    nb = len(bs)
+    block_id = np.ascontiguousarray(np.repeat(np.arange(nb, dtype=np.int32), bs))
+    packed, n_ind, bpr, offs = build_panel(ctx, wp)
+    tile, keep = _core.make_tile(None, M, offs, None, detect=True, bytes_per_record=bpr,
+                                 n_individuals=n_ind, device_ptr=packed.data_ptr())
+    raw = C.c_void_p()
+    _core.check(_core.lib().stp_extract_f2(ctx.h, C.byref(tile), _core._ptr(block_id), nb,
+                                           _core.STP_PREC_FP64, C.byref(raw)), "extract")
+    h = _core.F2Handle(raw, [f"p{i}" for i in range(P)], 0)
+    del packed
+    torch.cuda.empty_cache()
+    models = random_models(np.random.default_rng(SEED + 4), P, n_models)
+    _core.qpadm_models(h, models[:64], raw=True)          # warm-up (code load, buffers)
+    ctx.timing(True)
+    ctx.read_timing(reset=True)
+    t0 = time.perf_counter()
+    recs = _core.qpadm_models(h, models, raw=True)
+    dt = time.perf_counter() - t0
+    tm = ctx.read_timing(reset=True)
+    ctx.timing(False)
+    ok = sum(1 for r in recs if r.status in (0, 4))
+    log(f"[bench] qpadm: {n_models} models in {dt:.3f} s ({ok} ok); kernel {tm['qpadm']}")
+    out = {"metric": "qpAdm fits/sec", "value": n_models / dt, "unit": "fits/s", "models": n_models,
+           "seconds": dt, "kernel_ms": tm["qpadm"][0],
+           "workload": f"c4: {n_models} random models (1 target + 2-5 sources, 6-12 right) on the "
+                       f"resident c2 tensor ({P} populations, {nb} blocks, FP64), weights + p + "
+                       f"rank sweep + popdrop + LOO SEs per model",
+           "cpu_baseline": None}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        lib = oracle_lib.ref()
+        kind = "reference"
+        if lib is None:
+            lib, kind = oracle_lib.cref(), "port"
+        f2h = h._f2_numpy()
+        vph = h._vpair_numpy()
+        bsz = np.asarray(h.block_sizes, np.int32)
+        t0 = time.perf_counter()
+        n = 0
+        while n < len(models) and time.perf_counter() - t0 < 10.0:
+            t, l, r = models[n]
+            lib.qpadm(f2h, vph, bsz, t, l, r)
+            n += 1
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / cdt, "unit": "fits/s", "cores": 1, "kind": kind,
+                               "sample": f"the first {n} of the same models, {cdt:.1f} s single-threaded "
+                                         f"({os.path.basename(lib.path)})"}
+        log(f"[bench] qpadm cpu baseline {out['cpu_baseline']}")
+    return out
+
+
+def calibrate_copy():
+    """A plain 1 GiB device-to-device copy, three times (counter calibration)."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "split_bf16"])
+    ap.add_argument("--workload", default="c3", choices=["c2", "c3"])
+    ap.add_argument("--precision", default="split_bf16", choices=["fp64", "split_bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--qpadm-models", type=int, default=10_000)
+    ap.add_argument("--no-qpadm", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "design-only (block-range shards), --gpus must be 1"}))
@@ -198,12 +318,16 @@ def main():
     # Algorithmic work of the contraction: 4 MACs per (pair, SNP) = 8 W FLOP.
     f_alg = 8.0 * W
     f2_s = avg["f2"] / 1e3
-    peak = MI355X_PEAK["fp64_TFLOPs"] if prec == _core.STP_PREC_FP64 else MI355X_PEAK["bf16_TFLOPs"]
-    achieved = f_alg / f2_s / 1e12 if f2_s > 0 else 0.0
-    roofline = {"kernel": "f2_fp64_kernel" if prec == _core.STP_PREC_FP64 else "f2_split_kernel",
+    fp64 = prec == _core.STP_PREC_FP64
+    peak = MI355X_PEAK["fp64_TFLOPs"] if fp64 else MI355X_PEAK["bf16_TFLOPs"]
+    kf = 1.0 if fp64 else SPLIT_K
+    achieved = kf * f_alg / f2_s / 1e12 if f2_s > 0 else 0.0
+    roofline = {"kernel": "f2_fp64_kernel" if fp64 else "f2_split_kernel",
                 "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
-                "work": "8*W FLOP per launch, W = P(P+1)/2*M"}
+                "frac": achieved / peak, "traffic": profiled_traffic(args.workload, args.precision),
+                "k": kf, "algorithmic_TFLOPs": f_alg / f2_s / 1e12 if f2_s > 0 else 0.0,
+                "work": ("8*W FLOP per launch, W = P(P+1)/2*M" if fp64 else
+                         "k*8*W FLOP per launch (k = 8.25 BF16 FLOP per algorithmic FLOP), W = P(P+1)/2*M")}
     dec_bytes = n_ind * bpr + 4 * P * M
     stages = {k: {"avg_ms": avg[k]} for k in avg}
     stages["decode"]["hbm_GBs"] = dec_bytes / (avg["decode"] / 1e3) / 1e9 if avg["decode"] else None
@@ -214,6 +338,11 @@ def main():
         log("[bench] cpu baseline ...")
         cpu = cpu_baseline(packed, n_ind, bpr, offs, wp)
         log(f"[bench] cpu baseline {cpu}")
+    qp = None
+    if not args.no_qpadm:
+        del packed, a2, nn, f2, vp
+        torch.cuda.empty_cache()
+        qp = qpadm_bench(ctx, args.qpadm_models, not args.no_cpu_baseline)
 
     out = {
         "metric": METRIC, "value": value, "unit": "pop-pair·SNPs/s", "n_gpus": 1,
@@ -225,7 +354,7 @@ def main():
                                f"{nb} jackknife blocks, decode + f2 ({args.precision})",
                    "P": P, "M": M, "n_block": nb, "n_individuals": n_ind,
                    "precision": args.precision, "parallelism": "single GPU"},
-        "roofline": roofline, "stages": stages, "cpu_baseline": cpu,
+        "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp,
     }
     print(json.dumps(out), flush=True)
 

@@ -11,9 +11,12 @@ from ._core import (  # noqa: F401
     PRECISION_NAMES,
     STP_PREC_FP64,
     STP_PREC_SPLIT_BF16,
+    qpadm_models,
     read_f2,
     read_f2_host,
     run_extract_f2,
+    run_qpadm,
+    run_qpadm_search,
     upload_f2,
     write_f2_dir,
 )
@@ -27,3 +30,13 @@ def extract_f2(packed, n_snp, pop_offsets, block_id, n_block, pops=None, out="",
     return run_extract_f2(packed, n_snp, pop_offsets, block_id, n_block, pops=pops, out=out,
                           device=device, ploidy=ploidy, precision=precision,
                           sample_ploidy=sample_ploidy)
+
+
+def qpadm(f2, target, left, right, **kw):
+    """Facade over ``_core.run_qpadm`` (bindings/steppe/__init__.py qpadm)."""
+    return run_qpadm(f2, target, left, right, **kw)
+
+
+def qpadm_search(f2, target, lefts, right, **kw):
+    """Facade over ``_core.run_qpadm_search``."""
+    return run_qpadm_search(f2, target, lefts, right, **kw)

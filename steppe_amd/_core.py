@@ -100,6 +100,17 @@ class QpAdmResult(C.Structure):
         return d
 
 
+class QpAdmOptions(C.Structure):
+    """stp_qpadm_options (include/steppe_hip.h); reference: steppe::QpAdmOptions."""
+    _fields_ = [("fudge", C.c_double), ("als_iterations", C.c_int32), ("rank", C.c_int32),
+                ("rank_alpha", C.c_double), ("jackknife", C.c_int32), ("se_require_p", C.c_int32),
+                ("p_se_threshold", C.c_double)]
+
+
+# The reference's jackknife policy names (bindings/internal/bind_common.hpp:168-178).
+JACKKNIFE = {"none": 0, "feasible": 1, "feasible_only": 1, "all": 2}
+
+
 _lib = None
 
 
@@ -134,6 +145,8 @@ def lib():
         "stp_f2bin_write": (C.c_int, [C.c_char_p, vp, vp, vp, i32, i32]),
         "stp_f2bin_read_header": (C.c_int, [C.c_char_p, C.POINTER(i32), C.POINTER(i32)]),
         "stp_f2bin_read": (C.c_int, [C.c_char_p, vp, vp, vp]),
+        "stp_qpadm_batch": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, C.POINTER(QpAdmOptions),
+                                      C.POINTER(QpAdmResult)]),
         "stpx_timing": (C.c_int, [vp, i32]),
         "stpx_timing_read": (C.c_int, [vp, vp, vp, i32]),
         "stpx_synth_panel": (C.c_int, [vp, C.c_uint64, i32, i64, i64, i64, vp, vp, vp, vp, vp]),
@@ -440,3 +453,87 @@ def run_extract_f2(packed, n_snp: int, pop_offsets, block_id, n_block: int, pops
                       "n_snp_kept": int(n_snp)})
         return out
     return h
+
+
+# ---------------------------------------------------------------------------------------
+# qpAdm on a resident f2 handle (bindings/bind_qpadm.cpp:14-134)
+# ---------------------------------------------------------------------------------------
+def _resolve(f2: F2Handle, name, what: str) -> int:
+    """A population name (or index) -> index; unknown names raise KeyError, as the
+    reference's PopResolver does (bindings/bind_qpadm.cpp:24-31)."""
+    if isinstance(name, (int, np.integer)):
+        i = int(name)
+        if not 0 <= i < f2.P:
+            raise KeyError(f"{what}: index {i} out of range")
+        return i
+    try:
+        return f2.pops.index(str(name))
+    except ValueError:
+        raise KeyError(f"{what}: unknown population {name!r}") from None
+
+
+def _options(fudge, als_iterations, rank, rank_alpha, jackknife, se_require_p=False,
+             p_se_threshold=0.05) -> QpAdmOptions:
+    if isinstance(jackknife, str):
+        if jackknife not in JACKKNIFE:
+            raise ValueError(f"jackknife must be one of {sorted(JACKKNIFE)}")
+        jackknife = JACKKNIFE[jackknife]
+    return QpAdmOptions(float(fudge), int(als_iterations), int(rank), float(rank_alpha), int(jackknife),
+                        1 if se_require_p else 0, float(p_se_threshold))
+
+
+def qpadm_models(f2: F2Handle, models, rank=-1, fudge=1e-4, als_iterations=20, rank_alpha=0.05,
+                 jackknife="all", se_require_p=False, p_se_threshold=0.05, raw=False):
+    """Fit a batch of (target, left, right) models in one device call; results in
+    input order (the reference's fit_models_batched, src/device/backend.hpp:1636-1647).
+    ``raw=True`` returns the QpAdmResult records instead of dicts."""
+    tg, loff, lf, roff, rt = [], [0], [], [0], []
+    for i, (target, left, right) in enumerate(models):
+        if not left:
+            raise ValueError(f"qpadm: model {i} has an empty left source list")
+        if not right:
+            raise ValueError(f"qpadm: model {i} needs at least one right population")
+        tg.append(_resolve(f2, target, "target"))
+        lf.extend(_resolve(f2, x, "left") for x in left)
+        rt.extend(_resolve(f2, x, "right") for x in right)
+        loff.append(len(lf))
+        roff.append(len(rt))
+    n = len(tg)
+    out = (QpAdmResult * max(n, 1))()
+    if n:
+        arr = [np.ascontiguousarray(a, np.int32) for a in (tg, loff, lf, roff, rt)]
+        opts = _options(fudge, als_iterations, rank, rank_alpha, jackknife, se_require_p, p_se_threshold)
+        ctx = Context.get(f2.device)
+        check(lib().stp_qpadm_batch(ctx.h, f2.raw, n, *[_ptr(a) for a in arr], C.byref(opts), out), "qpadm")
+    recs = list(out)[:n]
+    return recs if raw else [r.to_dict() for r in recs]
+
+
+def run_qpadm(f2: F2Handle, target, left, right, rank=-1, fudge=1e-4, als_iterations=20, rank_alpha=0.05,
+              allow_negative_weights=True) -> dict:
+    """Single-model qpAdm (bindings/bind_qpadm.cpp:14-42); SEs for every model."""
+    if target is None or target == "":
+        raise ValueError("qpadm: target is required")
+    if not left:
+        raise ValueError("qpadm: left needs at least one source population")
+    if not right:
+        raise ValueError("qpadm: right needs at least one outgroup population")
+    return qpadm_models(f2, [(target, list(left), list(right))], rank=rank, fudge=fudge,
+                        als_iterations=als_iterations, rank_alpha=rank_alpha, jackknife="all")[0]
+
+
+def run_qpadm_search(f2: F2Handle, target, lefts, right, rank=-1, fudge=1e-4, als_iterations=20,
+                     rank_alpha=0.05, allow_negative_weights=True, jackknife="all") -> list:
+    """Batched qpAdm over left-source sets with a shared target and right set
+    (bindings/bind_qpadm.cpp:72-113); per-model dicts in input order."""
+    if target is None or target == "":
+        raise ValueError("qpadm_search: target is required")
+    if not lefts:
+        raise ValueError("qpadm_search: models needs at least one (left) entry")
+    if not right:
+        raise ValueError("qpadm_search: right needs at least one outgroup population")
+    for i, l in enumerate(lefts):
+        if not l:
+            raise ValueError(f"qpadm_search: model {i} has an empty left source list")
+    return qpadm_models(f2, [(target, list(l), list(right)) for l in lefts], rank=rank, fudge=fudge,
+                        als_iterations=als_iterations, rank_alpha=rank_alpha, jackknife=jackknife)
