@@ -162,6 +162,58 @@ int stp_qpadm_batch(stp_context* ctx, const stp_f2_handle* f2, int32_t n_models,
                     const int32_t* left_offsets, const int32_t* left, const int32_t* right_offsets,
                     const int32_t* right, const stp_qpadm_options* options, stp_qpadm_result* out);
 
+/* ---- qpGraph ------------------------------------------------------------------------ */
+
+typedef struct stp_qpgraph_topology stp_qpgraph_topology;
+typedef struct stp_qpgraph_set stp_qpgraph_set;
+
+/* Parse a parent->child edge list into an admixture graph whose leaves are named among
+ * names (the f2 handle's population labels) (reference: parse_qpgraph,
+ * src/core/qpadm/qpgraph_model.cpp:41-243).  Invalid graphs return STP_ERR_INVALID
+ * with the reference's message. */
+int stp_qpgraph_parse(int32_t n_edges, const char* const* from, const char* const* to, int32_t n_names,
+                      const char* const* names, stp_qpgraph_topology** out);
+int stp_qpgraph_topology_free(stp_qpgraph_topology* t);
+/* dims = [npop, nedge_norm, nadmix, npair, npath, base_leaf] (QpGraphModel fields). */
+int stp_qpgraph_topology_dims(const stp_qpgraph_topology* t, int32_t* dims);
+/* Labels: kind 0 leaf, 1 admixture-edge parent, 2 admixture child, 3 normal-edge parent,
+ * 4 normal-edge child (QpGraphResult's leaves / admix_from / admix_to / edge_from /
+ * edge_to); index within that list. */
+int stp_qpgraph_topology_label(const stp_qpgraph_topology* t, int32_t kind, int32_t index, char* buf,
+                               int32_t len);
+
+/* A resident set of graphs, graph g scored against f2[g]: the f3 basis
+ * (base; a, b), its block-jackknife covariance with the diag_f3 ridge and its inverse
+ * run on the device once (reference: assemble_f3_triples + jackknife_cov,
+ * src/core/qpadm/qpgraph_fit.cpp:57-80).  The same topology or handle may appear several
+ * times. */
+int stp_qpgraph_set_create(stp_context* ctx, int32_t n_graphs, const stp_qpgraph_topology* const* topologies,
+                           const stp_f2_handle* const* f2, double diag_f3, stp_qpgraph_set** out);
+int stp_qpgraph_set_free(stp_qpgraph_set* s);
+/* Per graph: 0 ok, 3 no surviving block or singular covariance. */
+int stp_qpgraph_set_status(const stp_qpgraph_set* s, int32_t* status);
+/* Row strides of the per-instance arrays of stp_qpgraph_eval: dmax (theta, gradient),
+ * emax (edge lengths) and pmax (fitted f3) = the largest nadmix / nedge_norm / npair of
+ * the set (dmax >= 1); pmax may be NULL. */
+int stp_qpgraph_set_strides(const stp_qpgraph_set* s, int32_t* dmax, int32_t* emax, int32_t* pmax);
+/* Graph g's f_obs (npair), unridged Q diagonal (npair) and Qinv (npair x npair, row-major). */
+int stp_qpgraph_set_basis(stp_context* ctx, const stp_qpgraph_set* s, int32_t graph, double* f_obs, double* qdiag,
+                          double* qinv);
+
+/* Batched score and adjoint (reference: qpgraph_score, src/core/qpadm/qpgraph_objective.hpp:163,
+ * evaluated by qpgraph_fit_fleet, src/device/cpu/cpu_backend.cpp:2936; the adjoint has no
+ * counterpart there: the reference differentiates by finite differences).  Instance i
+ * scores graph inst_graph[i] at theta[i*dmax .. +nadmix).
+ *   mode 0: the score with edge lengths from the ridge NNLS (constrained) or solve;
+ *   mode 1: mode 0 + d score / d theta through the edge-length solve;
+ *   mode 2: the score for given edge lengths bl[i*emax ..], with d / d theta and d / d bl.
+ * Outputs (host, may be NULL except score): score[n_inst], grad_theta[n_inst*dmax],
+ * grad_bl[n_inst*emax], bl_out[n_inst*emax] (the edge lengths used), fit[n_inst*pmax]
+ * (the fitted f3 basis).  A failed solve gives score = +inf. */
+int stp_qpgraph_eval(stp_context* ctx, stp_qpgraph_set* s, int32_t n_inst, const int32_t* inst_graph,
+                     const double* theta, const double* bl, int32_t mode, double fudge, int32_t constrained,
+                     double* score, double* grad_theta, double* grad_bl, double* bl_out, double* fit);
+
 /* ---- STPF2BK1 f2.bin (reference: src/device/f2_disk_format.hpp:17-50,
  *      src/app/f2_dir_writer.cpp:306-440, src/app/f2_dir_io.cpp:52-158) ------------- */
 
@@ -175,9 +227,10 @@ int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_s
 /* Enable per-stage device timing: HIP events recorded on the context's own stream
  * around the decode, prep and f2 contraction launches. */
 int stpx_timing(stp_context* ctx, int32_t enable);
-/* Accumulated milliseconds and launch counts per stage [decode, prep, f2, count, qpadm]
- * (f2 = the contraction kernel, count = the pair-count kernel, qpadm = the batched
- * fit kernels of one stp_qpadm_batch call); reset clears them. */
+/* Accumulated milliseconds and launch counts per stage [decode, prep, f2, count, qpadm,
+ * qpgraph] (f2 = the contraction kernel, count = the pair-count kernel, qpadm = the
+ * batched fit kernels of one stp_qpadm_batch call, qpgraph = one stp_qpgraph_eval
+ * launch); reset clears them. */
 int stpx_timing_read(stp_context* ctx, double* ms, int64_t* count, int32_t reset);
 
 /* ---- synthetic panels (bench/test input; no counterpart in the reference) ------------ */

@@ -12,7 +12,8 @@
 //   ref_assign_blocks    core::assign_blocks                src/core/domain/block_partition_rule.cpp:56
 //   ref_f2_blocks        ComputeBackend::compute_f2_blocks  src/device/cpu/cpu_backend.cpp:217
 //   ref_qpadm            steppe::run_qpadm (host tensor)    src/core/qpadm/qpadm_fit.cpp:216
-//   ref_qpgraph          steppe::run_qpgraph (host tensor)  src/core/qpadm/qpgraph_fit.cpp:127
+//   ref_qpgraph_fit      steppe::run_qpgraph (host tensor)  src/core/qpadm/qpgraph_fit.cpp:127
+//   ref_qpgraph_score    core::qpadm::qpgraph_score         src/core/qpadm/qpgraph_objective.hpp:163
 
 #include <cstdint>
 #include <cstring>
@@ -22,6 +23,8 @@
 #include <vector>
 
 #include "core/domain/block_partition_rule.hpp"
+#include "core/qpadm/qpgraph_model.hpp"
+#include "core/qpadm/qpgraph_objective.hpp"
 #include "core/internal/views.hpp"
 #include "device/backend.hpp"
 #include "device/backend_factory.hpp"
@@ -197,6 +200,70 @@ int ref_qpadm(const double* f2, const double* vpair, const std::int32_t* block_s
     } catch (const std::exception&) {
         return -1;
     }
+}
+
+static std::vector<steppe::QpGraphEdge> qg_edges(int n, const char* const* from, const char* const* to) {
+    std::vector<steppe::QpGraphEdge> e;
+    for (int i = 0; i < n; ++i) e.push_back(steppe::QpGraphEdge{from[i], to[i]});
+    return e;
+}
+
+static std::vector<std::string> qg_names(int n, const char* const* names) {
+    std::vector<std::string> v;
+    for (int i = 0; i < n; ++i) v.emplace_back(names[i]);
+    return v;
+}
+
+// Same outputs as cref_qpgraph_fit (oracle/cpu_ref_qpgraph.cpp).
+int ref_qpgraph_fit(const double* f2, const double* vpair, const std::int32_t* bsz, int P, int nb, int n_edges,
+                    const char* const* from, const char* const* to, int n_names, const char* const* names,
+                    double fudge, double diag_f3, int numstart, int constrained, int maxit, double tol,
+                    double* scal, double* theta, double* lo, double* hi, double* bl, double* fit) {
+    try {
+        for (int i = 0; i < 5; ++i) scal[i] = 0.0;
+        steppe::F2BlockTensor t;
+        t.P = P;
+        t.n_block = nb;
+        const std::size_t tot = static_cast<std::size_t>(P) * P * nb;
+        t.f2.assign(f2, f2 + tot);
+        if (vpair) t.vpair.assign(vpair, vpair + tot);
+        t.block_sizes.assign(bsz, bsz + nb);
+        steppe::QpGraphOptions o;
+        o.fudge = fudge;
+        o.diag_f3 = diag_f3;
+        o.numstart = numstart;
+        o.constrained = constrained != 0;
+        o.maxit = maxit;
+        o.tol = tol;
+        auto res = cpu_resources();
+        const steppe::QpGraphResult r = steppe::run_qpgraph(t, qg_edges(n_edges, from, to), qg_names(n_names, names), o, res);
+        scal[4] = static_cast<double>(static_cast<int>(r.status));
+        scal[0] = r.score;
+        scal[1] = r.restart_spread;
+        scal[2] = r.worst_residual_z;
+        put(theta, r.weight, r.weight.size());
+        put(lo, r.weight_lo, r.weight_lo.size());
+        put(hi, r.weight_hi, r.weight_hi.size());
+        put(bl, r.edge_length, r.edge_length.size());
+        (void)fit;
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+double ref_qpgraph_score(int n_edges, const char* const* from, const char* const* to, int n_names,
+                         const char* const* names, const double* theta, const double* f_obs, const double* qinv,
+                         double fudge, int constrained, double* bl, double* fit) {
+    const auto m = steppe::core::qpadm::parse_qpgraph(qg_edges(n_edges, from, to), qg_names(n_names, names));
+    if (!m.ok()) return std::nan("");
+    const std::vector<double> f(f_obs, f_obs + m.npair);
+    const std::vector<double> qi(qinv, qinv + static_cast<std::size_t>(m.npair) * m.npair);
+    std::vector<double> blv, fitv(static_cast<std::size_t>(m.npair), 0.0);
+    const double s = steppe::core::qpadm::qpgraph_score(m, theta, f, qi, fudge, constrained != 0, &blv, &fitv);
+    if (bl) put(bl, blv, blv.size());
+    if (fit) put(fit, fitv, fitv.size());
+    return s;
 }
 
 }  // extern "C"

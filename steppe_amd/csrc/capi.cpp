@@ -2,6 +2,7 @@
 //
 // Every entry point runs inside guard(): exceptions never cross the ABI; they are
 // mapped to status codes and the message is kept per thread for stp_last_error().
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -19,6 +20,15 @@ struct stp_context {
 struct stp_f2_handle {
     stp_context* ctx = nullptr;
     std::unique_ptr<stp::F2Blocks> blocks;
+};
+
+struct stp_qpgraph_topology {
+    stp::QgTopology t;
+};
+
+struct stp_qpgraph_set {
+    stp_context* ctx = nullptr;
+    std::unique_ptr<stp::QgSet> s;
 };
 
 namespace {
@@ -250,6 +260,118 @@ int stp_qpadm_batch(stp_context* ctx, const stp_f2_handle* f2, int32_t n_models,
         STP_REQUIRE(f2->ctx == ctx, "stp_qpadm_batch: the f2 handle belongs to another context");
         backend(ctx).qpadm_batch(*f2->blocks, n_models, target, left_offsets, left, right_offsets, right, *options,
                                  out);
+    });
+}
+
+int stp_qpgraph_parse(int32_t n_edges, const char* const* from, const char* const* to, int32_t n_names,
+                      const char* const* names, stp_qpgraph_topology** out) {
+    return guard([&] {
+        STP_REQUIRE(out && from && to && (names || n_names == 0), "stp_qpgraph_parse: null argument");
+        STP_REQUIRE(n_edges >= 0 && n_names >= 0, "stp_qpgraph_parse: negative count");
+        std::vector<std::pair<std::string, std::string>> e;
+        for (int i = 0; i < n_edges; ++i) {
+            STP_REQUIRE(from[i] && to[i], "stp_qpgraph_parse: null edge endpoint");
+            e.emplace_back(from[i], to[i]);
+        }
+        std::vector<std::string> nm;
+        for (int i = 0; i < n_names; ++i) nm.emplace_back(names[i] ? names[i] : "");
+        auto t = std::make_unique<stp_qpgraph_topology>();
+        t->t = stp::parse_qpgraph(e, nm);
+        *out = t.release();
+    });
+}
+
+int stp_qpgraph_topology_free(stp_qpgraph_topology* t) {
+    return guard([&] { delete t; });
+}
+
+int stp_qpgraph_topology_dims(const stp_qpgraph_topology* t, int32_t* dims) {
+    return guard([&] {
+        STP_REQUIRE(t && dims, "stp_qpgraph_topology_dims: null argument");
+        const auto& m = t->t;
+        const int32_t d[6] = {m.npop, m.ne, m.nadmix, m.npair, m.npath, m.base};
+        for (int i = 0; i < 6; ++i) dims[i] = d[i];
+    });
+}
+
+int stp_qpgraph_topology_label(const stp_qpgraph_topology* t, int32_t kind, int32_t index, char* buf,
+                               int32_t len) {
+    return guard([&] {
+        STP_REQUIRE(t && buf && len > 0, "stp_qpgraph_topology_label: null argument");
+        const auto& m = t->t;
+        const std::vector<std::string>* v = nullptr;
+        switch (kind) {
+            case 0: v = &m.leaves; break;
+            case 1: v = &m.admix_from; break;
+            case 2: v = &m.admix_to; break;
+            case 3: v = &m.edge_from; break;
+            case 4: v = &m.edge_to; break;
+            default: throw stp::ArgError("stp_qpgraph_topology_label: kind must be 0..4");
+        }
+        STP_REQUIRE(index >= 0 && index < static_cast<int32_t>(v->size()), "stp_qpgraph_topology_label: index out of range");
+        std::strncpy(buf, (*v)[index].c_str(), static_cast<std::size_t>(len - 1));
+        buf[len - 1] = '\0';
+    });
+}
+
+int stp_qpgraph_set_create(stp_context* ctx, int32_t n_graphs, const stp_qpgraph_topology* const* topologies,
+                           const stp_f2_handle* const* f2, double diag_f3, stp_qpgraph_set** out) {
+    return guard([&] {
+        STP_REQUIRE(out && topologies && f2 && n_graphs > 0, "stp_qpgraph_set_create: bad arguments");
+        std::vector<const stp::QgTopology*> tp(n_graphs);
+        std::vector<const stp::F2Blocks*> fb(n_graphs);
+        for (int g = 0; g < n_graphs; ++g) {
+            STP_REQUIRE(topologies[g] && f2[g] && f2[g]->blocks, "stp_qpgraph_set_create: null topology or handle");
+            STP_REQUIRE(f2[g]->ctx == ctx, "stp_qpgraph_set_create: an f2 handle belongs to another context");
+            tp[g] = &topologies[g]->t;
+            fb[g] = f2[g]->blocks.get();
+        }
+        auto s = std::make_unique<stp_qpgraph_set>();
+        s->ctx = ctx;
+        s->s.reset(backend(ctx).qpgraph_set_create(tp, fb.data(), diag_f3));
+        *out = s.release();
+    });
+}
+
+int stp_qpgraph_set_free(stp_qpgraph_set* s) {
+    return guard([&] {
+        if (s && s->ctx && s->ctx->be) s->ctx->be->activate();
+        delete s;
+    });
+}
+
+int stp_qpgraph_set_status(const stp_qpgraph_set* s, int32_t* status) {
+    return guard([&] {
+        STP_REQUIRE(s && s->s && status, "stp_qpgraph_set_status: null argument");
+        for (int g = 0; g < s->s->n; ++g) status[g] = s->s->host_status[g];
+    });
+}
+
+int stp_qpgraph_set_strides(const stp_qpgraph_set* s, int32_t* dmax, int32_t* emax, int32_t* pmax) {
+    return guard([&] {
+        STP_REQUIRE(s && s->s && dmax && emax, "stp_qpgraph_set_strides: null argument");
+        *dmax = std::max(s->s->dmax, 1);
+        *emax = s->s->ne_max;
+        if (pmax) *pmax = s->s->np_max;
+    });
+}
+
+int stp_qpgraph_set_basis(stp_context* ctx, const stp_qpgraph_set* s, int32_t graph, double* f_obs, double* qdiag,
+                          double* qinv) {
+    return guard([&] {
+        STP_REQUIRE(s && s->s && s->ctx == ctx, "stp_qpgraph_set_basis: bad set");
+        backend(ctx).qpgraph_set_basis(*s->s, graph, f_obs, qdiag, qinv);
+    });
+}
+
+int stp_qpgraph_eval(stp_context* ctx, stp_qpgraph_set* s, int32_t n_inst, const int32_t* inst_graph,
+                     const double* theta, const double* bl, int32_t mode, double fudge, int32_t constrained,
+                     double* score, double* grad_theta, double* grad_bl, double* bl_out, double* fit_out) {
+    return guard([&] {
+        STP_REQUIRE(s && s->s && s->ctx == ctx, "stp_qpgraph_eval: bad set");
+        STP_REQUIRE(n_inst >= 0 && (n_inst == 0 || (inst_graph && score)), "stp_qpgraph_eval: null argument");
+        backend(ctx).qpgraph_set_eval(*s->s, n_inst, inst_graph, theta, bl, mode, fudge, constrained, score,
+                                      grad_theta, grad_bl, bl_out, fit_out);
     });
 }
 

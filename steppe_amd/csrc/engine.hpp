@@ -15,6 +15,7 @@
 
 #include "common.hpp"
 #include "f2_layout.hpp"
+#include "qpgraph.hpp"
 #include "steppe_hip.h"
 
 namespace stp {
@@ -80,6 +81,21 @@ public:
     std::vector<std::int32_t> block_sizes;
 };
 
+// A resident set of qpGraph graphs: topology arena plus each graph's f3 basis (f_obs),
+// unridged Q diagonal and Qinv on the device, computed once; evaluations then move only
+// theta (and edge lengths) up and scores / gradients down.
+struct QgSet {
+    int n = 0, np_max = 0, ne_max = 0, npop_max = 0, npath_max = 0, dmax = 0;
+    std::vector<QgTopology> topo;
+    std::vector<std::int32_t> host_status;
+    DevBuf<QgTopoDev> d_topo;
+    DevBuf<std::int32_t> d_iarena, status;
+    DevBuf<double> d_darena, fobs, qdiag, qinv;
+    DevBuf<std::int32_t> i32;
+    DevBuf<std::int64_t> i64;
+    DevBuf<double> in, out;
+};
+
 class HipBackend {
 public:
     explicit HipBackend(int device);
@@ -114,8 +130,16 @@ public:
                      const std::int32_t* left, const std::int32_t* right_off, const std::int32_t* right,
                      const stp_qpadm_options& opt, stp_qpadm_result* out);
 
+    // qpGraph (qpgraph_host.cpp): build a resident set from topologies and f2 handles
+    // (the f3 basis and covariance run on the device), evaluate scores / adjoints.
+    QgSet* qpgraph_set_create(const std::vector<const QgTopology*>& topos, const F2Blocks* const* f2, double diag_f3);
+    void qpgraph_set_eval(QgSet& s, int n_inst, const std::int32_t* inst_graph, const double* theta,
+                          const double* bl_in, int mode, double fudge, int constrained, double* score,
+                          double* grad_theta, double* grad_bl, double* bl_out, double* fit_out);
+    void qpgraph_set_basis(const QgSet& s, int g, double* f_obs, double* qdiag, double* qinv);
+
     // Per-stage device timing with HIP events on this backend's stream (opt-in).
-    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kStages = 5 };
+    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kQpgraph = 5, kStages = 6 };
     void set_timing(bool on);
     void read_timing(double* ms, std::int64_t* count, bool reset);
 

@@ -198,3 +198,127 @@ def scaled_config(P: int, M: int, seed: int = SEED, block_lo: int = 200,
         bs = uniform_block_sizes(M, block_lo, block_hi, rng)
     return make_panel(sizes, M, fst, missing, haploid, blocks_from_sizes(bs), len(bs), rng,
                       bytes_per_record=bytes_per_record)
+
+
+# ---------------------------------------------------------------------------------------
+# admixture graphs with known parameters, and f2 block tensors simulated on them
+# ---------------------------------------------------------------------------------------
+def random_admixture_graph(rng, n_leaves: int, n_admix: int, drift_mean: float = 0.02,
+                           weight_lo: float = 0.1, weight_hi: float = 0.9):
+    """A random rooted binary tree on ``n_leaves`` leaves (P0..) with ``n_admix`` admixture
+    events.  An event inserts an admixed node m on a target edge x->y and a source node s
+    on another edge u->v (u not below y, so the graph stays acyclic) and adds s->m; m's
+    first-listed parent edge carries weight theta ~ U(weight_lo, weight_hi).  Normal edges
+    get drifts ~ Exp(drift_mean).  Returns (edges, theta, drift) with drift keyed by edge."""
+    nodes = [f"P{i}" for i in range(n_leaves)]
+    edges = []
+    pool = list(nodes)
+    k = 0
+    while len(pool) > 1:
+        a, b = rng.choice(len(pool), 2, replace=False)
+        par = f"n{k}"
+        k += 1
+        edges += [(par, pool[a]), (par, pool[b])]
+        pool = [p for i, p in enumerate(pool) if i not in (a, b)] + [par]
+    root = pool[0]
+    edges = [("R" if p == root else p, "R" if c == root else c) for p, c in edges]
+    admixed = set()
+    theta = []
+
+    def below(node):
+        out, stack = set(), [node]
+        while stack:
+            v = stack.pop()
+            for p, c in edges:
+                if p == v and c not in out:
+                    out.add(c)
+                    stack.append(c)
+        return out
+
+    for j in range(n_admix):
+        for _ in range(1000):
+            ti = int(rng.integers(len(edges)))
+            si = int(rng.integers(len(edges)))
+            x, y = edges[ti]
+            u, v = edges[si]
+            if ti == si or (x, y) in admixed or (u, v) in admixed:
+                continue
+            if u == y or u in below(y):
+                continue
+            break
+        else:
+            raise RuntimeError("could not place an admixture event")
+        m, s = f"a{j}", f"s{j}"
+        new = []
+        for e in edges:
+            if e == (x, y):
+                new += [(x, m), (m, y)]
+            elif e == (u, v):
+                new += [(u, s), (s, v)]
+            else:
+                new.append(e)
+        new.append((s, m))
+        edges = new
+        admixed.update({(x, m), (s, m)})
+        theta.append(float(rng.uniform(weight_lo, weight_hi)))
+    drift = {e: float(rng.exponential(drift_mean)) for e in edges if e not in admixed}
+    return edges, np.asarray(theta), drift
+
+
+def graph_leaf_weights(edges, theta):
+    """Leaf weights w[edge][leaf] of the normal edges for mixture weights theta: the summed
+    weight of the root-to-leaf paths through each edge (an admixed node's first-listed
+    parent edge has weight theta_j, the second 1 - theta_j)."""
+    children, parents = {}, {}
+    for p, c in edges:
+        children.setdefault(p, []).append(c)
+        parents.setdefault(c, []).append(p)
+    nodes = {n for e in edges for n in e}
+    root = next(n for n in nodes if n not in parents)
+    leaves = [n for n in dict.fromkeys(n for e in edges for n in e) if n not in children]
+    admix = [n for n in dict.fromkeys(n for e in edges for n in e) if len(parents.get(n, [])) == 2]
+    weight = {}
+    for j, m in enumerate(admix):
+        first = next(e for e in edges if e[1] == m)
+        for e in edges:
+            if e[1] == m:
+                weight[e] = theta[j] if e == first else 1.0 - theta[j]
+    normal = [e for e in edges if e not in weight]
+    W = {e: {l: 0.0 for l in leaves} for e in normal}
+
+    def walk(v, w, trail):
+        if v not in children:
+            for e in trail:
+                W[e][v] += w
+            return
+        for c in children[v]:
+            e = (v, c)
+            walk(c, w * weight.get(e, 1.0), trail + ([e] if e in W else []))
+
+    walk(root, 1.0, [])
+    return W, leaves
+
+
+def simulate_graph_f2(edges, theta, drift, n_block: int = 300, rng=None, noise: float = 0.05,
+                      block_size: int = 1000):
+    """FP64 f2 block tensor [P x P x n_block] (i + P*j + P*P*b) for the leaves of the graph
+    in order P0, P1, ...: the graph's f2(a, b) = sum_e drift_e (w_ea - w_eb)^2 plus
+    independent symmetric per-block noise of relative size ``noise``."""
+    rng = rng or np.random.default_rng(0)
+    W, leaves = graph_leaf_weights(edges, theta)
+    names = sorted(leaves, key=lambda s: int(s[1:]))
+    P = len(names)
+    f2 = np.zeros((P, P))
+    for e, wl in W.items():
+        d = drift[e]
+        for a in range(P):
+            for b in range(P):
+                f2[a, b] += d * (wl[names[a]] - wl[names[b]]) ** 2
+    scale = noise * max(f2.mean(), 1e-6)
+    out = np.empty((P, P, n_block))
+    for b in range(n_block):
+        z = rng.normal(0.0, scale, (P, P))
+        z = np.triu(z, 1)
+        z = z + z.T
+        out[:, :, b] = f2 + z
+    return np.asfortranarray(out), np.full(n_block, block_size, np.int32), names

@@ -147,6 +147,108 @@ class _Lib:
         return out.to_dict()
 
 
+    # ---- qpGraph ----------------------------------------------------------------
+    @staticmethod
+    def _strs(items):
+        arr = (C.c_char_p * max(len(items), 1))(*[x.encode() for x in items])
+        return arr
+
+    def _graph_args(self, edges, names):
+        fr = self._strs([a for a, _ in edges])
+        to = self._strs([b for _, b in edges])
+        nm = self._strs(list(names))
+        self._keep = (fr, to, nm)
+        return [C.c_int(len(edges)), fr, to, C.c_int(len(names)), nm]
+
+    def qpgraph_fit(self, f2, vpair, block_sizes, edges, names, fudge=1e-4, diag_f3=1e-5, numstart=10,
+                    constrained=True, maxit=200, tol=1e-9):
+        """run_qpgraph on a host tensor: dict with score, restart_spread, weight, weight_lo,
+        weight_hi, edge_length, worst_residual_z, status."""
+        f2 = np.asarray(f2, np.float64)
+        P, _, nb = f2.shape
+        fl = np.asfortranarray(f2).ravel(order="F")
+        vp = None if vpair is None else np.asfortranarray(np.asarray(vpair, np.float64)).ravel(order="F")
+        bs = np.ascontiguousarray(block_sizes, np.int32)
+        scal = np.zeros(5)
+        th, lo, hi = np.zeros(64), np.zeros(64), np.zeros(64)
+        bl, fit = np.zeros(256), np.zeros(4096)
+        f = self.fn("qpgraph_fit")
+        f.restype = C.c_int
+        rc = f(fl.ctypes.data_as(C.c_void_p), None if vp is None else vp.ctypes.data_as(C.c_void_p),
+               bs.ctypes.data_as(C.c_void_p), C.c_int(P), C.c_int(nb), *self._graph_args(edges, names),
+               C.c_double(fudge), C.c_double(diag_f3), C.c_int(numstart), C.c_int(1 if constrained else 0),
+               C.c_int(maxit), C.c_double(tol), scal.ctypes.data_as(C.c_void_p), th.ctypes.data_as(C.c_void_p),
+               lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), bl.ctypes.data_as(C.c_void_p),
+               fit.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        dims = cref().qpgraph_model(edges, names)
+        D, ne, npair = dims["nadmix"], dims["nedge_norm"], dims["npair"]
+        return {"score": scal[0], "restart_spread": scal[1], "worst_residual_z": scal[2],
+                "status": int(scal[4]), "weight": th[:D].copy(), "weight_lo": lo[:D].copy(),
+                "weight_hi": hi[:D].copy(), "edge_length": bl[:ne].copy(), "f3_fit": fit[:npair].copy()}
+
+    def qpgraph_score(self, edges, names, theta, f_obs, qinv, fudge=1e-4, constrained=True):
+        """(score, edge lengths, fitted f3) at mixture weights theta."""
+        dims = cref().qpgraph_model(edges, names)
+        bl = np.zeros(max(dims["nedge_norm"], 1))
+        fit = np.zeros(max(dims["npair"], 1))
+        th = np.ascontiguousarray(theta if theta is not None and len(theta) else np.zeros(1), np.float64)
+        fo = np.ascontiguousarray(f_obs, np.float64)
+        qi = np.asfortranarray(np.asarray(qinv, np.float64)).ravel(order="F")
+        f = self.fn("qpgraph_score")
+        f.restype = C.c_double
+        s = f(*self._graph_args(edges, names), th.ctypes.data_as(C.c_void_p), fo.ctypes.data_as(C.c_void_p),
+              qi.ctypes.data_as(C.c_void_p), C.c_double(fudge), C.c_int(1 if constrained else 0),
+              bl.ctypes.data_as(C.c_void_p), fit.ctypes.data_as(C.c_void_p))
+        return s, bl[:dims["nedge_norm"]].copy(), fit[:dims["npair"]].copy()
+
+    def qpgraph_model(self, edges, names):
+        d = np.zeros(8, np.int32)
+        f = self.fn("qpgraph_model")
+        f.restype = C.c_int
+        rc = f(*self._graph_args(edges, names), d.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise ValueError("invalid graph")
+        keys = ("npop", "nedge_norm", "nadmix", "npair", "npath", "base_leaf", "n_pe", "n_pae")
+        return {k: int(v) for k, v in zip(keys, d)}
+
+    def qpgraph_tables(self, edges, names):
+        d = self.qpgraph_model(edges, names)
+        ne, npop, npair = d["nedge_norm"], d["npop"], d["npair"]
+        t = {"pwts0": np.zeros(ne * npop), "pe_edge": np.zeros(d["n_pe"], np.int32),
+             "pe_leaf": np.zeros(d["n_pe"], np.int32), "pe_path": np.zeros(d["n_pe"], np.int32),
+             "pae_path": np.zeros(d["n_pae"], np.int32), "pae_id": np.zeros(d["n_pae"], np.int32),
+             "cmb1": np.zeros(npair, np.int32), "cmb2": np.zeros(npair, np.int32),
+             "leaf_to_f2": np.zeros(npop, np.int32)}
+        f = self.fn("qpgraph_tables")
+        f.restype = C.c_int
+        args = [t[k].ctypes.data_as(C.c_void_p) if t[k].size else None for k in
+                ("pwts0", "pe_edge", "pe_leaf", "pe_path", "pae_path", "pae_id", "cmb1", "cmb2", "leaf_to_f2")]
+        assert f(*self._graph_args(edges, names), *args) == 0
+        t["pwts0"] = t["pwts0"].reshape((ne, npop), order="F")
+        t.update(d)
+        return t
+
+    def qpgraph_basis(self, f2, vpair, block_sizes, edges, names, diag_f3=1e-5):
+        """(f_obs, Q, Qinv) of the f3 basis; Q/Qinv as (npair, npair) arrays."""
+        d = self.qpgraph_model(edges, names)
+        npair = d["npair"]
+        f2 = np.asarray(f2, np.float64)
+        P, _, nb = f2.shape
+        fl = np.asfortranarray(f2).ravel(order="F")
+        vp = None if vpair is None else np.asfortranarray(np.asarray(vpair, np.float64)).ravel(order="F")
+        bs = np.ascontiguousarray(block_sizes, np.int32)
+        fo, Q, Qi = np.zeros(npair), np.zeros(npair * npair), np.zeros(npair * npair)
+        f = self.fn("qpgraph_basis")
+        f.restype = C.c_int
+        rc = f(fl.ctypes.data_as(C.c_void_p), None if vp is None else vp.ctypes.data_as(C.c_void_p),
+               bs.ctypes.data_as(C.c_void_p), C.c_int(P), C.c_int(nb), *self._graph_args(edges, names),
+               C.c_double(diag_f3), fo.ctypes.data_as(C.c_void_p), Q.ctypes.data_as(C.c_void_p),
+               Qi.ctypes.data_as(C.c_void_p))
+        assert rc == 0, rc
+        return fo, Q.reshape((npair, npair), order="F"), Qi.reshape((npair, npair), order="F")
+
+
 _cache = {}
 
 

@@ -1,0 +1,197 @@
+"""qpGraph on the device: f3 basis and covariance, batched score, the hand-written
+adjoint and the L-BFGS fit, against the oracle, the torch FP64 reference gradient,
+central finite differences (h = 1e-6) and the ADMIXTOOLS 2 golden.
+
+Tolerances: f_obs / Q diagonal 1e-12 relative; Qinv 1e-9 of its largest entry; score
+1e-9 relative; adjoint vs the torch FP64 gradient 1e-7 relative (plus 1e-9 of the
+gradient's scale); adjoint vs central differences of the oracle's score 1e-4 relative
+(the h = 1e-6 central difference carries O(h^2) truncation and O(eps s / h) rounding);
+fitted optimum: no worse than the oracle's finite-difference search by more than 1e-7
+relative, the golden score within 1e-4 and its weight within 1e-5.
+"""
+import csv
+import os
+
+import numpy as np
+import pytest
+
+from fixtures import read_raw_fixture
+
+pytestmark = pytest.mark.gpu
+
+AT2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "at2")
+POPS = ["England_BellBeaker", "Czechia_EBA_CordedWare", "Turkey_N", "Mbuti", "Israel_Natufian",
+        "Iran_GanjDareh_N", "Han", "Papuan", "Karitiana"]
+
+
+def golden_edges():
+    return [(r["from"], r["to"]) for r in csv.DictReader(open(os.path.join(AT2, "golden_qpgraph_topology.csv")))]
+
+
+def simulated(seed, L, A, n_block=300):
+    from steppe_amd import synth
+    rng = np.random.default_rng(seed)
+    edges, theta, drift = synth.random_admixture_graph(rng, L, A)
+    f2, bs, names = synth.simulate_graph_f2(edges, theta, drift, n_block=n_block, rng=rng)
+    return f2, bs, edges, names
+
+
+@pytest.fixture(scope="module")
+def cases():
+    f2, bs = read_raw_fixture(os.path.join(AT2, "f2_qpgraph_9pop.bin"))
+    out = [(f2, bs, golden_edges(), POPS)]
+    out += [simulated(s, L, A) for s, L, A in ((11, 8, 1), (12, 10, 2), (13, 12, 3), (14, 12, 4))]
+    return out
+
+
+@pytest.fixture(scope="module")
+def gset(cases):
+    from steppe_amd import _core, qpgraph
+    topos, hs = [], []
+    for f2, bs, edges, names in cases:
+        h = _core.upload_f2(f2, None, bs, names)
+        hs.append(h)
+        topos.append(qpgraph.Topology(edges, names))
+    return qpgraph.GraphSet(topos, hs)
+
+
+def test_basis_matches_oracle(cref, cases, gset):
+    assert np.all(gset.status == 0)
+    for g, (f2, bs, edges, names) in enumerate(cases):
+        fo, Q, Qi = cref.qpgraph_basis(f2, None, bs, edges, names)
+        f_dev, qd_dev, qi_dev = gset.basis(g)
+        assert np.allclose(f_dev, fo, rtol=1e-12, atol=1e-18)
+        assert np.allclose(qd_dev, np.diag(Q), rtol=1e-12, atol=0)
+        err = np.abs(qi_dev - Qi).max() / np.abs(Qi).max()
+        print(f"graph {g}: Qinv max diff / max |Qinv| = {err:.2e}")
+        assert err <= 1e-9
+
+
+def _thetas(D, n, seed):
+    return np.random.default_rng(seed).uniform(0.05, 0.95, (n, D))
+
+
+def test_score_matches_oracle(cref, cases, gset):
+    worst = 0.0
+    for g, (f2, bs, edges, names) in enumerate(cases):
+        fo, _, Qi = cref.qpgraph_basis(f2, None, bs, edges, names)
+        D = gset.topologies[g].nadmix
+        th = _thetas(D, 6, g)
+        for con in (True, False):
+            r = gset.eval([g] * len(th), th, mode=0, constrained=con, want_fit=True)
+            for k in range(len(th)):
+                s, bl, fit = cref.qpgraph_score(edges, names, th[k], fo, Qi, constrained=con)
+                rel = abs(r["score"][k] - s) / abs(s)
+                worst = max(worst, rel)
+                assert rel <= 1e-9, (g, con, r["score"][k], s)
+                ne = len(bl)
+                assert np.allclose(r["bl"][k, :ne], bl, rtol=1e-7, atol=1e-12 * np.abs(bl).max())
+    print(f"score: worst relative difference {worst:.2e}")
+
+
+def _active_stable(cref, edges, names, th, fo, Qi, d, h):
+    a = cref.qpgraph_score(edges, names, th, fo, Qi)[1] > 0
+    for sgn in (1, -1):
+        t2 = th.copy()
+        t2[d] += sgn * h
+        if not np.array_equal(cref.qpgraph_score(edges, names, t2, fo, Qi)[1] > 0, a):
+            return False
+    return True
+
+
+def test_adjoint_against_torch_and_fd(cref, cases, gset):
+    import torch
+    import qpgraph_ref as qr
+    h = 1e-6
+    worst_t, worst_fd, n_fd = 0.0, 0.0, 0
+    for g, (f2, bs, edges, names) in enumerate(cases):
+        t = cref.qpgraph_tables(edges, names)
+        fo, _, Qi = cref.qpgraph_basis(f2, None, bs, edges, names)
+        D = t["nadmix"]
+        th = _thetas(D, 5, 100 + g)
+        r = gset.eval([g] * len(th), th, mode=1)
+        for k in range(len(th)):
+            gdev = r["grad_theta"][k, :D]
+            s, bl, _ = cref.qpgraph_score(edges, names, th[k], fo, Qi)
+            tt = torch.tensor(th[k], requires_grad=True)
+            val, _ = qr.profiled_score(t, tt, fo, Qi, bl > 0)
+            (gref,) = torch.autograd.grad(val, tt)
+            gref = gref.numpy()
+            scale = np.abs(gref).max() + 1e-300
+            err = np.abs(gdev - gref).max() / scale
+            worst_t = max(worst_t, err)
+            assert err <= 1e-7, (g, k, gdev, gref)
+            for d in range(D):
+                if not _active_stable(cref, edges, names, th[k], fo, Qi, d, h):
+                    continue
+                tp, tm = th[k].copy(), th[k].copy()
+                tp[d] += h
+                tm[d] -= h
+                fd = (cref.qpgraph_score(edges, names, tp, fo, Qi)[0] - cref.qpgraph_score(edges, names, tm, fo, Qi)[0]) / (2 * h)
+                e = abs(gdev[d] - fd) / max(abs(fd), 1e-6 * s)
+                worst_fd = max(worst_fd, e)
+                n_fd += 1
+                assert e <= 1e-4, (g, k, d, gdev[d], fd)
+    print(f"adjoint: worst vs torch FP64 {worst_t:.2e}; worst vs central FD {worst_fd:.2e} over {n_fd} checks")
+
+
+def test_joint_adjoint(cref, cases, gset):
+    import torch
+    import qpgraph_ref as qr
+    h = 1e-6
+    for g, (f2, bs, edges, names) in enumerate(cases):
+        t = cref.qpgraph_tables(edges, names)
+        fo, _, Qi = cref.qpgraph_basis(f2, None, bs, edges, names)
+        D, ne = t["nadmix"], t["nedge_norm"]
+        rng = np.random.default_rng(200 + g)
+        th = rng.uniform(0.05, 0.95, (3, D))
+        bl = rng.exponential(0.01, (3, ne))
+        r = gset.eval([g] * 3, th, bl=bl, mode=2)
+        for k in range(3):
+            tt = torch.tensor(th[k], requires_grad=True)
+            bb = torch.tensor(bl[k], requires_grad=True)
+            val = qr.joint_score(t, tt, bb, fo, Qi)
+            assert abs(r["score"][k] - float(val)) <= 1e-10 * abs(float(val))
+            gt, gb = torch.autograd.grad(val, (tt, bb))
+            assert np.allclose(r["grad_theta"][k, :D], gt.numpy(), rtol=1e-8, atol=1e-9 * np.abs(gt.numpy()).max())
+            assert np.allclose(r["grad_bl"][k, :ne], gb.numpy(), rtol=1e-8, atol=1e-9 * np.abs(gb.numpy()).max())
+            e = int(rng.integers(ne))
+            bp, bm = bl[k].copy(), bl[k].copy()
+            bp[e] += h
+            bm[e] -= h
+            fd = (float(qr.joint_score(t, tt.detach(), torch.tensor(bp), fo, Qi)) -
+                  float(qr.joint_score(t, tt.detach(), torch.tensor(bm), fo, Qi))) / (2 * h)
+            assert abs(r["grad_bl"][k, e] - fd) <= 1e-4 * max(abs(fd), 1e-3)
+
+
+def test_autograd_function(cases, gset):
+    import torch
+    from steppe_amd import qpgraph
+    Score = qpgraph.score_function()
+    g = 3
+    D = gset.topologies[g].nadmix
+    th = torch.tensor(_thetas(D, 4, 7), requires_grad=True)
+    val = Score.apply(th, gset, np.full(4, g, np.int32))
+    (grad,) = torch.autograd.grad(val.sum(), th)
+    r = gset.eval([g] * 4, th.detach().numpy(), mode=1)
+    assert np.allclose(grad.numpy(), r["grad_theta"][:, :D], rtol=0, atol=0)
+
+
+def test_fits(cref, cases, gset):
+    from steppe_amd import qpgraph
+    fits = qpgraph.fit_sets(gset, numstart=10)
+    for g, (f2, bs, edges, names) in enumerate(cases):
+        o = cref.qpgraph_fit(f2, None, bs, edges, names)
+        d = fits[g]
+        print(f"graph {g}: L-BFGS {d['score']:.10f} ({d['iterations']} it) vs oracle search {o['score']:.10f}")
+        assert d["score"] <= o["score"] * (1 + 1e-7) + 1e-12
+    # the golden graph through the single-fit entry point
+    from steppe_amd import _core
+    f2, bs, edges, names = cases[0]
+    h = _core.upload_f2(f2, None, bs, names)
+    r = qpgraph.run_qpgraph(h, edges)
+    assert r["status"] == "ok"
+    assert abs(r["score"] - 80.0674246076313) <= 1e-4
+    assert abs(r["weight"][0] - 0.153483829987482) <= 1e-5
+    assert r["restart_spread"] < 1e-4
+    assert r["admix_from"] == ["pSteppe"] and r["admix_to"] == ["aCW"]
