@@ -314,6 +314,15 @@ __device__ inline bf8 scale_2m8(bf8 a) {
     return __builtin_bit_cast(bf8, x);
 }
 
+// s_waitcnt on the vector-memory count only (gfx9 encoding: vmcnt in bits 3:0 and
+// 15:14; expcnt 6:4 and lgkmcnt 11:8 left at their maxima).  As a builtin, unlike
+// inline assembly, the compiler's wait-count tracking takes it into account.
+template <int N>
+__device__ inline void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt is six bits");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
 // Phase kinds of a K-step: 0 = q digits (planes 0-5), 1 = e0..e2 + V, 2 = e3..e5 + V.
 __device__ inline int phase_plane(int kind, int pp) {
     if (kind == 0) return pp;
@@ -322,13 +331,16 @@ __device__ inline int phase_plane(int kind, int pp) {
 }
 
 // Issue this wavefront's share of one phase's 1 KB chunks (direct global->LDS).
-// Every wavefront issues exactly (8 groups * planes) / 8 loads per phase.
-__device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, int kind,
-                                   std::int64_t t16, std::int64_t KS16, int gI0, int gJ0, int G32, int wave,
-                                   int lane) {
-    const int np = kind == 0 ? kDPlanes : kEPlanes;
-    const int nchunks = kGroups * np;
-    for (int ci = wave; ci < nchunks; ci += kWaves) {
+// Every wavefront issues exactly (8 groups * planes) / 8 = planes loads per phase; the
+// compile-time count lets the compiler's wait-count tracking see how many loads
+// follow each phase's, so that it adds no wait of its own beyond the explicit ones.
+template <int kind>
+__device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, std::int64_t t16,
+                                   std::int64_t KS16, int gI0, int gJ0, int G32, int wave, int lane) {
+    constexpr int np = kind == 0 ? kDPlanes : kEPlanes;
+#pragma unroll
+    for (int r = 0; r < np; ++r) {
+        const int ci = wave + kWaves * r;
         const int lg = ci / np, pp = ci - lg * np;
         int gg = lg < 4 ? gI0 + lg : gJ0 + lg - 4;
         if (gg >= G32) gg = 0;  // rows beyond P: any valid address; never read back
@@ -345,7 +357,11 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                                                        unsigned* __restrict__ counters, int P, int G32,
                                                        std::int64_t KS16, const double* __restrict__ vpair,
                                                        double* __restrict__ f2) {
-    __shared__ __attribute__((aligned(16))) std::uint8_t lds[kSlots * kSlotBytes];
+    // One LDS object per ring slot: the compiler then knows that reads of one slot do
+    // not depend on the direct loads still landing in the others.
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_d[kSlotBytes];
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_e1[kSlotBytes];
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_e2[kSlotBytes];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     // Wavefront-uniform values are moved to scalar registers so that every branch
@@ -354,9 +370,9 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
     const int wa = wave & 1;   // left sub-tile: groups 2wa, 2wa+1 of the left tile
     const int wc = wave >> 1;  // right sub-tile: group wc of the right tile
     const std::int64_t slab = static_cast<std::int64_t>(P) * P;
-    std::uint8_t* const slot_d = lds;
-    std::uint8_t* const slot_e1 = lds + kSlotBytes;
-    std::uint8_t* const slot_e2 = lds + 2 * kSlotBytes;
+    std::uint8_t* const slot_d = lds_d;
+    std::uint8_t* const slot_e1 = lds_e1;
+    std::uint8_t* const slot_e2 = lds_e2;
 
     const int home = static_cast<int>(blockIdx.x % kXcds);
     int xq = 0;
@@ -379,7 +395,6 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
             const int li = 2 * wa + x;
             act[x] = (gI0 + li < G32) && (gJ0 + wc < G32) && (gI0 < gJ0 || li <= wc);
         }
-        const bool any = act[0] || act[1];
 
         f16v L[2][5], H[2];
 #pragma unroll
@@ -391,19 +406,23 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
 
         // Prologue: the first K-step's first two phases in flight (the slot of a
         // phase is its kind, since a K-step has three phases and the ring three slots).
-        if (nsteps > 0) {
-            stage_phase(planes, slot_d, 0, t0, KS16, gI0, gJ0, G32, wave, lane);
-            stage_phase(planes, slot_e1, 1, t0, KS16, gI0, gJ0, G32, wave, lane);
-        }
+        // Every phase stages unconditionally (steps past the block end re-read the last
+        // K-step of the buffer and are never read back), so every wavefront issues the
+        // same number of loads between any two waits on every path: the explicit waits
+        // below are then exact and the compiler adds none of its own.
+        const std::int64_t t_last = KS16 - 1;
+        stage_phase<0>(planes, slot_d, t0 < t_last ? t0 : t_last, KS16, gI0, gJ0, G32, wave, lane);
+        stage_phase<1>(planes, slot_e1, t0 < t_last ? t0 : t_last, KS16, gI0, gJ0, G32, wave, lane);
 
         for (int step = 0; step < nsteps; ++step) {
             const bool more = step + 1 < nsteps;
+            const std::int64_t tn = t0 + step + 1 < t_last ? t0 + step + 1 : t_last;
             // ---- q digits.  Outstanding: this phase (6 loads), the e1 phase (4).
-            asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+            wait_vmcnt<4>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            stage_phase(planes, slot_e2, 2, t0 + step, KS16, gI0, gJ0, G32, wave, lane);
-            if (any) {
+            stage_phase<2>(planes, slot_e2, t0 + step, KS16, gI0, gJ0, G32, wave, lane);
+            {
                 bf8 dj[6];
 #pragma unroll
                 for (int k = 0; k < 6; ++k) dj[k] = lds_frag(slot_d, 4 + wc, k, kDPlanes, lane);
@@ -423,11 +442,11 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                 }
             }
             // ---- e0..e2 + V.  Outstanding: this phase (4), the e2 phase (4).
-            asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+            wait_vmcnt<4>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            if (more) stage_phase(planes, slot_d, 0, t0 + step + 1, KS16, gI0, gJ0, G32, wave, lane);
-            if (any) {
+            stage_phase<0>(planes, slot_d, tn, KS16, gI0, gJ0, G32, wave, lane);
+            {
                 bf8 ej[3];
 #pragma unroll
                 for (int m = 0; m < 3; ++m) ej[m] = lds_frag(slot_e1, 4 + wc, m, kEPlanes, lane);
@@ -447,13 +466,12 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                 }
             }
             // ---- e3..e5 + V (e5 pre-scaled into level 4).  Outstanding: this phase
-            // and, when issued, the next q phase (6).
-            if (more) asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
-            else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            // and the next q phase (6).
+            wait_vmcnt<6>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            if (more) stage_phase(planes, slot_e1, 1, t0 + step + 1, KS16, gI0, gJ0, G32, wave, lane);
-            if (any) {
+            stage_phase<1>(planes, slot_e1, tn, KS16, gI0, gJ0, G32, wave, lane);
+            {
                 bf8 ej[3];
 #pragma unroll
                 for (int m = 0; m < 3; ++m) ej[m] = lds_frag(slot_e2, 4 + wc, m, kEPlanes, lane);
@@ -522,6 +540,9 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
             }
         }
     }
+    // The last item's look-ahead loads may still be landing in LDS: drain them before
+    // the workgroup's LDS can be handed to another workgroup.
+    wait_vmcnt<0>();
 }
 
 }  // namespace

@@ -315,10 +315,8 @@ def simulate_graph_f2(edges, theta, drift, n_block: int = 300, rng=None, noise: 
             for b in range(P):
                 f2[a, b] += d * (wl[names[a]] - wl[names[b]]) ** 2
     scale = noise * max(f2.mean(), 1e-6)
-    out = np.empty((P, P, n_block))
-    for b in range(n_block):
-        z = rng.normal(0.0, scale, (P, P))
-        z = np.triu(z, 1)
-        z = z + z.T
-        out[:, :, b] = f2 + z
+    # one (P x P) draw per block, in block order (the same stream as drawing block by block)
+    z = np.triu(rng.normal(0.0, scale, (n_block, P, P)), 1)
+    z = z + z.transpose(0, 2, 1)
+    out = np.transpose(f2[None, :, :] + z, (1, 2, 0))
     return np.asfortranarray(out), np.full(n_block, block_size, np.int32), names

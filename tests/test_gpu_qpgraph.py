@@ -4,8 +4,11 @@ central finite differences (h = 1e-6) and the ADMIXTOOLS 2 golden.
 
 Tolerances: f_obs / Q diagonal 1e-12 relative; Qinv 1e-9 of its largest entry; score
 1e-9 relative; adjoint vs the torch FP64 gradient 1e-7 relative (plus 1e-9 of the
-gradient's scale); adjoint vs central differences of the oracle's score 1e-4 relative
-(the h = 1e-6 central difference carries O(h^2) truncation and O(eps s / h) rounding);
+gradient's scale); adjoint vs central differences (h = 1e-6) of the oracle's score:
+1e-4 relative plus 1e-13 s / h absolute, the difference quotient's rounding for a score s
+computed to ~1e-14 relative (a small gradient component of a large score is otherwise
+below the quotient's noise: at s = 1446, d/dtheta = 3.5e-3 moves by 5e-6 between
+h = 1e-5 and 1e-6);
 fitted optimum: no worse than the oracle's finite-difference search by more than 1e-7
 relative, the golden score within 1e-4 and its weight within 1e-5.
 """
@@ -20,6 +23,7 @@ from fixtures import read_raw_fixture
 pytestmark = pytest.mark.gpu
 
 AT2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "at2")
+FD_NOISE = 1e-13
 POPS = ["England_BellBeaker", "Czechia_EBA_CordedWare", "Turkey_N", "Mbuti", "Israel_Natufian",
         "Iran_GanjDareh_N", "Han", "Papuan", "Karitiana"]
 
@@ -128,11 +132,15 @@ def test_adjoint_against_torch_and_fd(cref, cases, gset):
                 tp[d] += h
                 tm[d] -= h
                 fd = (cref.qpgraph_score(edges, names, tp, fo, Qi)[0] - cref.qpgraph_score(edges, names, tm, fo, Qi)[0]) / (2 * h)
-                e = abs(gdev[d] - fd) / max(abs(fd), 1e-6 * s)
+                # 1e-4 relative, plus the difference quotient's own rounding: the score
+                # carries ~1e-14 relative error, amplified by s / h
+                tol = 1e-4 * abs(fd) + FD_NOISE * s / h
+                e = abs(gdev[d] - fd) / tol
                 worst_fd = max(worst_fd, e)
                 n_fd += 1
-                assert e <= 1e-4, (g, k, d, gdev[d], fd)
-    print(f"adjoint: worst vs torch FP64 {worst_t:.2e}; worst vs central FD {worst_fd:.2e} over {n_fd} checks")
+                assert e <= 1.0, (g, k, d, gdev[d], fd, tol)
+    print(f"adjoint: worst vs torch FP64 {worst_t:.2e}; worst vs central FD {worst_fd:.2e} of tolerance "
+          f"over {n_fd} checks")
 
 
 def test_joint_adjoint(cref, cases, gset):
@@ -161,7 +169,7 @@ def test_joint_adjoint(cref, cases, gset):
             bm[e] -= h
             fd = (float(qr.joint_score(t, tt.detach(), torch.tensor(bp), fo, Qi)) -
                   float(qr.joint_score(t, tt.detach(), torch.tensor(bm), fo, Qi))) / (2 * h)
-            assert abs(r["grad_bl"][k, e] - fd) <= 1e-4 * max(abs(fd), 1e-3)
+            assert abs(r["grad_bl"][k, e] - fd) <= 1e-4 * abs(fd) + FD_NOISE * abs(float(val)) / h
 
 
 def test_autograd_function(cases, gset):
