@@ -1,7 +1,8 @@
 """Benchmark: the f2 block-tensor extraction on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2] [--precision split_bf16|fp64]
-                    [--qpadm-models N] [--no-qpadm] [--no-cpu-baseline]
+                    [--qpadm-models N] [--no-qpadm] [--qpgraph-graphs N] [--no-qpgraph]
+                    [--no-cpu-baseline]
     python bench.py --calibrate-copy      # a plain 1 GiB device copy, for counter calibration
 
 One step = one pass of the hot path over the synthetic panel already resident in
@@ -30,7 +31,10 @@ sources) or the restated oracle, single-threaded, on a bounded sample of the sam
 workload.  ``qpadm``: the batched qpAdm metric (fits/s) for 10,000 random models
 (1 target + 2-5 sources, 6-12 right populations, drawn without overlap) on the
 resident c2 tensor, with its own single-threaded reference CPU rate on a sample of
-the same models.
+the same models.  ``qpgraph``: qpGraph fits/s for 5,000 random graphs (8-12 leaves, 1-4
+admixture events, each with its own simulated f2 over 300 blocks), 10 restarts each to
+convergence by batched L-BFGS on the device score and adjoint, with the reference CPU
+rate and the score agreement on a sample of the same graphs.
 """
 from __future__ import annotations
 
@@ -229,6 +233,86 @@ def qpadm_bench(ctx, n_models: int, cpu: bool):
     return out
 
 
+def random_graphs(rng, count: int):
+    """count random admixture graphs, 8-12 leaves and 1-4 admixture events each, with
+    their simulated f2 over 300 blocks."""
+    from steppe_amd import synth
+    out = []
+    for _ in range(count):
+        L = int(rng.integers(8, 13))
+        A = int(rng.integers(1, 5))
+        edges, theta, drift = synth.random_admixture_graph(rng, L, A)
+        f2, bs, names = synth.simulate_graph_f2(edges, theta, drift, n_block=300, rng=rng)
+        out.append((edges, f2, bs, names))
+    return out
+
+
+def qpgraph_bench(ctx, n_graphs: int, cpu: bool, numstart: int = 10):
+    """C5: n_graphs random graphs, each fitted from ``numstart`` restarts by the device
+    L-BFGS on the score + adjoint (one workgroup per restart, to convergence); the f3
+    basis and covariance of every graph are computed on the device inside the timed
+    region."""
+    from steppe_amd import _core, qpgraph
+    t0 = time.perf_counter()
+    graphs = random_graphs(np.random.default_rng(SEED + 5), n_graphs)
+    handles = [_core.upload_f2(f2, None, bs, names) for _, f2, bs, names in graphs]
+    log(f"[bench] qpgraph: {n_graphs} graphs simulated and resident in {time.perf_counter() - t0:.1f} s")
+    # warm-up: code load and staging buffers
+    w = qpgraph.GraphSet([qpgraph.Topology(graphs[0][0], graphs[0][3])], handles[:1])
+    w.fit_fleet(numstart=2, maxit=3)
+    del w
+    ctx.timing(True)
+    ctx.read_timing(reset=True)
+    t0 = time.perf_counter()
+    topos = [qpgraph.Topology(edges, names) for edges, _, _, names in graphs]
+    gset = qpgraph.GraphSet(topos, handles)
+    t_basis = time.perf_counter() - t0
+    fits = qpgraph.fit_sets(gset, numstart=numstart)
+    dt = time.perf_counter() - t0
+    tm = ctx.read_timing(reset=True)
+    ctx.timing(False)
+    ok = sum(1 for f in fits if f is not None and f["status"] == 0)
+    kern_ms = tm["qpgraph"][0]
+    iters = np.array([f["iterations"] for f in fits if f is not None])
+    log(f"[bench] qpgraph: {n_graphs} fits in {dt:.3f} s (basis {t_basis:.3f} s, {ok} ok); device fit + final "
+        f"evaluation kernels {kern_ms:.1f} ms; L-BFGS iterations per graph (most over restarts) mean "
+        f"{iters.mean():.1f}, max {iters.max()}")
+    out = {"metric": "qpGraph fits/sec", "value": n_graphs / dt, "unit": "fits/s", "graphs": n_graphs, "ok": ok,
+           "seconds": dt, "basis_seconds": t_basis, "restarts": numstart, "kernel_ms": kern_ms,
+           "iterations_mean": float(iters.mean()), "iterations_max": int(iters.max()),
+           "workload": f"c5: {n_graphs} random graphs (8-12 leaves, 1-4 admixture events), each with its own "
+                       f"simulated f2 over 300 blocks resident in HBM; device f3 basis + covariance, then "
+                       f"{numstart} restarts per graph, each run to convergence by projected L-BFGS on the "
+                       f"device score and adjoint gradient inside one kernel (FP64)",
+           "cpu_baseline": None}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        lib = oracle_lib.ref()
+        kind = "reference"
+        if lib is None:
+            lib, kind = oracle_lib.cref(), "port"
+        t0 = time.perf_counter()
+        n, worse = 0, 0
+        excess = 0.0
+        while n < n_graphs and time.perf_counter() - t0 < 10.0:
+            edges, f2, bs, names = graphs[n]
+            r = lib.qpgraph_fit(f2, None, bs, edges, names)
+            mine = fits[n]["score"] if fits[n] is not None else float("inf")
+            rel = (mine - r["score"]) / abs(r["score"])
+            excess = max(excess, rel)
+            worse += rel > 1e-7
+            n += 1
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / cdt, "unit": "fits/s", "cores": 1, "kind": kind,
+                               "sample": f"the first {n} of the same graphs, numstart 10, {cdt:.1f} s "
+                                         f"single-threaded ({os.path.basename(lib.path)})"}
+        out["vs_cpu_sample"] = {"graphs": n, "device_worse_by_1e-7_rel": int(worse),
+                                "max_rel_score_excess": excess}
+        log(f"[bench] qpgraph cpu baseline {out['cpu_baseline']} {out['vs_cpu_sample']}")
+    return out
+
+
 def calibrate_copy():
     """A plain 1 GiB device-to-device copy, three times (counter calibration)."""
     import torch
@@ -251,6 +335,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--qpadm-models", type=int, default=10_000)
     ap.add_argument("--no-qpadm", action="store_true")
+    ap.add_argument("--qpgraph-graphs", type=int, default=5_000)
+    ap.add_argument("--no-qpgraph", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -343,6 +429,12 @@ def main():
         del packed, a2, nn, f2, vp
         torch.cuda.empty_cache()
         qp = qpadm_bench(ctx, args.qpadm_models, not args.no_cpu_baseline)
+    qg = None
+    if not args.no_qpgraph:
+        if args.no_qpadm:
+            del packed, a2, nn, f2, vp
+            torch.cuda.empty_cache()
+        qg = qpgraph_bench(ctx, args.qpgraph_graphs, not args.no_cpu_baseline)
 
     out = {
         "metric": METRIC, "value": value, "unit": "pop-pair·SNPs/s", "n_gpus": 1,
@@ -354,7 +446,7 @@ def main():
                                f"{nb} jackknife blocks, decode + f2 ({args.precision})",
                    "P": P, "M": M, "n_block": nb, "n_individuals": n_ind,
                    "precision": args.precision, "parallelism": "single GPU"},
-        "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp,
+        "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg,
     }
     print(json.dumps(out), flush=True)
 

@@ -214,6 +214,49 @@ int stp_qpgraph_eval(stp_context* ctx, stp_qpgraph_set* s, int32_t n_inst, const
                      const double* theta, const double* bl, int32_t mode, double fudge, int32_t constrained,
                      double* score, double* grad_theta, double* grad_bl, double* bl_out, double* fit);
 
+/* A set from a given f3 basis and inverse covariance per graph, the inputs of the
+ * reference's fleet fit (ComputeBackend::qpgraph_fit_fleet, src/device/backend.hpp:1176;
+ * qpgraph_fit_fleet_batch, :1188): f_obs holds npair values per graph and qinv
+ * npair x npair row-major per graph, concatenated in graph order.  The set's Q diagonal
+ * (stp_qpgraph_set_basis) is reported as zeros. */
+int stp_qpgraph_set_create_basis(stp_context* ctx, int32_t n_graphs, const stp_qpgraph_topology* const* topologies,
+                                 const double* f_obs, const double* qinv, stp_qpgraph_set** out);
+
+/* Options of the device fit.  The reference's coordinate search with finite
+ * differences (cpu_backend.cpp:2936-3045) is replaced by a projected L-BFGS on [0,1]^D
+ * driven by the device adjoint, one instance per workgroup to convergence: it stops when
+ * the projected gradient is <= tol * 1e-2, the step <= tol * 1e-3, the relative decrease
+ * <= 1e-14, the line search fails, or after maxit iterations.  Defaults (the
+ * reference's QpGraphOptions): numstart 10, maxit 200, tol 1e-9, fudge 1e-4,
+ * constrained 1. */
+typedef struct stp_qpgraph_fit_options {
+    int32_t numstart;
+    int32_t maxit;
+    double tol;
+    double fudge;
+    int32_t constrained;
+} stp_qpgraph_fit_options;
+
+/* Fit instances from given start points: instance i fits graph inst_graph[i] from
+ * theta0[i*dmax ..]; outputs theta_out[n_inst*dmax] (may be NULL), score_out[n_inst]
+ * (+inf when the start point's edge-length system is singular) and iterations[n_inst]
+ * (-1 then; may be NULL).  numstart is not used. */
+int stp_qpgraph_fit(stp_context* ctx, stp_qpgraph_set* s, int32_t n_inst, const int32_t* inst_graph,
+                    const double* theta0, const stp_qpgraph_fit_options* options, double* theta_out,
+                    double* score_out, int32_t* iterations);
+
+/* Multistart fit of every graph of the set (reference: qpgraph_fit_fleet per topology,
+ * src/device/cpu/cpu_backend.cpp:2936-3045, with its splitmix start points): numstart
+ * restarts per graph (one when it has no admixture), all restarts of all graphs in one
+ * launch; per graph the best score, the restart spread (max - min score), the best
+ * theta and each coordinate's range over the restarts (rows of dmax), the edge lengths
+ * (rows of emax) and fitted f3 basis (rows of pmax) at the best theta, the largest
+ * iteration count and a status (0 ok, 3 singular covariance or edge-length system).
+ * Any output except score and status may be NULL. */
+int stp_qpgraph_fit_fleet(stp_context* ctx, stp_qpgraph_set* s, const stp_qpgraph_fit_options* options, double* score,
+                          double* restart_spread, double* theta, double* theta_lo, double* theta_hi,
+                          double* edge_length, double* f3_fit, int32_t* iterations, int32_t* status);
+
 /* ---- STPF2BK1 f2.bin (reference: src/device/f2_disk_format.hpp:17-50,
  *      src/app/f2_dir_writer.cpp:306-440, src/app/f2_dir_io.cpp:52-158) ------------- */
 
@@ -229,8 +272,8 @@ int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_s
 int stpx_timing(stp_context* ctx, int32_t enable);
 /* Accumulated milliseconds and launch counts per stage [decode, prep, f2, count, qpadm,
  * qpgraph] (f2 = the contraction kernel, count = the pair-count kernel, qpadm = the
- * batched fit kernels of one stp_qpadm_batch call, qpgraph = one stp_qpgraph_eval
- * launch); reset clears them. */
+ * batched fit kernels of one stp_qpadm_batch call, qpgraph = one stp_qpgraph_eval or
+ * stp_qpgraph_fit launch); reset clears them. */
 int stpx_timing_read(stp_context* ctx, double* ms, int64_t* count, int32_t reset);
 
 /* ---- synthetic panels (bench/test input; no counterpart in the reference) ------------ */

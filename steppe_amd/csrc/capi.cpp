@@ -375,6 +375,72 @@ int stp_qpgraph_eval(stp_context* ctx, stp_qpgraph_set* s, int32_t n_inst, const
     });
 }
 
+int stp_qpgraph_set_create_basis(stp_context* ctx, int32_t n_graphs, const stp_qpgraph_topology* const* topologies,
+                                 const double* f_obs, const double* qinv, stp_qpgraph_set** out) {
+    return guard([&] {
+        STP_REQUIRE(out && topologies && f_obs && qinv && n_graphs > 0, "stp_qpgraph_set_create_basis: bad arguments");
+        *out = nullptr;
+        std::vector<const stp::QgTopology*> tp(n_graphs);
+        for (int g = 0; g < n_graphs; ++g) {
+            STP_REQUIRE(topologies[g], "stp_qpgraph_set_create_basis: null topology");
+            tp[g] = &topologies[g]->t;
+        }
+        auto s = std::make_unique<stp_qpgraph_set>();
+        s->ctx = ctx;
+        s->s.reset(backend(ctx).qpgraph_set_from_basis(tp, f_obs, qinv));
+        *out = s.release();
+    });
+}
+
+namespace {
+stp::QgFitOptions fit_options(const stp_qpgraph_fit_options* o) {
+    stp::QgFitOptions f;
+    if (o) {
+        STP_REQUIRE(o->maxit >= 0 && o->tol >= 0.0, "qpgraph: maxit and tol must be >= 0");
+        f.maxit = o->maxit;
+        f.gtol = o->tol * 1e-2;
+        f.xtol = o->tol * 1e-3;
+        f.fudge = o->fudge;
+        f.constrained = o->constrained != 0;
+    }
+    return f;
+}
+}  // namespace
+
+int stp_qpgraph_fit(stp_context* ctx, stp_qpgraph_set* s, int32_t n_inst, const int32_t* inst_graph,
+                    const double* theta0, const stp_qpgraph_fit_options* options, double* theta_out,
+                    double* score_out, int32_t* iterations) {
+    return guard([&] {
+        STP_REQUIRE(s && s->s && s->ctx == ctx, "stp_qpgraph_fit: bad set");
+        STP_REQUIRE(n_inst >= 0 && (n_inst == 0 || (inst_graph && theta0 && score_out)), "stp_qpgraph_fit: null argument");
+        backend(ctx).qpgraph_set_fit(*s->s, n_inst, inst_graph, theta0, fit_options(options), theta_out, score_out,
+                                     iterations);
+    });
+}
+
+int stp_qpgraph_fit_fleet(stp_context* ctx, stp_qpgraph_set* s, const stp_qpgraph_fit_options* options, double* score,
+                          double* restart_spread, double* theta, double* theta_lo, double* theta_hi,
+                          double* edge_length, double* f3_fit, int32_t* iterations, int32_t* status) {
+    return guard([&] {
+        STP_REQUIRE(s && s->s && s->ctx == ctx, "stp_qpgraph_fit_fleet: bad set");
+        STP_REQUIRE(score && status, "stp_qpgraph_fit_fleet: null argument");
+        stp::QgFleetOut r;
+        backend(ctx).qpgraph_set_fit_fleet(*s->s, options ? options->numstart : 10, fit_options(options), r);
+        auto put = [](const auto& v, auto* dst) {
+            if (dst) std::copy(v.begin(), v.end(), dst);
+        };
+        put(r.score, score);
+        put(r.restart_spread, restart_spread);
+        put(r.theta, theta);
+        put(r.theta_lo, theta_lo);
+        put(r.theta_hi, theta_hi);
+        put(r.edge_length, edge_length);
+        put(r.f3_fit, f3_fit);
+        put(r.iterations, iterations);
+        put(r.status, status);
+    });
+}
+
 int stp_f2bin_write(const char* path, const double* f2, const double* vpair, const int32_t* block_sizes,
                     int32_t P, int32_t n_block) {
     return guard([&] {

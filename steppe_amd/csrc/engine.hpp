@@ -96,6 +96,21 @@ struct QgSet {
     DevBuf<double> in, out;
 };
 
+// Options of the device qpGraph fit (projected L-BFGS per instance).
+struct QgFitOptions {
+    int maxit = 200;
+    double gtol = 1e-11, xtol = 1e-12, ftol = 1e-14;
+    double fudge = 1e-4;
+    int constrained = 1;
+};
+
+// Per-graph results of a multistart fit of every graph of a set (row strides: dmax for
+// the theta arrays, ne_max for edge_length, np_max for f3_fit).
+struct QgFleetOut {
+    std::vector<double> score, restart_spread, theta, theta_lo, theta_hi, edge_length, f3_fit;
+    std::vector<std::int32_t> status, iterations;
+};
+
 class HipBackend {
 public:
     explicit HipBackend(int device);
@@ -133,10 +148,20 @@ public:
     // qpGraph (qpgraph_host.cpp): build a resident set from topologies and f2 handles
     // (the f3 basis and covariance run on the device), evaluate scores / adjoints.
     QgSet* qpgraph_set_create(const std::vector<const QgTopology*>& topos, const F2Blocks* const* f2, double diag_f3);
+    // A set from a given f3 basis and Qinv per graph (concatenated in graph order:
+    // npair values, then npair x npair row-major), as the reference's fleet fit receives.
+    QgSet* qpgraph_set_from_basis(const std::vector<const QgTopology*>& topos, const double* f_obs,
+                                  const double* qinv);
     void qpgraph_set_eval(QgSet& s, int n_inst, const std::int32_t* inst_graph, const double* theta,
                           const double* bl_in, int mode, double fudge, int constrained, double* score,
                           double* grad_theta, double* grad_bl, double* bl_out, double* fit_out);
     void qpgraph_set_basis(const QgSet& s, int g, double* f_obs, double* qdiag, double* qinv);
+    // Device L-BFGS fits of instances from given start points (theta0[i*dmax ..]).
+    void qpgraph_set_fit(QgSet& s, int n_inst, const std::int32_t* inst_graph, const double* theta0,
+                         const QgFitOptions& o, double* theta_out, double* score_out, std::int32_t* iters_out);
+    // numstart restarts per graph from the reference's start points, best per graph,
+    // then its edge lengths and fitted f3 (the reference's qpgraph_fit_fleet per graph).
+    void qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOptions& o, QgFleetOut& out);
 
     // Per-stage device timing with HIP events on this backend's stream (opt-in).
     enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kQpgraph = 5, kStages = 6 };
@@ -148,6 +173,7 @@ public:
                      const std::int8_t* ind_haploid, std::uint8_t* d_packed);
 
 private:
+    void qpgraph_upload_topologies(QgSet& s, const std::vector<const QgTopology*>& topos);
     struct PlanDev {
         DevBuf<std::int64_t> s0, k0;
         DevBuf<std::int32_t> len, kpad, chunk_block;

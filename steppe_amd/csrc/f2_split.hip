@@ -57,9 +57,10 @@ constexpr int kPlanes = 13;            // d0..d5 (0-5), e0..e5 (6-11), V (12)
 constexpr int kVPlane = 12;
 constexpr int kChunk = 1024;           // bytes per (32-row group, 16-SNP step, plane)
 constexpr int kGroups = 8;             // 32-row groups per workgroup tile: 4 left + 4 right
-constexpr int kSlotBytes = kGroups * 6 * kChunk;   // 48 KB (largest phase: 6 planes)
-constexpr int kSlots = 3;
-constexpr int kDPlanes = 6, kEPlanes = 4;  // planes per phase: q digits; 3 w digits + V
+constexpr int kDPlanes = 6, kE1Planes = 3, kE2Planes = 4;  // planes per phase: q digits; e0..e2; e3..e5 + V
+constexpr int kDSlotBytes = kGroups * kDPlanes * kChunk;    // 48 KB
+constexpr int kE1SlotBytes = kGroups * kE1Planes * kChunk;  // 24 KB
+constexpr int kE2SlotBytes = kGroups * kE2Planes * kChunk;  // 32 KB
 constexpr int kWaves = 8;
 
 // ---------------------------------------------------------------------------------
@@ -323,11 +324,16 @@ __device__ inline void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
-// Phase kinds of a K-step: 0 = q digits (planes 0-5), 1 = e0..e2 + V, 2 = e3..e5 + V.
+// Phase kinds of a K-step: 0 = q digits d0..d5 (planes 0-5), 1 = e0..e2 (planes 6-8),
+// 2 = e3..e5 + V (planes 9-11, 12).
+template <int kind>
+constexpr int phase_planes() {
+    return kind == 0 ? kDPlanes : (kind == 1 ? kE1Planes : kE2Planes);
+}
 __device__ inline int phase_plane(int kind, int pp) {
     if (kind == 0) return pp;
-    if (pp == 3) return kVPlane;
-    return (kind == 1 ? 6 : 9) + pp;
+    if (kind == 1) return 6 + pp;
+    return pp < 3 ? 9 + pp : kVPlane;
 }
 
 // Issue this wavefront's share of one phase's 1 KB chunks (direct global->LDS).
@@ -337,7 +343,7 @@ __device__ inline int phase_plane(int kind, int pp) {
 template <int kind>
 __device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, std::int64_t t16,
                                    std::int64_t KS16, int gI0, int gJ0, int G32, int wave, int lane) {
-    constexpr int np = kind == 0 ? kDPlanes : kEPlanes;
+    constexpr int np = phase_planes<kind>();
 #pragma unroll
     for (int r = 0; r < np; ++r) {
         const int ci = wave + kWaves * r;
@@ -358,10 +364,12 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                                                        std::int64_t KS16, const double* __restrict__ vpair,
                                                        double* __restrict__ f2) {
     // One LDS object per ring slot: the compiler then knows that reads of one slot do
-    // not depend on the direct loads still landing in the others.
-    __shared__ __attribute__((aligned(16))) std::uint8_t lds_d[kSlotBytes];
-    __shared__ __attribute__((aligned(16))) std::uint8_t lds_e1[kSlotBytes];
-    __shared__ __attribute__((aligned(16))) std::uint8_t lds_e2[kSlotBytes];
+    // not depend on the direct loads still landing in the others.  Two q-digit slots
+    // alternate between K-steps; one slot each for e0..e2 and for e3..e5 + V.
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_d0[kDSlotBytes];
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_d1[kDSlotBytes];
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_e1[kE1SlotBytes];
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_e2[kE2SlotBytes];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     // Wavefront-uniform values are moved to scalar registers so that every branch
@@ -370,9 +378,7 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
     const int wa = wave & 1;   // left sub-tile: groups 2wa, 2wa+1 of the left tile
     const int wc = wave >> 1;  // right sub-tile: group wc of the right tile
     const std::int64_t slab = static_cast<std::int64_t>(P) * P;
-    std::uint8_t* const slot_d = lds_d;
-    std::uint8_t* const slot_e1 = lds_e1;
-    std::uint8_t* const slot_e2 = lds_e2;
+    const std::int64_t t_last = KS16 - 1;
 
     const int home = static_cast<int>(blockIdx.x % kXcds);
     int xq = 0;
@@ -388,6 +394,9 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
         const int gJ0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(wi.tj));
         const std::int64_t t0 = bt.k0[b] / 16;
         const int nsteps = __builtin_amdgcn_readfirstlane(bt.kpad[b] / 16);
+        // K-step index of step s, clamped: look-ahead loads past the block end (and the
+        // buffer end) re-read a valid K-step and are never read back.
+        auto kstep = [&](int s) { return t0 + s < t_last ? t0 + s : t_last; };
 
         bool act[2];
 #pragma unroll
@@ -404,24 +413,26 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
             H[x] = f16v{};
         }
 
-        // Prologue: the first K-step's first two phases in flight (the slot of a
-        // phase is its kind, since a K-step has three phases and the ring three slots).
-        // Every phase stages unconditionally (steps past the block end re-read the last
-        // K-step of the buffer and are never read back), so every wavefront issues the
-        // same number of loads between any two waits on every path: the explicit waits
-        // below are then exact and the compiler adds none of its own.
-        const std::int64_t t_last = KS16 - 1;
-        stage_phase<0>(planes, slot_d, t0 < t_last ? t0 : t_last, KS16, gI0, gJ0, G32, wave, lane);
-        stage_phase<1>(planes, slot_e1, t0 < t_last ? t0 : t_last, KS16, gI0, gJ0, G32, wave, lane);
+        // Load schedule (per wavefront; all loads stage unconditionally, so every path
+        // issues the same counts and the explicit waits below are exact):
+        //   prologue:        D(0) -> d0, E1(0), D(1) -> d1
+        //   step s, phase d:  wait until <= 9 loads are outstanding (D(s) done), barrier,
+        //                     issue E2(s); q-digit products from D(s)
+        //   phase e1:         wait for all (E1(s), E2(s)), barrier, issue D(s+2) into
+        //                     D(s)'s slot; e0..e2 products (V from E2)
+        //   phase e2:         barrier (E1 free), issue E1(s+1); e3..e5 products
+        // D(s) is in flight for a step and a half, E1 for phases e2 + d, E2 for phase d.
+        stage_phase<0>(planes, lds_d0, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
+        stage_phase<1>(planes, lds_e1, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
+        stage_phase<0>(planes, lds_d1, kstep(1), KS16, gI0, gJ0, G32, wave, lane);
 
-        for (int step = 0; step < nsteps; ++step) {
-            const bool more = step + 1 < nsteps;
-            const std::int64_t tn = t0 + step + 1 < t_last ? t0 + step + 1 : t_last;
-            // ---- q digits.  Outstanding: this phase (6 loads), the e1 phase (4).
-            wait_vmcnt<4>();
+        auto step_body = [&]<int par>(int step) {
+            std::uint8_t* const slot_d = par ? lds_d1 : lds_d0;
+            // ---- q digits
+            wait_vmcnt<9>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            stage_phase<2>(planes, slot_e2, t0 + step, KS16, gI0, gJ0, G32, wave, lane);
+            stage_phase<2>(planes, lds_e2, kstep(step), KS16, gI0, gJ0, G32, wave, lane);
             {
                 bf8 dj[6];
 #pragma unroll
@@ -441,51 +452,49 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                     }
                 }
             }
-            // ---- e0..e2 + V.  Outstanding: this phase (4), the e2 phase (4).
-            wait_vmcnt<4>();
+            // ---- e0..e2 (V from the e3..e5 slot)
+            wait_vmcnt<0>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            stage_phase<0>(planes, slot_d, tn, KS16, gI0, gJ0, G32, wave, lane);
+            stage_phase<0>(planes, slot_d, kstep(step + 2), KS16, gI0, gJ0, G32, wave, lane);
             {
                 bf8 ej[3];
 #pragma unroll
-                for (int m = 0; m < 3; ++m) ej[m] = lds_frag(slot_e1, 4 + wc, m, kEPlanes, lane);
-                const bf8 vj = lds_frag(slot_e1, 4 + wc, 3, kEPlanes, lane);
+                for (int m = 0; m < 3; ++m) ej[m] = lds_frag(lds_e1, 4 + wc, m, kE1Planes, lane);
+                const bf8 vj = lds_frag(lds_e2, 4 + wc, 3, kE2Planes, lane);
 #pragma unroll
                 for (int x = 0; x < 2; ++x) {
                     if (!act[x]) continue;
 #pragma unroll
                     for (int m = 0; m < 3; ++m) {
-                        const bf8 ei = lds_frag(slot_e1, 2 * wa + x, m, kEPlanes, lane);
+                        const bf8 ei = lds_frag(lds_e1, 2 * wa + x, m, kE1Planes, lane);
                         L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][m], 0, 0, 0);
                     }
-                    const bf8 vi = lds_frag(slot_e1, 2 * wa + x, 3, kEPlanes, lane);
+                    const bf8 vi = lds_frag(lds_e2, 2 * wa + x, 3, kE2Planes, lane);
 #pragma unroll
                     for (int m = 0; m < 3; ++m)
                         L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vi, ej[m], L[x][m], 0, 0, 0);
                 }
             }
-            // ---- e3..e5 + V (e5 pre-scaled into level 4).  Outstanding: this phase
-            // and the next q phase (6).
-            wait_vmcnt<6>();
+            // ---- e3..e5 + V (e5 pre-scaled into level 4)
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            stage_phase<1>(planes, slot_e1, tn, KS16, gI0, gJ0, G32, wave, lane);
+            stage_phase<1>(planes, lds_e1, kstep(step + 1), KS16, gI0, gJ0, G32, wave, lane);
             {
                 bf8 ej[3];
 #pragma unroll
-                for (int m = 0; m < 3; ++m) ej[m] = lds_frag(slot_e2, 4 + wc, m, kEPlanes, lane);
-                const bf8 vj = lds_frag(slot_e2, 4 + wc, 3, kEPlanes, lane);
+                for (int m = 0; m < 3; ++m) ej[m] = lds_frag(lds_e2, 4 + wc, m, kE2Planes, lane);
+                const bf8 vj = lds_frag(lds_e2, 4 + wc, 3, kE2Planes, lane);
 #pragma unroll
                 for (int x = 0; x < 2; ++x) {
                     if (!act[x]) continue;
 #pragma unroll
                     for (int m = 0; m < 3; ++m) {
-                        const bf8 ei = lds_frag(slot_e2, 2 * wa + x, m, kEPlanes, lane);
+                        const bf8 ei = lds_frag(lds_e2, 2 * wa + x, m, kE2Planes, lane);
                         const int t = m == 2 ? 4 : 3 + m;
                         L[x][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][t], 0, 0, 0);
                     }
-                    const bf8 vi = lds_frag(slot_e2, 2 * wa + x, 3, kEPlanes, lane);
+                    const bf8 vi = lds_frag(lds_e2, 2 * wa + x, 3, kE2Planes, lane);
 #pragma unroll
                     for (int m = 0; m < 3; ++m) {
                         const int t = m == 2 ? 4 : 3 + m;
@@ -493,7 +502,7 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                     }
                 }
                 // Every 8 K-steps (128 SNPs) and at the block end: exact carries.
-                if ((step & 7) == 7 || !more) {
+                if ((step & 7) == 7 || step + 1 == nsteps) {
 #pragma unroll
                     for (int x = 0; x < 2; ++x) {
                         if (!act[x]) continue;
@@ -512,7 +521,16 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                     }
                 }
             }
+        };
+        // Steps alternate between the two q-digit slots; unrolled by two so that each
+        // slot is a distinct LDS object in every access, with an odd last step after
+        // the loop (so that both bodies always run inside it).
+        int step = 0;
+        for (; step + 1 < nsteps; step += 2) {
+            step_body.template operator()<0>(step);
+            step_body.template operator()<1>(step + 1);
         }
+        if (step < nsteps) step_body.template operator()<0>(step);
 
         // Epilogue: num = -2 sum_t U_t L_t in FP64; f2 = num / vpair.
         double* f2b = f2 + slab * b;

@@ -170,6 +170,259 @@ __device__ bool nnls_wave(int n, const double* C, const double* sc, const double
     return true;
 }
 
+// Per-workgroup state of one instance evaluation (LDS).
+struct EvalShared {
+    int* passive;    // [64] NNLS passive set
+    int* slot;       // [64] compaction scratch
+    double* theta;   // [kMaxD] the instance's mixture weights (input)
+    double* thbar;   // [kMaxD] d score / d theta (mode >= 1)
+    double* scal;    // [4]: 0 kappa, 1 score
+    int* ok;
+};
+
+// Evaluates one instance; every thread of the workgroup calls it.  The score of
+// sh.theta goes to sh.scal[1], the edge lengths to L.bl (in the joint mode, mode 2,
+// L.bl holds the given edge lengths on entry), for mode >= 1 the adjoint to sh.thbar,
+// and in the joint mode d score / d bl to L.blbar.  fit (global, may be null) receives
+// P bl.  Returns false on every thread when an edge-length system is singular; the
+// score is then +inf.
+template <int T>
+__device__ bool qg_eval_one(const QgEvalArgs& g, const QgTopoDev& tp, const double* __restrict__ G,
+                            const double* __restrict__ f, int mode, const EvalLds& L, const EvalShared& sh,
+                            double* fit) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const std::int32_t* ia = g.iarena;
+    const int np = tp.npair, ne = tp.ne, npop = tp.npop, ncc = npop - 1, D = tp.nadmix, npath = tp.npath;
+    // ---- leaf weights from the path tables ----
+    for (int p = tid; p < npath; p += kThreads) {
+        double w = 1.0;
+        for (int t = ia[tp.o_pae_off + p]; t < ia[tp.o_pae_off + p + 1]; ++t) {
+            const int id = ia[tp.o_pae_id + t];
+            const double th = sh.theta[(id - 1) / 2];
+            w *= (id % 2 == 1) ? th : (1.0 - th);
+        }
+        L.path[p] = w;
+    }
+    for (int x = tid; x < ne * npop; x += kThreads) L.pw[x] = g.darena[tp.o_pwts0 + x];
+    __syncthreads();
+    for (int gr = tid; gr < tp.ngrp; gr += kThreads) {
+        double s = 0.0;
+        for (int t = ia[tp.o_grp_off + gr]; t < ia[tp.o_grp_off + gr + 1]; ++t) s += L.path[ia[tp.o_grp_paths + t]];
+        L.pw[ia[tp.o_grp_edge + gr] + ne * ia[tp.o_grp_leaf + gr]] = s;
+    }
+    __syncthreads();
+    for (int x = tid; x < ne * ncc; x += kThreads) {
+        const int col = x / ne, e = x - (x / ne) * ne;
+        const int leaf = col < tp.base ? col : col + 1;
+        L.pc[x] = L.pw[e + ne * leaf] - L.pw[e + ne * tp.base];
+    }
+    __syncthreads();
+    for (int x = tid; x < np * ne; x += kThreads) {
+        const int k = x / ne, e = x - (x / ne) * ne;
+        L.P[x] = L.pc[e + ne * ia[tp.o_cmb1 + k]] * L.pc[e + ne * ia[tp.o_cmb2 + k]];
+    }
+    __syncthreads();
+    if (mode != 2) {
+        // ---- W = G P, qf = G f ----
+        for (int x = tid; x < np * ne; x += kThreads) {
+            const int r = x / ne, e = x - (x / ne) * ne;
+            const double* Gr = G + static_cast<std::int64_t>(r) * np;
+            double acc = 0.0;
+            for (int c = 0; c < np; ++c) acc += Gr[c] * L.P[c * ne + e];
+            L.W[x] = acc;
+        }
+        for (int r = tid; r < np; r += kThreads) {
+            const double* Gr = G + static_cast<std::int64_t>(r) * np;
+            double acc = 0.0;
+            for (int c = 0; c < np; ++c) acc += Gr[c] * f[c];
+            L.qf[r] = acc;
+        }
+        __syncthreads();
+        // ---- cc = P^T W, q = P^T qf ----
+        for (int x = tid; x < ne * ne; x += kThreads) {
+            const int e1 = x / ne, e2 = x - (x / ne) * ne;
+            double acc = 0.0;
+            for (int r = 0; r < np; ++r) acc += L.P[r * ne + e1] * L.W[r * ne + e2];
+            L.C[x] = acc;
+        }
+        for (int e1 = tid; e1 < ne; e1 += kThreads) {
+            double rr = 0.0;
+            for (int r = 0; r < np; ++r) rr += L.P[r * ne + e1] * L.qf[r];
+            L.q[e1] = rr;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double trm = 0.0;
+            for (int e = 0; e < ne; ++e) trm += L.C[e * ne + e];
+            const double ridge = g.fudge * trm / static_cast<double>(ne);
+            for (int e = 0; e < ne; ++e) L.C[e * ne + e] += ridge;
+            *sh.ok = 1;
+        }
+        __syncthreads();
+        for (int e = tid; e < ne; e += kThreads) {
+            L.sc[e] = sqrt(L.C[e * ne + e]);
+            L.q1[e] = L.q[e] / L.sc[e];
+        }
+        __syncthreads();
+        // ---- edge lengths (wavefront 0) ----
+        if (wave == 0) {
+            bool ok;
+            if (g.constrained) {
+                ok = nnls_wave<T>(ne, L.C, L.sc, L.q1, L.lam, sh.passive, sh.slot, L.prow);
+            } else {
+                double z = 0.0;
+                ok = solve_sub<T>(L.C, ne, nullptr, ne, false, L.sc, lane < ne ? L.q1[lane] : 0.0, z, L.prow);
+                if (lane < ne) {
+                    L.lam[lane] = z;
+                    sh.passive[lane] = 1;
+                }
+            }
+            if (lane == 0 && !ok) *sh.ok = 0;
+        }
+        __syncthreads();
+        if (!*sh.ok) {
+            __syncthreads();
+            if (tid == 0) sh.scal[1] = __builtin_inf();
+            __syncthreads();
+            return false;
+        }
+        for (int e = tid; e < ne; e += kThreads) L.bl[e] = L.lam[e] / L.sc[e];
+    }
+    __syncthreads();
+    // ---- residual and score ----
+    for (int k = tid; k < np; k += kThreads) {
+        double fv = 0.0;
+        for (int e = 0; e < ne; ++e) fv += L.P[k * ne + e] * L.bl[e];
+        L.res[k] = f[k] - fv;
+        if (fit) fit[k] = fv;
+    }
+    __syncthreads();
+    for (int a = tid; a < np; a += kThreads) {
+        const double* Ga = G + static_cast<std::int64_t>(a) * np;
+        double row = 0.0, col = 0.0;
+        for (int b = 0; b < np; ++b) row += Ga[b] * L.res[b];
+        L.Gr[a] = row;
+        if (mode != 0) {
+            for (int b = 0; b < np; ++b) col += G[static_cast<std::int64_t>(b) * np + a] * L.res[b];
+            L.GTr[a] = col;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int a = 0; a < np; ++a) s += L.res[a] * L.Gr[a];
+        sh.scal[1] = s;
+    }
+    if (mode == 0) {
+        __syncthreads();
+        return true;
+    }
+    // ---- adjoint ----
+    for (int k = tid; k < np; k += kThreads) L.u[k] = L.Gr[k] + L.GTr[k];  // rbar
+    __syncthreads();
+    for (int e = tid; e < ne; e += kThreads) {
+        double s = 0.0;
+        for (int k = 0; k < np; ++k) s += L.P[k * ne + e] * L.u[k];
+        L.blbar[e] = -s;
+    }
+    __syncthreads();
+    if (mode == 2) {
+        for (int x = tid; x < np * ne; x += kThreads) {
+            const int k = x / ne, e = x - (x / ne) * ne;
+            L.W[x] = -L.u[k] * L.bl[e];
+        }
+    } else {
+        // lambda = C_AA^-T blbar_A on wavefront 0
+        if (wave == 0) {
+            const bool pas = lane < ne && sh.passive[lane] != 0;
+            const unsigned long long mask = __ballot(pas);
+            const int nA = __builtin_popcountll(mask);
+            if (pas) sh.slot[__builtin_popcountll(mask & ((1ull << lane) - 1ull))] = lane;
+            __builtin_amdgcn_wave_barrier();
+            const int ea = lane < nA ? sh.slot[lane] : 0;
+            double z = 0.0;
+            const bool ok =
+                nA == 0 || solve_sub<T>(L.C, ne, sh.slot, nA, true, nullptr, lane < nA ? L.blbar[ea] : 0.0, z, L.prow);
+            if (lane < ne) L.lam[lane] = 0.0;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < nA) L.lam[ea] = ok ? z : 0.0;
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                double rho = 0.0;
+                for (int e = 0; e < ne; ++e) rho -= L.lam[e] * L.bl[e];
+                sh.scal[0] = g.fudge / static_cast<double>(ne) * rho;  // kappa
+            }
+        }
+        __syncthreads();
+        const double kappa = sh.scal[0];
+        for (int c = tid; c < np; c += kThreads) {
+            double s = 0.0;
+            for (int e = 0; e < ne; ++e) s += L.P[c * ne + e] * L.lam[e];
+            L.v[c] = s;  // P lambda
+        }
+        __syncthreads();
+        for (int k = tid; k < np; k += kThreads) {
+            double s = 0.0;
+            for (int c = 0; c < np; ++c) s += G[static_cast<std::int64_t>(c) * np + k] * L.v[c];
+            L.qf[k] = s;  // G^T P lambda
+        }
+        __syncthreads();
+        for (int x = tid; x < np * ne; x += kThreads) {
+            const int k = x / ne, e = x - (x / ne) * ne;
+            double w2 = 0.0;
+            for (int c = 0; c < np; ++c) w2 += G[static_cast<std::int64_t>(c) * np + k] * L.P[c * ne + e];
+            L.W[x] = -L.bl[e] * (L.u[k] + L.qf[k]) + L.lam[e] * L.Gr[k] + kappa * (L.W[x] + w2);
+        }
+    }
+    __syncthreads();
+    // ---- Pbar -> pcbar -> pwbar -> pathbar -> thetabar ----
+    for (int e = tid; e < ne; e += kThreads) {
+        for (int c = 0; c < ncc; ++c) L.pcbar[e + ne * c] = 0.0;
+        for (int k = 0; k < np; ++k) {
+            const int a = ia[tp.o_cmb1 + k], b = ia[tp.o_cmb2 + k];
+            const double pb = L.W[k * ne + e];
+            L.pcbar[e + ne * a] += pb * L.pc[e + ne * b];
+            L.pcbar[e + ne * b] += pb * L.pc[e + ne * a];
+        }
+        double sum = 0.0;
+        for (int c = 0; c < ncc; ++c) {
+            const int leaf = c < tp.base ? c : c + 1;
+            L.pw[e + ne * leaf] = L.pcbar[e + ne * c];
+            sum += L.pcbar[e + ne * c];
+        }
+        L.pw[e + ne * tp.base] = -sum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int p = 0; p < npath; ++p) L.pathbar[p] = 0.0;
+        for (int gr = 0; gr < tp.ngrp; ++gr) {
+            const double v = L.pw[ia[tp.o_grp_edge + gr] + ne * ia[tp.o_grp_leaf + gr]];
+            for (int t = ia[tp.o_grp_off + gr]; t < ia[tp.o_grp_off + gr + 1]; ++t) L.pathbar[ia[tp.o_grp_paths + t]] += v;
+        }
+        for (int d = 0; d < D; ++d) sh.thbar[d] = 0.0;
+        for (int p = 0; p < npath; ++p) {
+            const double pb = L.pathbar[p];
+            if (pb == 0.0) continue;
+            const int t0 = ia[tp.o_pae_off + p], t1 = ia[tp.o_pae_off + p + 1];
+            for (int t = t0; t < t1; ++t) {
+                double others = 1.0;
+                for (int u = t0; u < t1; ++u) {
+                    if (u == t) continue;
+                    const int id = ia[tp.o_pae_id + u];
+                    const double th = sh.theta[(id - 1) / 2];
+                    others *= (id % 2 == 1) ? th : (1.0 - th);
+                }
+                const int id = ia[tp.o_pae_id + t];
+                sh.thbar[(id - 1) / 2] += (id % 2 == 1 ? pb : -pb) * others;
+            }
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
 template <int T>
 __global__ __launch_bounds__(kThreads) void qg_eval_kernel(QgEvalArgs g) {
     extern __shared__ __attribute__((aligned(16))) double lds[];
@@ -177,10 +430,8 @@ __global__ __launch_bounds__(kThreads) void qg_eval_kernel(QgEvalArgs g) {
     __shared__ int s_passive[64], s_slot[64];
     __shared__ double s_theta[kMaxD], s_thbar[kMaxD], s_scal[4];
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const EvalLds L = eval_layout(lds, g);
-    const std::int32_t* ia = g.iarena;
+    const EvalShared sh{s_passive, s_slot, s_theta, s_thbar, s_scal, &s_ok};
     for (;;) {
         if (tid == 0) s_inst = static_cast<int>(atomicAdd(g.counter, 1u));
         __syncthreads();
@@ -188,240 +439,207 @@ __global__ __launch_bounds__(kThreads) void qg_eval_kernel(QgEvalArgs g) {
         __syncthreads();
         if (i >= g.n_inst) break;
         const QgTopoDev tp = g.topo[g.inst_topo[i]];
-        const int np = tp.npair, ne = tp.ne, npop = tp.npop, ncc = npop - 1, D = tp.nadmix, npath = tp.npath;
+        const int ne = tp.ne, D = tp.nadmix;
         const double* G = g.qinv + g.inst_qinv[i];
         const double* f = g.fobs + g.inst_fobs[i];
         if (tid < D) s_theta[tid] = g.theta[static_cast<std::int64_t>(i) * g.dmax + tid];
-        __syncthreads();
-        // ---- leaf weights from the path tables ----
-        for (int p = tid; p < npath; p += kThreads) {
-            double w = 1.0;
-            for (int t = ia[tp.o_pae_off + p]; t < ia[tp.o_pae_off + p + 1]; ++t) {
-                const int id = ia[tp.o_pae_id + t];
-                const double th = s_theta[(id - 1) / 2];
-                w *= (id % 2 == 1) ? th : (1.0 - th);
-            }
-            L.path[p] = w;
-        }
-        for (int x = tid; x < ne * npop; x += kThreads) L.pw[x] = g.darena[tp.o_pwts0 + x];
-        __syncthreads();
-        for (int gr = tid; gr < tp.ngrp; gr += kThreads) {
-            double s = 0.0;
-            for (int t = ia[tp.o_grp_off + gr]; t < ia[tp.o_grp_off + gr + 1]; ++t) s += L.path[ia[tp.o_grp_paths + t]];
-            L.pw[ia[tp.o_grp_edge + gr] + ne * ia[tp.o_grp_leaf + gr]] = s;
-        }
-        __syncthreads();
-        for (int x = tid; x < ne * ncc; x += kThreads) {
-            const int col = x / ne, e = x - (x / ne) * ne;
-            const int leaf = col < tp.base ? col : col + 1;
-            L.pc[x] = L.pw[e + ne * leaf] - L.pw[e + ne * tp.base];
-        }
-        __syncthreads();
-        for (int x = tid; x < np * ne; x += kThreads) {
-            const int k = x / ne, e = x - (x / ne) * ne;
-            L.P[x] = L.pc[e + ne * ia[tp.o_cmb1 + k]] * L.pc[e + ne * ia[tp.o_cmb2 + k]];
-        }
-        __syncthreads();
-        if (g.mode != 2) {
-            // ---- W = G P, qf = G f ----
-            for (int x = tid; x < np * ne; x += kThreads) {
-                const int r = x / ne, e = x - (x / ne) * ne;
-                const double* Gr = G + static_cast<std::int64_t>(r) * np;
-                double acc = 0.0;
-                for (int c = 0; c < np; ++c) acc += Gr[c] * L.P[c * ne + e];
-                L.W[x] = acc;
-            }
-            for (int r = tid; r < np; r += kThreads) {
-                const double* Gr = G + static_cast<std::int64_t>(r) * np;
-                double acc = 0.0;
-                for (int c = 0; c < np; ++c) acc += Gr[c] * f[c];
-                L.qf[r] = acc;
-            }
-            __syncthreads();
-            // ---- cc = P^T W, q = P^T qf ----
-            for (int x = tid; x < ne * ne; x += kThreads) {
-                const int e1 = x / ne, e2 = x - (x / ne) * ne;
-                double acc = 0.0;
-                for (int r = 0; r < np; ++r) acc += L.P[r * ne + e1] * L.W[r * ne + e2];
-                L.C[x] = acc;
-            }
-            for (int e1 = tid; e1 < ne; e1 += kThreads) {
-                double rr = 0.0;
-                for (int r = 0; r < np; ++r) rr += L.P[r * ne + e1] * L.qf[r];
-                L.q[e1] = rr;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                double trm = 0.0;
-                for (int e = 0; e < ne; ++e) trm += L.C[e * ne + e];
-                const double ridge = g.fudge * trm / static_cast<double>(ne);
-                for (int e = 0; e < ne; ++e) L.C[e * ne + e] += ridge;
-                s_ok = 1;
-            }
-            __syncthreads();
-            for (int e = tid; e < ne; e += kThreads) {
-                L.sc[e] = sqrt(L.C[e * ne + e]);
-                L.q1[e] = L.q[e] / L.sc[e];
-            }
-            __syncthreads();
-            // ---- edge lengths (wavefront 0) ----
-            if (wave == 0) {
-                bool ok;
-                if (g.constrained) {
-                    ok = nnls_wave<T>(ne, L.C, L.sc, L.q1, L.lam, s_passive, s_slot, L.prow);
-                } else {
-                    double z = 0.0;
-                    ok = solve_sub<T>(L.C, ne, nullptr, ne, false, L.sc, lane < ne ? L.q1[lane] : 0.0, z, L.prow);
-                    if (lane < ne) {
-                        L.lam[lane] = z;
-                        s_passive[lane] = 1;
-                    }
-                }
-                if (lane == 0 && !ok) s_ok = 0;
-            }
-            __syncthreads();
-            if (!s_ok) {
-                if (tid == 0) g.score[i] = __builtin_inf();
-                __syncthreads();
-                continue;
-            }
-            for (int e = tid; e < ne; e += kThreads) L.bl[e] = L.lam[e] / L.sc[e];
-        } else {
+        if (g.mode == 2)
             for (int e = tid; e < ne; e += kThreads) L.bl[e] = g.bl_in[static_cast<std::int64_t>(i) * g.emax + e];
+        __syncthreads();
+        double* fit = g.fit_out ? g.fit_out + static_cast<std::int64_t>(i) * g.np_max : nullptr;
+        const bool ok = qg_eval_one<T>(g, tp, G, f, g.mode, L, sh, fit);
+        if (tid == 0) g.score[i] = s_scal[1];
+        if (ok) {
+            if (g.bl_out)
+                for (int e = tid; e < ne; e += kThreads) g.bl_out[static_cast<std::int64_t>(i) * g.emax + e] = L.bl[e];
+            if (g.mode != 0 && tid < D) g.grad_theta[static_cast<std::int64_t>(i) * g.dmax + tid] = s_thbar[tid];
+            if (g.mode == 2)
+                for (int e = tid; e < ne; e += kThreads)
+                    g.grad_bl[static_cast<std::int64_t>(i) * g.emax + e] = L.blbar[e];
         }
         __syncthreads();
-        // ---- residual and score ----
-        for (int k = tid; k < np; k += kThreads) {
-            double fit = 0.0;
-            for (int e = 0; e < ne; ++e) fit += L.P[k * ne + e] * L.bl[e];
-            L.res[k] = f[k] - fit;
-            if (g.fit_out) g.fit_out[static_cast<std::int64_t>(i) * g.np_max + k] = fit;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Projected L-BFGS on [0, 1]^D, one instance per workgroup, to convergence on the
+// device (replacing the reference's finite-difference coordinate search,
+// cpu_backend.cpp:2936-3045).  Per iteration: variables at a bound whose gradient
+// points outward are held; the two-loop recursion over the last kHist pairs gives the
+// direction on the free variables (steepest descent when it is not a descent
+// direction, with the history reset); the projected step is accepted by Armijo
+// backtracking (c1 = 1e-4, halving, at most 30 trials).  Stops when the projected
+// gradient is below gtol, the step below xtol, the relative decrease below ftol, the
+// line search fails, or after maxit iterations.  The L-BFGS bookkeeping (D <= 16) runs
+// on one thread; every evaluation uses the whole workgroup.
+// ---------------------------------------------------------------------------------
+constexpr int kHist = 8;
+constexpr int kMaxBacktrack = 30;
+constexpr double kArmijo = 1e-4;
+
+template <int T>
+__global__ __launch_bounds__(kThreads) void qg_fit_kernel(QgEvalArgs g, QgFitArgs fa) {
+    extern __shared__ __attribute__((aligned(16))) double lds[];
+    __shared__ int s_inst, s_ok, s_stop, s_acc;
+    __shared__ int s_passive[64], s_slot[64];
+    __shared__ double s_theta[kMaxD], s_thbar[kMaxD], s_scal[4];
+    __shared__ double s_x[kMaxD], s_gx[kMaxD], s_d[kMaxD], s_S[kHist][kMaxD], s_Y[kHist][kMaxD], s_rho[kHist];
+    // thread 0's L-BFGS work vectors (LDS rather than registers: the evaluation
+    // already needs most of the register file)
+    __shared__ double s_pg[kMaxD], s_q[kMaxD], s_alpha[kHist];
+    __shared__ int s_held[kMaxD];
+    __shared__ double s_f;
+    const int tid = threadIdx.x;
+    const EvalLds L = eval_layout(lds, g);
+    const EvalShared sh{s_passive, s_slot, s_theta, s_thbar, s_scal, &s_ok};
+    for (;;) {
+        if (tid == 0) s_inst = static_cast<int>(atomicAdd(g.counter, 1u));
+        __syncthreads();
+        const int i = __builtin_amdgcn_readfirstlane(s_inst);
+        __syncthreads();
+        if (i >= g.n_inst) break;
+        const QgTopoDev tp = g.topo[g.inst_topo[i]];
+        const int D = tp.nadmix;
+        const double* G = g.qinv + g.inst_qinv[i];
+        const double* f = g.fobs + g.inst_fobs[i];
+        if (tid < D) {
+            const double x0 = fa.theta0[static_cast<std::int64_t>(i) * g.dmax + tid];
+            s_x[tid] = s_theta[tid] = x0 < 0.0 ? 0.0 : (x0 > 1.0 ? 1.0 : x0);
         }
         __syncthreads();
-        for (int a = tid; a < np; a += kThreads) {
-            const double* Ga = G + static_cast<std::int64_t>(a) * np;
-            double row = 0.0, col = 0.0;
-            for (int b = 0; b < np; ++b) row += Ga[b] * L.res[b];
-            L.Gr[a] = row;
-            if (g.mode != 0) {
-                for (int b = 0; b < np; ++b) col += G[static_cast<std::int64_t>(b) * np + a] * L.res[b];
-                L.GTr[a] = col;
+        int iters = 0, status = 0;
+        if (!qg_eval_one<T>(g, tp, G, f, D > 0 ? 1 : 0, L, sh, nullptr)) {
+            status = 1;
+        } else if (D > 0) {
+            if (tid == 0) {
+                s_f = s_scal[1];
+                for (int d = 0; d < D; ++d) s_gx[d] = s_thbar[d];
             }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double s = 0.0;
-            for (int a = 0; a < np; ++a) s += L.res[a] * L.Gr[a];
-            g.score[i] = s;
-        }
-        if (g.bl_out)
-            for (int e = tid; e < ne; e += kThreads) g.bl_out[static_cast<std::int64_t>(i) * g.emax + e] = L.bl[e];
-        if (g.mode == 0) {
-            __syncthreads();
-            continue;
-        }
-        // ---- adjoint ----
-        for (int k = tid; k < np; k += kThreads) L.u[k] = L.Gr[k] + L.GTr[k];  // rbar
-        __syncthreads();
-        for (int e = tid; e < ne; e += kThreads) {
-            double s = 0.0;
-            for (int k = 0; k < np; ++k) s += L.P[k * ne + e] * L.u[k];
-            L.blbar[e] = -s;
-        }
-        __syncthreads();
-        if (g.mode == 2) {
-            for (int e = tid; e < ne; e += kThreads) g.grad_bl[static_cast<std::int64_t>(i) * g.emax + e] = L.blbar[e];
-            for (int x = tid; x < np * ne; x += kThreads) {
-                const int k = x / ne, e = x - (x / ne) * ne;
-                L.W[x] = -L.u[k] * L.bl[e];
-            }
-        } else {
-            // lambda = C_AA^-T blbar_A on wavefront 0
-            if (wave == 0) {
-                const bool pas = lane < ne && s_passive[lane] != 0;
-                const unsigned long long mask = __ballot(pas);
-                const int nA = __builtin_popcountll(mask);
-                if (pas) s_slot[__builtin_popcountll(mask & ((1ull << lane) - 1ull))] = lane;
-                __builtin_amdgcn_wave_barrier();
-                const int ea = lane < nA ? s_slot[lane] : 0;
-                double z = 0.0;
-                const bool ok =
-                    nA == 0 || solve_sub<T>(L.C, ne, s_slot, nA, true, nullptr, lane < nA ? L.blbar[ea] : 0.0, z, L.prow);
-                if (lane < ne) L.lam[lane] = 0.0;
-                __builtin_amdgcn_wave_barrier();
-                if (lane < nA) L.lam[ea] = ok ? z : 0.0;
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0) {
-                    double rho = 0.0;
-                    for (int e = 0; e < ne; ++e) rho -= L.lam[e] * L.bl[e];
-                    s_scal[0] = g.fudge / static_cast<double>(ne) * rho;  // kappa
-                }
-            }
-            __syncthreads();
-            const double kappa = s_scal[0];
-            for (int c = tid; c < np; c += kThreads) {
-                double s = 0.0;
-                for (int e = 0; e < ne; ++e) s += L.P[c * ne + e] * L.lam[e];
-                L.v[c] = s;  // P lambda
-            }
-            __syncthreads();
-            for (int k = tid; k < np; k += kThreads) {
-                double s = 0.0;
-                for (int c = 0; c < np; ++c) s += G[static_cast<std::int64_t>(c) * np + k] * L.v[c];
-                L.qf[k] = s;  // G^T P lambda
-            }
-            __syncthreads();
-            for (int x = tid; x < np * ne; x += kThreads) {
-                const int k = x / ne, e = x - (x / ne) * ne;
-                double w2 = 0.0;
-                for (int c = 0; c < np; ++c) w2 += G[static_cast<std::int64_t>(c) * np + k] * L.P[c * ne + e];
-                L.W[x] = -L.bl[e] * (L.u[k] + L.qf[k]) + L.lam[e] * L.Gr[k] + kappa * (L.W[x] + w2);
-            }
-        }
-        __syncthreads();
-        // ---- Pbar -> pcbar -> pwbar -> pathbar -> thetabar ----
-        for (int e = tid; e < ne; e += kThreads) {
-            for (int c = 0; c < ncc; ++c) L.pcbar[e + ne * c] = 0.0;
-            for (int k = 0; k < np; ++k) {
-                const int a = ia[tp.o_cmb1 + k], b = ia[tp.o_cmb2 + k];
-                const double pb = L.W[k * ne + e];
-                L.pcbar[e + ne * a] += pb * L.pc[e + ne * b];
-                L.pcbar[e + ne * b] += pb * L.pc[e + ne * a];
-            }
-            double sum = 0.0;
-            for (int c = 0; c < ncc; ++c) {
-                const int leaf = c < tp.base ? c : c + 1;
-                L.pw[e + ne * leaf] = L.pcbar[e + ne * c];
-                sum += L.pcbar[e + ne * c];
-            }
-            L.pw[e + ne * tp.base] = -sum;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            for (int p = 0; p < npath; ++p) L.pathbar[p] = 0.0;
-            for (int gr = 0; gr < tp.ngrp; ++gr) {
-                const double v = L.pw[ia[tp.o_grp_edge + gr] + ne * ia[tp.o_grp_leaf + gr]];
-                for (int t = ia[tp.o_grp_off + gr]; t < ia[tp.o_grp_off + gr + 1]; ++t) L.pathbar[ia[tp.o_grp_paths + t]] += v;
-            }
-            for (int d = 0; d < D; ++d) s_thbar[d] = 0.0;
-            for (int p = 0; p < npath; ++p) {
-                const double pb = L.pathbar[p];
-                if (pb == 0.0) continue;
-                const int t0 = ia[tp.o_pae_off + p], t1 = ia[tp.o_pae_off + p + 1];
-                for (int t = t0; t < t1; ++t) {
-                    double others = 1.0;
-                    for (int u = t0; u < t1; ++u) {
-                        if (u == t) continue;
-                        const int id = ia[tp.o_pae_id + u];
-                        const double th = s_theta[(id - 1) / 2];
-                        others *= (id % 2 == 1) ? th : (1.0 - th);
+            int hist = 0;  // pairs stored so far (uniform: every thread tracks it)
+            for (int it = 0; it < fa.maxit; ++it) {
+                __syncthreads();
+                if (tid == 0) {
+                    int* held = s_held;
+                    double *pg = s_pg, *q = s_q, *alpha = s_alpha;
+                    double pgmax = 0.0;
+                    for (int d = 0; d < D; ++d) {
+                        held[d] = (s_x[d] <= 0.0 && s_gx[d] > 0.0) || (s_x[d] >= 1.0 && s_gx[d] < 0.0);
+                        pg[d] = held[d] ? 0.0 : s_gx[d];
+                        q[d] = pg[d];
+                        pgmax = fmax(pgmax, fabs(pg[d]));
                     }
-                    const int id = ia[tp.o_pae_id + t];
-                    s_thbar[(id - 1) / 2] += (id % 2 == 1 ? pb : -pb) * others;
+                    s_stop = pgmax <= fa.gtol;
+                    const int k = hist < kHist ? hist : kHist;
+                    for (int a = 0; a < k; ++a) {  // newest to oldest
+                        const int sl = (hist - 1 - a) % kHist;
+                        double sq = 0.0;
+                        for (int d = 0; d < D; ++d)
+                            if (!held[d]) sq += s_S[sl][d] * q[d];
+                        alpha[a] = s_rho[sl] * sq;
+                        for (int d = 0; d < D; ++d)
+                            if (!held[d]) q[d] -= alpha[a] * s_Y[sl][d];
+                    }
+                    double gamma = 1.0;
+                    if (hist > 0) {
+                        const int sl = (hist - 1) % kHist;
+                        double sy = 0.0, yy = 0.0;
+                        for (int d = 0; d < D; ++d)
+                            if (!held[d]) {
+                                sy += s_S[sl][d] * s_Y[sl][d];
+                                yy += s_Y[sl][d] * s_Y[sl][d];
+                            }
+                        if (yy > 0.0) gamma = sy / yy;
+                        if (!(gamma > 0.0)) gamma = 1.0;
+                    }
+                    for (int d = 0; d < D; ++d) q[d] *= gamma;
+                    for (int a = k - 1; a >= 0; --a) {  // oldest to newest
+                        const int sl = (hist - 1 - a) % kHist;
+                        double yr = 0.0;
+                        for (int d = 0; d < D; ++d)
+                            if (!held[d]) yr += s_Y[sl][d] * q[d];
+                        const double b = s_rho[sl] * yr;
+                        for (int d = 0; d < D; ++d)
+                            if (!held[d]) q[d] += (alpha[a] - b) * s_S[sl][d];
+                    }
+                    double slope = 0.0;
+                    for (int d = 0; d < D; ++d) {
+                        s_d[d] = held[d] ? 0.0 : -q[d];
+                        slope += s_d[d] * pg[d];
+                    }
+                    s_acc = 0;
+                    if (slope >= 0.0) {
+                        for (int d = 0; d < D; ++d) s_d[d] = -pg[d];
+                        s_acc = 1;  // history reset
+                    }
                 }
+                __syncthreads();
+                if (s_stop) break;
+                if (s_acc) hist = 0;
+                double step = 1.0;
+                bool accepted = false;
+                for (int bt = 0; bt < kMaxBacktrack; ++bt) {
+                    __syncthreads();
+                    if (tid < D) {
+                        const double xn = s_x[tid] + step * s_d[tid];
+                        s_theta[tid] = xn < 0.0 ? 0.0 : (xn > 1.0 ? 1.0 : xn);
+                    }
+                    __syncthreads();
+                    const bool ok = qg_eval_one<T>(g, tp, G, f, 1, L, sh, nullptr);
+                    if (tid == 0) {
+                        double dec = 0.0;
+                        for (int d = 0; d < D; ++d) dec += s_gx[d] * (s_theta[d] - s_x[d]);
+                        s_acc = ok && s_scal[1] <= s_f + kArmijo * dec;
+                    }
+                    __syncthreads();
+                    if (s_acc) {
+                        accepted = true;
+                        break;
+                    }
+                    step *= 0.5;
+                }
+                if (!accepted) break;
+                __syncthreads();
+                if (tid == 0) {
+                    double sy = 0.0, ss = 0.0, yy = 0.0, dx = 0.0;
+                    double *sv = s_pg, *yv = s_q;  // the direction's work vectors are free now
+                    for (int d = 0; d < D; ++d) {
+                        sv[d] = s_theta[d] - s_x[d];
+                        yv[d] = s_thbar[d] - s_gx[d];
+                        sy += sv[d] * yv[d];
+                        ss += sv[d] * sv[d];
+                        yy += yv[d] * yv[d];
+                        dx = fmax(dx, fabs(sv[d]));
+                    }
+                    s_acc = sy > 1e-16 * sqrt(ss) * sqrt(yy);
+                    if (s_acc) {
+                        const int sl = hist % kHist;
+                        for (int d = 0; d < D; ++d) {
+                            s_S[sl][d] = sv[d];
+                            s_Y[sl][d] = yv[d];
+                        }
+                        s_rho[sl] = 1.0 / sy;
+                    }
+                    const double fn = s_scal[1];
+                    const double df = s_f - fn;
+                    for (int d = 0; d < D; ++d) {
+                        s_x[d] = s_theta[d];
+                        s_gx[d] = s_thbar[d];
+                    }
+                    s_f = fn;
+                    s_stop = dx <= fa.xtol || df <= fa.ftol * (1.0 + fabs(fn));
+                }
+                __syncthreads();
+                if (s_acc) ++hist;
+                ++iters;
+                if (s_stop) break;
             }
-            for (int d = 0; d < D; ++d) g.grad_theta[static_cast<std::int64_t>(i) * g.dmax + d] = s_thbar[d];
         }
+        __syncthreads();
+        if (tid == 0) {
+            fa.score_out[i] = status ? __builtin_inf() : (D > 0 ? s_f : s_scal[1]);
+            fa.iters_out[i] = status ? -1 : iters;
+        }
+        if (tid < D) fa.theta_out[static_cast<std::int64_t>(i) * g.dmax + tid] = s_x[tid];
         __syncthreads();
     }
 }
@@ -535,6 +753,23 @@ void launch_qg_eval(int T, const QgEvalArgs& a, int n_workgroups, hipStream_t st
     switch (T) {
         case 32: go(qg_eval_kernel<32>); break;
         case 64: go(qg_eval_kernel<64>); break;
+        default: throw ArgError("qpgraph: unsupported edge-count bucket " + std::to_string(T));
+    }
+    STP_LAUNCH_CHECK();
+}
+
+void launch_qg_fit(int T, const QgEvalArgs& a, const QgFitArgs& fa, int n_workgroups, hipStream_t st) {
+    if (a.n_inst <= 0) return;
+    const std::size_t lds = qg_eval_lds_bytes(a);
+    auto go = [&](auto kernel) {
+        if (lds > 64 * 1024)
+            STP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        kernel<<<static_cast<unsigned>(n_workgroups), kThreads, lds, st>>>(a, fa);
+    };
+    switch (T) {
+        case 32: go(qg_fit_kernel<32>); break;
+        case 64: go(qg_fit_kernel<64>); break;
         default: throw ArgError("qpgraph: unsupported edge-count bucket " + std::to_string(T));
     }
     STP_LAUNCH_CHECK();

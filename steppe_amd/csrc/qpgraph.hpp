@@ -66,6 +66,16 @@ struct QgEvalArgs {
     int np_max = 0, ne_max = 0, npop_max = 0, npath_max = 0;
 };
 
+// Device L-BFGS fits of a batch of instances (graph + start point), one per workgroup.
+struct QgFitArgs {
+    const double* theta0;     // [n_inst][dmax] start points
+    double* theta_out;        // [n_inst][dmax] final mixture weights
+    double* score_out;        // [n_inst] final score (+inf: singular edge-length system)
+    std::int32_t* iters_out;  // [n_inst] L-BFGS iterations taken; -1 when the start failed
+    int maxit = 200;
+    double gtol = 1e-11, xtol = 1e-12, ftol = 1e-14;
+};
+
 struct QgBasisArgs {
     const QgTopoDev* topo;
     const std::int32_t* iarena;
@@ -89,6 +99,7 @@ struct QgBasisArgs {
 };
 
 void launch_qg_eval(int T, const QgEvalArgs& a, int n_workgroups, hipStream_t st);
+void launch_qg_fit(int T, const QgEvalArgs& a, const QgFitArgs& fa, int n_workgroups, hipStream_t st);
 std::size_t qg_eval_lds_bytes(const QgEvalArgs& a);
 void launch_qg_basis(const QgBasisArgs& a, int n_workgroups, hipStream_t st);
 std::size_t qg_basis_lds_bytes(const QgBasisArgs& a);

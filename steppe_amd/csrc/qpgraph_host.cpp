@@ -11,6 +11,8 @@
 // (edge, leaf) cell crossed by fewer than all of the leaf's paths is variable and is the
 // sum of its paths' weights; f3 basis pairs are the centred columns a <= b.
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <functional>
 #include <map>
 #include <memory>
@@ -176,14 +178,11 @@ int ne_bucket(int ne) { return ne <= 32 ? 32 : 64; }
 
 }  // namespace
 
-QgSet* HipBackend::qpgraph_set_create(const std::vector<const QgTopology*>& topos, const F2Blocks* const* f2,
-                                      double diag_f3) {
-    activate();
+void HipBackend::qpgraph_upload_topologies(QgSet& s, const std::vector<const QgTopology*>& topos) {
     const int n = static_cast<int>(topos.size());
     STP_REQUIRE(n > 0, "qpgraph: no graphs");
-    auto set = std::make_unique<QgSet>();
+    QgSet* set = &s;
     set->n = n;
-    // ---- topology arena ----
     std::vector<QgTopoDev> hdr(n);
     std::vector<std::int32_t> ia;
     std::vector<double> da;
@@ -233,6 +232,15 @@ QgSet* HipBackend::qpgraph_set_create(const std::vector<const QgTopology*>& topo
     if (!da.empty())
         STP_HIP(hipMemcpyAsync(set->d_darena.get(), da.data(), da.size() * sizeof(double), hipMemcpyHostToDevice,
                                stream_));
+    sync();  // the host staging vectors end with this function
+}
+
+QgSet* HipBackend::qpgraph_set_create(const std::vector<const QgTopology*>& topos, const F2Blocks* const* f2,
+                                      double diag_f3) {
+    activate();
+    auto set = std::make_unique<QgSet>();
+    qpgraph_upload_topologies(*set, topos);
+    const int n = set->n;
     // ---- f3 basis and covariance per graph, on the device ----
     const int NP = set->np_max;
     std::vector<std::int32_t> job_topo(n), job_P(n), job_nb(n), l2f(static_cast<std::size_t>(n) * set->npop_max, 0);
@@ -313,6 +321,35 @@ QgSet* HipBackend::qpgraph_set_create(const std::vector<const QgTopology*>& topo
     sync();
     set->host_status = st;
     // instance offsets of each graph's data
+    return set.release();
+}
+
+QgSet* HipBackend::qpgraph_set_from_basis(const std::vector<const QgTopology*>& topos, const double* f_obs,
+                                           const double* qinv) {
+    activate();
+    auto set = std::make_unique<QgSet>();
+    qpgraph_upload_topologies(*set, topos);
+    const int n = set->n, NP = set->np_max;
+    std::vector<double> fo(static_cast<std::size_t>(n) * NP, 0.0), qi(static_cast<std::size_t>(n) * NP * NP, 0.0);
+    std::size_t of = 0, oq = 0;
+    for (int g = 0; g < n; ++g) {
+        const std::size_t m = static_cast<std::size_t>(topos[g]->npair);
+        std::copy(f_obs + of, f_obs + of + m, fo.begin() + static_cast<std::ptrdiff_t>(g) * NP);
+        std::copy(qinv + oq, qinv + oq + m * m, qi.begin() + static_cast<std::ptrdiff_t>(g) * NP * NP);
+        of += m;
+        oq += m * m;
+    }
+    set->fobs.alloc(fo.size());
+    set->qdiag.alloc(fo.size());
+    set->qinv.alloc(qi.size());
+    set->status.alloc(n);
+    STP_HIP(hipMemcpyAsync(set->fobs.get(), fo.data(), fo.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    STP_HIP(hipMemcpyAsync(set->qinv.get(), qi.data(), qi.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    std::vector<double> zeros(fo.size(), 0.0);
+    STP_HIP(hipMemcpyAsync(set->qdiag.get(), zeros.data(), zeros.size() * sizeof(double), hipMemcpyHostToDevice,
+                           stream_));
+    sync();
+    set->host_status.assign(n, 0);
     return set.release();
 }
 
@@ -400,6 +437,172 @@ void HipBackend::qpgraph_set_eval(QgSet& s, int n_inst, const std::int32_t* inst
         STP_HIP(hipMemcpyAsync(fit_out, d_fit, static_cast<std::size_t>(n_inst) * NP * sizeof(double),
                                hipMemcpyDeviceToHost, stream_));
     sync();
+}
+
+namespace {
+
+// The reference's multistart point of restart inst, coordinate dim: a splitmix64
+// finaliser of a per-(inst, dim) counter, top 52 bits as a fraction
+// (src/core/qpadm/qpgraph_opt_constants.hpp:18-26, cpu_backend.cpp:2966-2973).
+double multistart_theta(unsigned inst, int dim) {
+    std::uint64_t z = static_cast<std::uint64_t>(inst) * 0x100000001B3ull +
+                      static_cast<std::uint64_t>(dim) * 0x9E3779B97F4A7C15ull + 0xD1B54A32D192ED03ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    const double x = static_cast<double>(z & 0xFFFFFFFFFFFFFull) / 4503599627370496.0;
+    return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x);
+}
+
+}  // namespace
+
+void HipBackend::qpgraph_set_fit(QgSet& s, int n_inst, const std::int32_t* inst_graph, const double* theta0,
+                                 const QgFitOptions& o, double* theta_out, double* score_out,
+                                 std::int32_t* iters_out) {
+    activate();
+    if (n_inst <= 0) return;
+    STP_REQUIRE(o.maxit >= 0, "qpgraph: maxit must be >= 0");
+    const int NP = s.np_max, D = std::max(s.dmax, 1), NE = s.ne_max;
+    std::vector<std::int64_t> offs(2 * static_cast<std::size_t>(n_inst));
+    int T = 32;
+    for (int i = 0; i < n_inst; ++i) {
+        const int g = inst_graph[i];
+        STP_REQUIRE(g >= 0 && g < s.n, "qpgraph: instance graph index out of range");
+        STP_REQUIRE(s.host_status[g] == 0, "qpgraph: graph " + std::to_string(g) + " has no valid covariance");
+        offs[i] = static_cast<std::int64_t>(g) * NP;
+        offs[n_inst + i] = static_cast<std::int64_t>(g) * NP * NP;
+        T = std::max(T, ne_bucket(s.topo[g].ne));
+    }
+    s.i32.ensure(2 * static_cast<std::size_t>(n_inst));
+    s.i64.ensure(2 * static_cast<std::size_t>(n_inst));
+    s.in.ensure(static_cast<std::size_t>(n_inst) * D);
+    s.out.ensure(static_cast<std::size_t>(n_inst) * (D + 1));
+    STP_HIP(hipMemcpyAsync(s.i32.get(), inst_graph, n_inst * sizeof(std::int32_t), hipMemcpyHostToDevice, stream_));
+    STP_HIP(hipMemcpyAsync(s.i64.get(), offs.data(), offs.size() * sizeof(std::int64_t), hipMemcpyHostToDevice, stream_));
+    STP_HIP(hipMemcpyAsync(s.in.get(), theta0, static_cast<std::size_t>(n_inst) * D * sizeof(double),
+                           hipMemcpyHostToDevice, stream_));
+    QgEvalArgs a{};
+    a.topo = s.d_topo.get();
+    a.iarena = s.d_iarena.get();
+    a.darena = s.d_darena.get();
+    a.n_inst = n_inst;
+    a.inst_topo = s.i32.get();
+    a.inst_fobs = s.i64.get();
+    a.inst_qinv = s.i64.get() + n_inst;
+    a.fobs = s.fobs.get();
+    a.qinv = s.qinv.get();
+    a.dmax = D;
+    a.emax = NE;
+    a.mode = 1;
+    a.fudge = o.fudge;
+    a.constrained = o.constrained;
+    a.counter = counter_.get() + 50;
+    a.np_max = NP;
+    a.ne_max = NE;
+    a.npop_max = s.npop_max;
+    a.npath_max = std::max(s.npath_max, 1);
+    STP_REQUIRE(qg_eval_lds_bytes(a) <= 150 * 1024, "qpgraph: graphs too large for the device score");
+    QgFitArgs fa{};
+    fa.theta0 = s.in.get();
+    fa.theta_out = s.out.get();
+    fa.score_out = s.out.get() + static_cast<std::size_t>(n_inst) * D;
+    fa.iters_out = s.i32.get() + n_inst;
+    fa.maxit = o.maxit;
+    fa.gtol = o.gtol;
+    fa.xtol = o.xtol;
+    fa.ftol = o.ftol;
+    launch_zero_u32(a.counter, 1, stream_);
+    mark_begin(kQpgraph);
+    launch_qg_fit(T, a, fa, std::min(n_inst, 2 * n_cu_), stream_);
+    mark_end(kQpgraph);
+    if (theta_out)
+        STP_HIP(hipMemcpyAsync(theta_out, fa.theta_out, static_cast<std::size_t>(n_inst) * D * sizeof(double),
+                               hipMemcpyDeviceToHost, stream_));
+    STP_HIP(hipMemcpyAsync(score_out, fa.score_out, n_inst * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (iters_out)
+        STP_HIP(hipMemcpyAsync(iters_out, fa.iters_out, n_inst * sizeof(std::int32_t), hipMemcpyDeviceToHost,
+                               stream_));
+    sync();
+}
+
+void HipBackend::qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOptions& o, QgFleetOut& out) {
+    activate();
+    STP_REQUIRE(numstart >= 1, "qpgraph: numstart must be >= 1");
+    const int n = s.n, D = std::max(s.dmax, 1), NE = s.ne_max, NP = s.np_max;
+    out.score.assign(n, std::numeric_limits<double>::infinity());
+    out.restart_spread.assign(n, 0.0);
+    out.status.assign(n, 3);
+    out.iterations.assign(n, 0);
+    out.theta.assign(static_cast<std::size_t>(n) * D, 0.0);
+    out.theta_lo.assign(static_cast<std::size_t>(n) * D, 0.0);
+    out.theta_hi.assign(static_cast<std::size_t>(n) * D, 0.0);
+    out.edge_length.assign(static_cast<std::size_t>(n) * NE, 0.0);
+    out.f3_fit.assign(static_cast<std::size_t>(n) * NP, 0.0);
+    // restarts of every fittable graph, the reference's start points
+    std::vector<std::int32_t> ig, first(n + 1, 0);
+    std::vector<double> x0;
+    for (int g = 0; g < n; ++g) {
+        first[g] = static_cast<std::int32_t>(ig.size());
+        if (s.host_status[g] != 0) continue;
+        const int d = s.topo[g].nadmix;
+        const int rs = d > 0 ? numstart : 1;
+        for (int r = 0; r < rs; ++r) {
+            ig.push_back(g);
+            for (int j = 0; j < D; ++j) x0.push_back(j < d ? multistart_theta(static_cast<unsigned>(r), j) : 0.0);
+        }
+    }
+    first[n] = static_cast<std::int32_t>(ig.size());
+    const int m = static_cast<int>(ig.size());
+    if (m == 0) return;
+    std::vector<double> th(static_cast<std::size_t>(m) * D), sc(m);
+    std::vector<std::int32_t> iters(m);
+    qpgraph_set_fit(s, m, ig.data(), x0.data(), o, th.data(), sc.data(), iters.data());
+    // best restart per graph (first of equal scores), spread and per-coordinate range
+    std::vector<std::int32_t> best_g;
+    std::vector<double> best_th;
+    for (int g = 0; g < n; ++g) {
+        if (first[g] == first[g + 1]) continue;
+        const int d = s.topo[g].nadmix;
+        int b = -1;
+        double smin = std::numeric_limits<double>::infinity(), smax = -std::numeric_limits<double>::infinity();
+        int it = 0;
+        for (int i = first[g]; i < first[g + 1]; ++i) {
+            smin = std::min(smin, sc[i]);
+            smax = std::max(smax, sc[i]);
+            it = std::max(it, iters[i]);
+            if (b < 0 || sc[i] < sc[b]) b = i;
+        }
+        out.iterations[g] = it;
+        out.restart_spread[g] = (std::isfinite(smin) && std::isfinite(smax)) ? smax - smin : 0.0;
+        for (int j = 0; j < d; ++j) {
+            double lo = 1.0, hi = 0.0;
+            for (int i = first[g]; i < first[g + 1]; ++i) {
+                lo = std::min(lo, th[static_cast<std::size_t>(i) * D + j]);
+                hi = std::max(hi, th[static_cast<std::size_t>(i) * D + j]);
+            }
+            out.theta[static_cast<std::size_t>(g) * D + j] = th[static_cast<std::size_t>(b) * D + j];
+            out.theta_lo[static_cast<std::size_t>(g) * D + j] = lo;
+            out.theta_hi[static_cast<std::size_t>(g) * D + j] = hi;
+        }
+        out.score[g] = sc[b];
+        best_g.push_back(g);
+        best_th.insert(best_th.end(), th.begin() + static_cast<std::ptrdiff_t>(b) * D,
+                       th.begin() + static_cast<std::ptrdiff_t>(b + 1) * D);
+    }
+    // edge lengths and fitted f3 at each graph's best weights
+    const int nb = static_cast<int>(best_g.size());
+    std::vector<double> fsc(nb), bl(static_cast<std::size_t>(nb) * NE), fit(static_cast<std::size_t>(nb) * NP);
+    qpgraph_set_eval(s, nb, best_g.data(), best_th.data(), nullptr, 0, o.fudge, o.constrained, fsc.data(), nullptr,
+                     nullptr, bl.data(), fit.data());
+    for (int k = 0; k < nb; ++k) {
+        const int g = best_g[k];
+        if (!std::isfinite(fsc[k])) continue;
+        out.status[g] = 0;
+        std::copy(bl.begin() + static_cast<std::ptrdiff_t>(k) * NE, bl.begin() + static_cast<std::ptrdiff_t>(k + 1) * NE,
+                  out.edge_length.begin() + static_cast<std::ptrdiff_t>(g) * NE);
+        std::copy(fit.begin() + static_cast<std::ptrdiff_t>(k) * NP, fit.begin() + static_cast<std::ptrdiff_t>(k + 1) * NP,
+                  out.f3_fit.begin() + static_cast<std::ptrdiff_t>(g) * NP);
+    }
 }
 
 void HipBackend::qpgraph_set_basis(const QgSet& s, int g, double* f_obs, double* qdiag, double* qinv) {

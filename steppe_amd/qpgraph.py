@@ -8,11 +8,11 @@
   edge-length NNLS, or d score / d (theta, edge lengths) in the joint form).
 * ``QpGraphScore`` / ``QpGraphJointScore``: ``torch.autograd.Function`` wrappers of the
   device score whose backward is the device adjoint.
-* ``lbfgs_box``: a batched projected L-BFGS over [0, 1]^D driven by one device call per
-  evaluation round for all instances.
+* ``GraphSet.fit`` / ``GraphSet.fit_fleet`` / ``fit_sets``: projected L-BFGS on the
+  adjoint gradient, run to convergence on the device (one workgroup per restart), from
+  the reference's splitmix multistart points.
 * ``run_qpgraph``: the reference's single-graph fit (bindings/bind_qpgraph.cpp:14-36) with
-  the finite-difference coordinate search replaced by L-BFGS on the adjoint gradient,
-  started from the reference's splitmix multistart points.
+  the finite-difference coordinate search replaced by that device L-BFGS.
 """
 from __future__ import annotations
 
@@ -23,6 +23,12 @@ import numpy as np
 from . import _core
 
 _SIG_DONE = False
+
+
+class FitOptions(C.Structure):
+    """stp_qpgraph_fit_options (include/steppe_hip.h)."""
+    _fields_ = [("numstart", C.c_int32), ("maxit", C.c_int32), ("tol", C.c_double), ("fudge", C.c_double),
+                ("constrained", C.c_int32)]
 
 
 def _lib():
@@ -41,6 +47,9 @@ def _lib():
             "stp_qpgraph_set_strides": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
             "stp_qpgraph_set_basis": (C.c_int, [vp, vp, i32, vp, vp, vp]),
             "stp_qpgraph_eval": (C.c_int, [vp, vp, i32, vp, vp, vp, i32, dp, i32, vp, vp, vp, vp, vp]),
+            "stp_qpgraph_set_create_basis": (C.c_int, [vp, i32, vp, vp, vp, C.POINTER(vp)]),
+            "stp_qpgraph_fit": (C.c_int, [vp, vp, i32, vp, vp, C.POINTER(FitOptions), vp, vp, vp]),
+            "stp_qpgraph_fit_fleet": (C.c_int, [vp, vp, C.POINTER(FitOptions)] + [vp] * 9),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -108,20 +117,44 @@ class Topology:
 class GraphSet:
     """Graphs resident on one device, graph g scored against f2 handle f2s[g]."""
 
-    def __init__(self, topologies, f2s, diag_f3=1e-5):
-        if len(topologies) != len(f2s) or not topologies:
-            raise ValueError("qpgraph: one f2 handle per topology is required")
+    def __init__(self, topologies, f2s=None, diag_f3=1e-5, basis=None, device=0):
+        """Graph g against f2 handle f2s[g] (basis and covariance computed on the device),
+        or, with ``basis`` = [(f_obs, Qinv), ...], against a given f3 basis and inverse
+        covariance per graph (the reference fleet fit's inputs)."""
         self.topologies = list(topologies)
-        self.f2s = list(f2s)
-        dev = f2s[0].device
-        if any(h.device != dev for h in f2s):
-            raise ValueError("qpgraph: every f2 handle must be on the same device")
-        self.ctx = _core.Context.get(dev)
-        n = len(topologies)
-        tps = (C.c_void_p * n)(*[t.raw for t in topologies])
-        hs = (C.c_void_p * n)(*[h.raw for h in f2s])
+        if not self.topologies:
+            raise ValueError("qpgraph: no graphs")
+        n = len(self.topologies)
+        tps = (C.c_void_p * n)(*[t.raw for t in self.topologies])
         s = C.c_void_p()
-        _core.check(_lib().stp_qpgraph_set_create(self.ctx.h, n, tps, hs, float(diag_f3), C.byref(s)), "qpgraph set")
+        if basis is None:
+            if f2s is None or len(f2s) != n:
+                raise ValueError("qpgraph: one f2 handle per topology is required")
+            self.f2s = list(f2s)
+            dev = f2s[0].device
+            if any(h.device != dev for h in f2s):
+                raise ValueError("qpgraph: every f2 handle must be on the same device")
+            self.ctx = _core.Context.get(dev)
+            hs = (C.c_void_p * n)(*[h.raw for h in f2s])
+            _core.check(_lib().stp_qpgraph_set_create(self.ctx.h, n, tps, hs, float(diag_f3), C.byref(s)),
+                        "qpgraph set")
+        else:
+            if len(basis) != n:
+                raise ValueError("qpgraph: one (f_obs, Qinv) per topology is required")
+            fo, qi = [], []
+            for t, (f, q) in zip(self.topologies, basis):
+                f = np.asarray(f, np.float64).reshape(-1)
+                q = np.asarray(q, np.float64)
+                if f.size != t.npair or q.shape != (t.npair, t.npair):
+                    raise ValueError("qpgraph: basis shapes do not match the graph's npair")
+                fo.append(f)
+                qi.append(q.reshape(-1))
+            fo = np.ascontiguousarray(np.concatenate(fo))
+            qi = np.ascontiguousarray(np.concatenate(qi))
+            self.f2s = []
+            self.ctx = _core.Context.get(device)
+            _core.check(_lib().stp_qpgraph_set_create_basis(self.ctx.h, n, tps, _p(fo), _p(qi), C.byref(s)),
+                        "qpgraph set")
         self._h = s
         self.n = n
         dm, em, pm = C.c_int32(), C.c_int32(), C.c_int32()
@@ -179,6 +212,42 @@ class GraphSet:
         return out
 
 
+    @staticmethod
+    def _options(numstart=10, maxit=200, tol=1e-9, fudge=1e-4, constrained=True):
+        return FitOptions(int(numstart), int(maxit), float(tol), float(fudge), 1 if constrained else 0)
+
+    def fit(self, inst_graph, theta0, maxit=200, tol=1e-9, fudge=1e-4, constrained=True):
+        """Device L-BFGS of each instance from its start point.  Returns (theta (n, dmax),
+        score (n,), iterations (n,); -1 where the start's edge-length system is singular)."""
+        ig = np.ascontiguousarray(inst_graph, np.int32)
+        n = len(ig)
+        th = np.zeros((n, self.dmax))
+        t = np.asarray(theta0, np.float64).reshape(n, -1)
+        th[:, :t.shape[1]] = t
+        th = np.ascontiguousarray(th)
+        out = np.zeros((n, self.dmax))
+        score = np.zeros(n)
+        iters = np.zeros(n, np.int32)
+        opt = self._options(1, maxit, tol, fudge, constrained)
+        _core.check(_lib().stp_qpgraph_fit(self.ctx.h, self._h, n, _p(ig), _p(th), C.byref(opt), _p(out), _p(score),
+                                           _p(iters)), "qpgraph fit")
+        return out, score, iters
+
+    def fit_fleet(self, numstart=10, maxit=200, tol=1e-9, fudge=1e-4, constrained=True):
+        """Multistart fit of every graph on the device (stp_qpgraph_fit_fleet)."""
+        n = self.n
+        r = {"score": np.zeros(n), "restart_spread": np.zeros(n), "theta": np.zeros((n, self.dmax)),
+             "theta_lo": np.zeros((n, self.dmax)), "theta_hi": np.zeros((n, self.dmax)),
+             "edge_length": np.zeros((n, self.emax)), "f3_fit": np.zeros((n, self.pmax)),
+             "iterations": np.zeros(n, np.int32), "status": np.zeros(n, np.int32)}
+        opt = self._options(numstart, maxit, tol, fudge, constrained)
+        _core.check(_lib().stp_qpgraph_fit_fleet(
+            self.ctx.h, self._h, C.byref(opt), *[_p(r[k]) for k in ("score", "restart_spread", "theta", "theta_lo",
+                                                                     "theta_hi", "edge_length", "f3_fit",
+                                                                     "iterations", "status")]), "qpgraph fleet")
+        return r
+
+
 # ---------------------------------------------------------------------------------------
 # torch.autograd wrappers
 # ---------------------------------------------------------------------------------------
@@ -232,163 +301,26 @@ def joint_score_function():
 
 
 # ---------------------------------------------------------------------------------------
-# batched projected L-BFGS on [0, 1]^D
+# multistart fits and the single-graph fit
 # ---------------------------------------------------------------------------------------
-def lbfgs_box(fun, x0, lo=0.0, hi=1.0, m=8, max_iter=200, gtol=1e-9, xtol=1e-12, ftol=1e-14,
-              max_backtrack=30, c1=1e-4):
-    """Minimises n independent box-constrained problems at once.  ``fun(x, idx)`` returns
-    (f, g) for the rows idx of x (one device call per round).  Per instance: the search
-    direction is the two-loop L-BFGS direction on the free variables (a variable at a
-    bound whose gradient points outward is held), the step is projected onto the box and
-    accepted by Armijo backtracking.  Stops an instance when its projected gradient is
-    below gtol, its step below xtol, or its relative decrease below ftol.  Returns
-    (x, f, iterations)."""
-    x = np.clip(np.array(x0, dtype=np.float64), lo, hi)
-    n, D = x.shape
-    f, g = fun(x, np.arange(n))
-    S = np.zeros((n, m, D))
-    Y = np.zeros((n, m, D))
-    rho = np.zeros((n, m))
-    hist = np.zeros(n, np.int64)
-    active = np.ones(n, bool)
-    iters = np.zeros(n, np.int64)
-    for _ in range(max_iter):
-        idx = np.nonzero(active)[0]
-        if idx.size == 0:
-            break
-        xi, gi = x[idx], g[idx]
-        held = ((xi <= lo) & (gi > 0)) | ((xi >= hi) & (gi < 0))
-        pg = np.where(held, 0.0, gi)
-        done = np.abs(pg).max(axis=1) <= gtol
-        # two-loop recursion on the free variables
-        q = pg.copy()
-        k = hist[idx]
-        alpha = np.zeros((idx.size, m))
-        for j in range(m - 1, -1, -1):           # newest (j = m-1) to oldest
-            use = (m - 1 - j) < np.minimum(k, m)  # the entry's age is m-1-j
-            slot = (k - 1 - (m - 1 - j)) % m
-            s = S[idx, slot] * ~held
-            y = Y[idx, slot] * ~held
-            r = rho[idx, slot]
-            a = np.where(use, r * np.einsum("ij,ij->i", s, q), 0.0)
-            alpha[:, j] = a
-            q = q - a[:, None] * y
-        last = (k - 1) % m
-        sy = np.einsum("ij,ij->i", S[idx, last] * ~held, Y[idx, last] * ~held)
-        yy = np.einsum("ij,ij->i", Y[idx, last] * ~held, Y[idx, last] * ~held)
-        gamma = np.where((k > 0) & (yy > 0), sy / np.where(yy > 0, yy, 1.0), 1.0)
-        gamma = np.where(gamma > 0, gamma, 1.0)
-        r_ = gamma[:, None] * q
-        for j in range(m):                       # oldest to newest
-            use = (m - 1 - j) < np.minimum(k, m)
-            slot = (k - 1 - (m - 1 - j)) % m
-            s = S[idx, slot] * ~held
-            y = Y[idx, slot] * ~held
-            rr = rho[idx, slot]
-            b = np.where(use, rr * np.einsum("ij,ij->i", y, r_), 0.0)
-            r_ = r_ + (alpha[:, j] - b)[:, None] * s
-        d = -r_ * ~held
-        slope = np.einsum("ij,ij->i", d, pg)
-        bad = slope >= 0
-        d[bad] = -pg[bad]
-        hist[idx[bad]] = 0
-        # projected Armijo backtracking, all instances of the round together
-        step = np.ones(idx.size)
-        x_new = np.clip(xi + d, lo, hi)
-        todo = ~done
-        f_new = f[idx].copy()
-        g_new = gi.copy()
-        for _bt in range(max_backtrack):
-            cur = np.nonzero(todo)[0]
-            if cur.size == 0:
-                break
-            fc, gc = fun(x_new[cur], idx[cur])
-            dec = np.einsum("ij,ij->i", gi[cur], x_new[cur] - xi[cur])
-            ok = fc <= f[idx[cur]] + c1 * dec
-            f_new[cur[ok]] = fc[ok]
-            g_new[cur[ok]] = gc[ok]
-            todo[cur[ok]] = False
-            nok = cur[~ok]
-            step[nok] *= 0.5
-            x_new[nok] = np.clip(xi[nok] + step[nok, None] * d[nok], lo, hi)
-        failed = todo & ~done
-        # accept
-        acc = ~done & ~failed
-        sv = x_new - xi
-        yv = g_new - gi
-        syv = np.einsum("ij,ij->i", sv, yv)
-        upd = acc & (syv > 1e-16 * np.linalg.norm(sv, axis=1) * np.linalg.norm(yv, axis=1))
-        ii = idx[upd]
-        slot = hist[ii] % m
-        S[ii, slot] = sv[upd]
-        Y[ii, slot] = yv[upd]
-        rho[ii, slot] = 1.0 / syv[upd]
-        hist[ii] += 1
-        ia = idx[acc]
-        df = f[ia] - f_new[acc]
-        dx = np.abs(sv[acc]).max(axis=1)
-        x[ia] = x_new[acc]
-        f[ia] = f_new[acc]
-        g[ia] = g_new[acc]
-        iters[ia] += 1
-        stop = np.zeros(idx.size, bool)
-        stop[done | failed] = True
-        stop[np.nonzero(acc)[0][(dx <= xtol) | (df <= ftol * (1.0 + np.abs(f[ia])))]] = True
-        active[idx[stop]] = False
-    return x, f, iters
-
-
-# ---------------------------------------------------------------------------------------
-# the reference's multistart points and the single-graph fit
-# ---------------------------------------------------------------------------------------
-_M64 = (1 << 64) - 1
-
-
-def init_theta(inst: int, dim: int) -> float:
-    """The reference's splitmix start point for restart ``inst``, coordinate ``dim``
-    (src/core/qpadm/qpgraph_opt_constants.hpp:18-26, cpu_backend.cpp:2966-2973)."""
-    z = (inst * 0x100000001B3 + dim * 0x9E3779B97F4A7C15 + 0xD1B54A32D192ED03) & _M64
-    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
-    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
-    z = z ^ (z >> 31)
-    return (z & 0xFFFFFFFFFFFFF) / 4503599627370496.0
-
-
 def fit_sets(gset: GraphSet, numstart=10, fudge=1e-4, constrained=True, maxit=200, tol=1e-9):
     """Fits every graph of the set from ``numstart`` restarts each, all restarts of all
-    graphs in one L-BFGS batch.  Returns per graph: best score, restart spread, best
-    theta, per-coordinate min / max over restarts, and the L-BFGS iterations used."""
-    D = [t.nadmix for t in gset.topologies]
-    inst_graph, x0 = [], []
-    for g, d in enumerate(D):
-        if gset.status[g] != 0:
-            continue
-        for s in range(numstart if d > 0 else 1):
-            inst_graph.append(g)
-            x0.append([init_theta(s, j) for j in range(d)] + [0.0] * (gset.dmax - d))
-    inst_graph = np.asarray(inst_graph, np.int32)
-    x0 = np.asarray(x0, np.float64).reshape(len(inst_graph), gset.dmax)
-    mask = np.zeros_like(x0)
-    for i, g in enumerate(inst_graph):
-        mask[i, :D[g]] = 1.0
-
-    def fun(x, idx):
-        r = gset.eval(inst_graph[idx], x, mode=1, fudge=fudge, constrained=constrained)
-        return r["score"], r["grad_theta"] * mask[idx]
-
-    x, f, it = lbfgs_box(fun, x0, max_iter=maxit, gtol=tol * 1e-2)
+    graphs in one device launch.  Returns per graph (None when its covariance is
+    singular): best score, restart spread, best theta, per-coordinate min / max over
+    restarts, edge lengths and fitted f3 at the best theta, and the most L-BFGS
+    iterations any restart used."""
+    r = gset.fit_fleet(numstart=numstart, maxit=maxit, tol=tol, fudge=fudge, constrained=constrained)
     out = []
-    for g in range(gset.n):
-        sel = np.nonzero(inst_graph == g)[0]
-        if sel.size == 0:
+    for g, t in enumerate(gset.topologies):
+        if gset.status[g] != 0:
             out.append(None)
             continue
-        fs = f[sel]
-        b = sel[int(np.argmin(fs))]
-        d = D[g]
-        out.append({"score": float(f[b]), "restart_spread": float(fs.max() - fs.min()),
-                    "theta": x[b, :d].copy(), "theta_lo": x[sel, :d].min(axis=0), "theta_hi": x[sel, :d].max(axis=0),
-                    "iterations": int(it[sel].max())})
+        d = t.nadmix
+        out.append({"score": float(r["score"][g]), "restart_spread": float(r["restart_spread"][g]),
+                    "theta": r["theta"][g, :d].copy(), "theta_lo": r["theta_lo"][g, :d].copy(),
+                    "theta_hi": r["theta_hi"][g, :d].copy(), "edge_length": r["edge_length"][g, :t.nedge].copy(),
+                    "f3_fit": r["f3_fit"][g, :t.npair].copy(), "iterations": int(r["iterations"][g]),
+                    "status": int(r["status"][g])})
     return out
 
 
@@ -405,14 +337,12 @@ def run_qpgraph(f2, edges, numstart=10, fudge=1e-4, diag_f3=1e-5, constrained=Tr
         return dict(base, status="non_spd_covariance", score=0.0, restart_spread=0.0, weight=[], weight_lo=[],
                     weight_hi=[], edge_length=[], worst_residual_z=0.0, worst_pop2="", worst_pop3="")
     fit = fit_sets(gset, numstart=numstart, fudge=fudge, constrained=constrained, maxit=maxit, tol=tol)[0]
-    r = gset.eval([0], fit["theta"][None, :] if topo.nadmix else None, mode=0, fudge=fudge,
-                  constrained=constrained, want_fit=True)
-    score = float(r["score"][0])
-    if not np.isfinite(score):
+    score = fit["score"]
+    if fit["status"] != 0 or not np.isfinite(score):
         return dict(base, status="non_spd_covariance", score=score, restart_spread=0.0, weight=[], weight_lo=[],
                     weight_hi=[], edge_length=[], worst_residual_z=0.0, worst_pop2="", worst_pop3="")
     f_obs, qdiag, _ = gset.basis(0)
-    fitted = r["fit"][0, :topo.npair]
+    fitted = fit["f3_fit"]
     with np.errstate(invalid="ignore", divide="ignore"):
         z = (f_obs - fitted) / np.where(qdiag > 0, np.sqrt(qdiag), np.nan)
     worst, wk = 0.0, -1
@@ -422,4 +352,4 @@ def run_qpgraph(f2, edges, numstart=10, fudge=1e-4, diag_f3=1e-5, constrained=Tr
     p2, p3 = topo.centred_pair(wk) if wk >= 0 else ("", "")
     return dict(base, status="ok", score=score, restart_spread=fit["restart_spread"],
                 weight=fit["theta"].tolist(), weight_lo=fit["theta_lo"].tolist(), weight_hi=fit["theta_hi"].tolist(),
-                edge_length=r["bl"][0, :topo.nedge].tolist(), worst_residual_z=worst, worst_pop2=p2, worst_pop3=p3)
+                edge_length=fit["edge_length"].tolist(), worst_residual_z=worst, worst_pop2=p2, worst_pop3=p3)

@@ -203,3 +203,46 @@ def test_fits(cref, cases, gset):
     assert abs(r["weight"][0] - 0.153483829987482) <= 1e-5
     assert r["restart_spread"] < 1e-4
     assert r["admix_from"] == ["pSteppe"] and r["admix_to"] == ["aCW"]
+
+
+def test_device_fit_entry_points(cref, cases, gset):
+    """stp_qpgraph_fit from explicit start points: the returned score is the score of the
+    returned theta, and no restart ends above its start; a set built from a given basis
+    (the reference fleet's inputs) fits to the same optimum as the f2-built set; a tree
+    (no admixture) fits in one evaluation."""
+    from steppe_amd import _core, qpgraph, synth
+    rng = np.random.default_rng(31)
+    ig = np.repeat(np.arange(len(cases), dtype=np.int32), 4)
+    x0 = rng.uniform(0.0, 1.0, (len(ig), gset.dmax))
+    th, sc, it = gset.fit(ig, x0)
+    assert np.all(it >= 0)
+    again = gset.eval(ig, th, mode=0)["score"]
+    assert np.array_equal(again, sc)
+    start = gset.eval(ig, x0, mode=0)["score"]
+    assert np.all(sc <= start)
+    # the same graphs from the oracle's basis
+    basis = []
+    for f2, bs, edges, names in cases:
+        fo, _, Qi = cref.qpgraph_basis(f2, None, bs, edges, names)
+        basis.append((fo, Qi))
+    gb = qpgraph.GraphSet(gset.topologies, basis=basis)
+    a = gset.fit_fleet()
+    b = gb.fit_fleet()
+    assert np.all(a["status"] == 0) and np.all(b["status"] == 0)
+    rel = np.abs(a["score"] - b["score"]) / np.abs(b["score"])
+    print(f"fleet from f2 vs from the oracle's basis: max rel score diff {rel.max():.2e}")
+    assert rel.max() <= 1e-8
+    # a tree: no admixture weights, one restart
+    trng = np.random.default_rng(5)
+    edges, theta, drift = synth.random_admixture_graph(trng, 7, 0)
+    f2, bs, names = synth.simulate_graph_f2(edges, theta, drift, rng=trng)
+    h = _core.upload_f2(f2, None, bs, names)
+    tt = qpgraph.Topology(edges, names)
+    assert tt.nadmix == 0
+    gt = qpgraph.GraphSet([tt], [h])
+    r = gt.fit_fleet()
+    o = cref.qpgraph_fit(f2, None, bs, edges, names)
+    assert r["status"][0] == 0 and r["iterations"][0] == 0 and r["restart_spread"][0] == 0.0
+    assert abs(r["score"][0] - o["score"]) <= 1e-9 * abs(o["score"])
+    assert np.allclose(r["edge_length"][0, :tt.nedge], o["edge_length"], rtol=1e-7,
+                       atol=1e-12 * np.abs(o["edge_length"]).max())
