@@ -35,6 +35,7 @@ constexpr int kLanes = 64;
 constexpr int kTileSnps = kSnpPerLane * kLanes;  // 1024
 constexpr int kTileBytes = kTileSnps / 4;        // 256
 constexpr int kFlushEvery = 60;                  // 60 * 4 <= 255 per byte lane
+constexpr int kBatch = 8;                        // records loaded ahead per wavefront
 constexpr int kPloidyWindow = 1000;
 
 template <bool kAligned>
@@ -59,7 +60,8 @@ __global__ __launch_bounds__(256) void decode_counts_kernel(
     int n_pop, std::uint16_t* __restrict__ out_a2, std::uint16_t* __restrict__ out_n,
     std::int64_t ld) {
     const int lane = threadIdx.x & 63;
-    const int slot = blockIdx.y * (blockDim.x / 64) + (threadIdx.x >> 6);
+    // wavefront-uniform: population offsets and ploidies then come through scalar loads
+    const int slot = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.y * (blockDim.x / 64) + (threadIdx.x >> 6)));
     const int n_slots = gridDim.y * (blockDim.x / 64);
     const std::int64_t tile = blockIdx.x;
     const std::int64_t boff = tile * kTileBytes + 4 * lane;
@@ -71,30 +73,43 @@ __global__ __launch_bounds__(256) void decode_counts_kernel(
         std::uint32_t alo[4] = {0, 0, 0, 0}, ahi[4] = {0, 0, 0, 0};
         std::uint32_t nlo[4] = {0, 0, 0, 0}, nhi[4] = {0, 0, 0, 0};
         int pending = 0;
-        for (std::uint64_t g = g0; g < g1; ++g) {
-            const int pl = ploidy[g];
-            if (pl != 1 && pl != 2) continue;  // the reference skips any other ploidy
-            const std::uint32_t w = load_codes<kAligned>(packed + g * bpr, boff, bpr);
-            const int sh = (pl == 2) ? 1 : 0;  // a2 += 2*code, n += 2 for diploids
+        // Individuals in batches of kBatch: all of a batch's loads are issued before any
+        // is decoded, so each wavefront keeps kBatch records' words in flight.
+        for (std::uint64_t gb = g0; gb < g1; gb += kBatch) {
+            // lane u < kBatch fetches the ploidy of individual gb + u (one load for the batch)
+            const int myp = (lane < kBatch && gb + lane < g1) ? ploidy[gb + lane] : 0;
+            std::uint32_t wb[kBatch];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const std::uint32_t x = (w >> (2 * k)) & 0x03030303u;
-                const std::uint32_t miss = x & (x >> 1) & 0x01010101u;  // code == 3
-                const std::uint32_t ok = miss ^ 0x01010101u;
-                const std::uint32_t code = x & ~(miss * 3u);
-                a8[k] += code << sh;
-                n8[k] += ok << sh;
+            for (int u = 0; u < kBatch; ++u) {
+                const std::uint64_t g = gb + u;
+                wb[u] = g < g1 ? load_codes<kAligned>(packed + g * bpr, boff, bpr) : 0u;
             }
-            if (++pending == kFlushEvery) {
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int pl = __builtin_amdgcn_readlane(myp, u);
+                if (pl != 1 && pl != 2) continue;  // the reference skips any other ploidy
+                const std::uint32_t w = wb[u];
+                const int sh = (pl == 2) ? 1 : 0;  // a2 += 2*code, n += 2 for diploids
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    alo[k] += a8[k] & 0x00FF00FFu;
-                    ahi[k] += (a8[k] >> 8) & 0x00FF00FFu;
-                    nlo[k] += n8[k] & 0x00FF00FFu;
-                    nhi[k] += (n8[k] >> 8) & 0x00FF00FFu;
-                    a8[k] = n8[k] = 0;
+                    const std::uint32_t x = (w >> (2 * k)) & 0x03030303u;
+                    const std::uint32_t miss = x & (x >> 1) & 0x01010101u;  // code == 3
+                    const std::uint32_t ok = miss ^ 0x01010101u;
+                    const std::uint32_t code = x & ~(miss * 3u);
+                    a8[k] += code << sh;
+                    n8[k] += ok << sh;
                 }
-                pending = 0;
+                if (++pending == kFlushEvery) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        alo[k] += a8[k] & 0x00FF00FFu;
+                        ahi[k] += (a8[k] >> 8) & 0x00FF00FFu;
+                        nlo[k] += n8[k] & 0x00FF00FFu;
+                        nhi[k] += (n8[k] >> 8) & 0x00FF00FFu;
+                        a8[k] = n8[k] = 0;
+                    }
+                    pending = 0;
+                }
             }
         }
 #pragma unroll

@@ -106,15 +106,39 @@ __device__ __noinline__ bool solve_sub(const double* C, int ld, const int* idx, 
 // wavefront (lane i <-> unknown i, n <= T <= 64), as the reference's nnls_active_set
 // (qpgraph_objective.hpp:41-100): tolerance 1e-12, at most 3n+30 outer and inner steps.
 // The solution goes to bls (LDS), the final passive set to passive (LDS).
+// With warm set, the passive set held in passive_out (from an earlier solve of a nearby
+// problem) is tried first: when the system restricted to it has an all-positive
+// solution, that solution is the starting point of the outer loop (a feasible point
+// that is optimal on its passive set, as the algorithm requires); otherwise the solve
+// starts from zero.  The minimiser is unique, so the result is the same.
 template <int T>
 __device__ bool nnls_wave(int n, const double* C, const double* sc, const double* q1, double* bls, int* passive_out,
-                          int* slot, double* prow) {
+                          int* slot, double* prow, bool warm) {
     const int lane = lane_id();
     const bool act = lane < n;
     auto A = [&](int i, int j) { return C[i * n + j] / (sc[i] * sc[j]); };
     if (act) bls[lane] = 0.0;
     bool passive = false;
     __builtin_amdgcn_wave_barrier();
+    if (warm) {
+        passive = act && passive_out[lane] != 0;
+        const unsigned long long mask = __ballot(passive);
+        const int p = __builtin_popcountll(mask);
+        if (p > 0) {
+            if (passive) slot[__builtin_popcountll(mask & ((1ull << lane) - 1ull))] = lane;
+            __builtin_amdgcn_wave_barrier();
+            const int pa = lane < p ? slot[lane] : 0;
+            double z = 0.0;
+            const bool ok = solve_sub<T>(C, n, slot, p, false, sc, lane < p ? q1[pa] : 0.0, z, prow);
+            const bool all_pos = ok && popc(lane < p && z <= kNnlsEps) == 0;
+            if (all_pos) {
+                if (lane < p) bls[pa] = z;
+            } else {
+                passive = false;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
     const int max_iter = 3 * n + 30;
     for (int outer = 0; outer < max_iter; ++outer) {
         double w = 0.0;
@@ -189,7 +213,7 @@ struct EvalShared {
 template <int T>
 __device__ bool qg_eval_one(const QgEvalArgs& g, const QgTopoDev& tp, const double* __restrict__ G,
                             const double* __restrict__ f, int mode, const EvalLds& L, const EvalShared& sh,
-                            double* fit) {
+                            double* fit, bool warm = false) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -230,12 +254,14 @@ __device__ bool qg_eval_one(const QgEvalArgs& g, const QgTopoDev& tp, const doub
             const int r = x / ne, e = x - (x / ne) * ne;
             const double* Gr = G + static_cast<std::int64_t>(r) * np;
             double acc = 0.0;
+#pragma unroll 8
             for (int c = 0; c < np; ++c) acc += Gr[c] * L.P[c * ne + e];
             L.W[x] = acc;
         }
         for (int r = tid; r < np; r += kThreads) {
             const double* Gr = G + static_cast<std::int64_t>(r) * np;
             double acc = 0.0;
+#pragma unroll 8
             for (int c = 0; c < np; ++c) acc += Gr[c] * f[c];
             L.qf[r] = acc;
         }
@@ -270,7 +296,7 @@ __device__ bool qg_eval_one(const QgEvalArgs& g, const QgTopoDev& tp, const doub
         if (wave == 0) {
             bool ok;
             if (g.constrained) {
-                ok = nnls_wave<T>(ne, L.C, L.sc, L.q1, L.lam, sh.passive, sh.slot, L.prow);
+                ok = nnls_wave<T>(ne, L.C, L.sc, L.q1, L.lam, sh.passive, sh.slot, L.prow, warm);
             } else {
                 double z = 0.0;
                 ok = solve_sub<T>(L.C, ne, nullptr, ne, false, L.sc, lane < ne ? L.q1[lane] : 0.0, z, L.prow);
@@ -302,9 +328,11 @@ __device__ bool qg_eval_one(const QgEvalArgs& g, const QgTopoDev& tp, const doub
     for (int a = tid; a < np; a += kThreads) {
         const double* Ga = G + static_cast<std::int64_t>(a) * np;
         double row = 0.0, col = 0.0;
+#pragma unroll 8
         for (int b = 0; b < np; ++b) row += Ga[b] * L.res[b];
         L.Gr[a] = row;
         if (mode != 0) {
+#pragma unroll 8
             for (int b = 0; b < np; ++b) col += G[static_cast<std::int64_t>(b) * np + a] * L.res[b];
             L.GTr[a] = col;
         }
@@ -365,6 +393,7 @@ __device__ bool qg_eval_one(const QgEvalArgs& g, const QgTopoDev& tp, const doub
         __syncthreads();
         for (int k = tid; k < np; k += kThreads) {
             double s = 0.0;
+#pragma unroll 8
             for (int c = 0; c < np; ++c) s += G[static_cast<std::int64_t>(c) * np + k] * L.v[c];
             L.qf[k] = s;  // G^T P lambda
         }
@@ -372,6 +401,7 @@ __device__ bool qg_eval_one(const QgEvalArgs& g, const QgTopoDev& tp, const doub
         for (int x = tid; x < np * ne; x += kThreads) {
             const int k = x / ne, e = x - (x / ne) * ne;
             double w2 = 0.0;
+#pragma unroll 8
             for (int c = 0; c < np; ++c) w2 += G[static_cast<std::int64_t>(c) * np + k] * L.P[c * ne + e];
             L.W[x] = -L.bl[e] * (L.u[k] + L.qf[k]) + L.lam[e] * L.Gr[k] + kappa * (L.W[x] + w2);
         }
@@ -584,7 +614,7 @@ __global__ __launch_bounds__(kThreads) void qg_fit_kernel(QgEvalArgs g, QgFitArg
                         s_theta[tid] = xn < 0.0 ? 0.0 : (xn > 1.0 ? 1.0 : xn);
                     }
                     __syncthreads();
-                    const bool ok = qg_eval_one<T>(g, tp, G, f, 1, L, sh, nullptr);
+                    const bool ok = qg_eval_one<T>(g, tp, G, f, 1, L, sh, nullptr, true);
                     if (tid == 0) {
                         double dec = 0.0;
                         for (int d = 0; d < D; ++d) dec += s_gx[d] * (s_theta[d] - s_x[d]);
