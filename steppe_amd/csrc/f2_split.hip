@@ -76,30 +76,46 @@ constexpr int kWaves = 8;
 constexpr int kStageLd = 65;  // LDS row stride (doubles) of the staged unit
 
 // Staging reads of the two frequency sources.  Each thread stages 8 elements of the
-// 32 x 64 unit; (q - 1/2, h) goes to sq/sh, with h = -1 marking an invalid entry.
+// 32 x 64 unit: fetch() issues the global loads into registers (so a workgroup can
+// fetch its next unit while it converts the current one), put() turns them into
+// (q - 1/2, h) in sq/sh, with h = -1 marking an invalid entry.
 // h = q(1-q)/max(n-1, 1) as in the reference's het_correction
 // (src/core/internal/f2_estimator.hpp:22-29).
 struct CountsSrc {
     const std::uint16_t* a2;
     const std::uint16_t* n;
     std::int64_t ld;
+    struct Raw {
+        unsigned a[8], c[8];
+    };
     // Lanes run along the SNP axis: each wavefront reads 128 contiguous bytes per row.
-    __device__ void stage(int r0, int P, std::int64_t snp0, int live, int tid, double* sq, double* sh) const {
+    __device__ Raw fetch(int r0, int P, std::int64_t snp0, int live, int tid) const {
+        Raw raw;
+        const int kk = tid & 63;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row = (tid >> 6) + 4 * i;
+            raw.a[i] = raw.c[i] = 0u;
+            if (r0 + row < P && kk < live) {
+                const std::int64_t o = static_cast<std::int64_t>(r0 + row) * ld + snp0 + kk;
+                raw.a[i] = a2[o];
+                raw.c[i] = n[o];
+            }
+        }
+        return raw;
+    }
+    __device__ void put(const Raw& raw, int tid, double* sq, double* sh) const {
         const int kk = tid & 63;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int row = (tid >> 6) + 4 * i;
             double qp = 0.0, h = -1.0;
-            if (r0 + row < P && kk < live) {
-                const std::int64_t o = static_cast<std::int64_t>(r0 + row) * ld + snp0 + kk;
-                const unsigned c = n[o];
-                if (c) {
-                    const double nd = static_cast<double>(c);
-                    const double q = static_cast<double>(a2[o]) / (2.0 * nd);
-                    const double nm1 = nd - 1.0;
-                    h = q * (1.0 - q) / (nm1 > 1.0 ? nm1 : 1.0);
-                    qp = q - 0.5;
-                }
+            if (raw.c[i]) {
+                const double nd = static_cast<double>(raw.c[i]);
+                const double q = static_cast<double>(raw.a[i]) / (2.0 * nd);
+                const double nm1 = nd - 1.0;
+                h = q * (1.0 - q) / (nm1 > 1.0 ? nm1 : 1.0);
+                qp = q - 0.5;
             }
             sq[row * kStageLd + kk] = qp;
             sh[row * kStageLd + kk] = h;
@@ -111,27 +127,62 @@ struct QvnSrc {
     const double* q;
     const double* v;
     const double* n;
+    struct Raw {
+        double q[8], nm1[8];
+        bool ok[8];
+    };
     // Q/V/N are population-fastest (i + P*s): lanes run along the population axis.
-    __device__ void stage(int r0, int P, std::int64_t snp0, int live, int tid, double* sq, double* sh) const {
+    __device__ Raw fetch(int r0, int P, std::int64_t snp0, int live, int tid) const {
+        Raw raw;
+        const int row = tid & 31;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int kk = (tid >> 5) + 8 * i;
+            raw.ok[i] = false;
+            raw.q[i] = raw.nm1[i] = 0.0;
+            if (r0 + row < P && kk < live) {
+                const std::int64_t o = r0 + row + static_cast<std::int64_t>(P) * (snp0 + kk);
+                raw.ok[i] = v[o] != 0.0;
+                raw.q[i] = q[o];
+                raw.nm1[i] = n[o] - 1.0;
+            }
+        }
+        return raw;
+    }
+    __device__ void put(const Raw& raw, int tid, double* sq, double* sh) const {
         const int row = tid & 31;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int kk = (tid >> 5) + 8 * i;
             double qp = 0.0, h = -1.0;
-            if (r0 + row < P && kk < live) {
-                const std::int64_t o = r0 + row + static_cast<std::int64_t>(P) * (snp0 + kk);
-                if (v[o] != 0.0) {
-                    const double qq = q[o];
-                    const double nm1 = n[o] - 1.0;
-                    h = qq * (1.0 - qq) / (nm1 > 1.0 ? nm1 : 1.0);
-                    qp = qq - 0.5;
-                }
+            if (raw.ok[i]) {
+                const double qq = raw.q[i], nm1 = raw.nm1[i];
+                h = qq * (1.0 - qq) / (nm1 > 1.0 ? nm1 : 1.0);
+                qp = qq - 0.5;
             }
             sq[row * kStageLd + kk] = qp;
             sh[row * kStageLd + kk] = h;
         }
     }
 };
+
+// Rows, SNP range and block of a prep unit (32-row group, 64-SNP chunk).
+struct PrepUnit {
+    int g, live;
+    std::int64_t c, snp0;
+};
+
+__device__ inline PrepUnit prep_unit(std::int64_t u, std::int64_t n_chunks, const BlockTable& bt,
+                                     const std::int32_t* __restrict__ chunk_block) {
+    PrepUnit pu;
+    pu.g = static_cast<int>(u / n_chunks);
+    pu.c = u - static_cast<std::int64_t>(pu.g) * n_chunks;
+    const int b = chunk_block[pu.c];
+    const std::int64_t local0 = 64 * pu.c - bt.k0[b];
+    pu.snp0 = bt.s0[b] + local0;
+    pu.live = static_cast<int>(max<std::int64_t>(0, min<std::int64_t>(64, bt.len[b] - local0)));
+    return pu;
+}
 
 __device__ inline unsigned bf16_bits(double integer_digit) {
     // Integer digits |d| <= 2^14 and their 2^-8 multiples have <= 8 significant
@@ -152,17 +203,23 @@ __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32
     const int t4 = tid >> 6, lane = tid & 63;
     const std::int64_t n_units = static_cast<std::int64_t>(G32) * n_chunks;
     // Units run chunk-fastest so consecutive workgroups read adjacent row segments and
-    // write adjacent plane chunks.
-    for (std::int64_t u = blockIdx.x; u < n_units; u += gridDim.x) {
-        const int g = static_cast<int>(u / n_chunks);
-        const std::int64_t c = u - static_cast<std::int64_t>(g) * n_chunks;
-        const int b = chunk_block[c];
-        const std::int64_t local0 = 64 * c - bt.k0[b];
-        const std::int64_t snp0 = bt.s0[b] + local0;
-        const int live = static_cast<int>(max<std::int64_t>(0, min<std::int64_t>(64, bt.len[b] - local0)));
+    // write adjacent plane chunks.  The next unit's inputs are fetched into registers
+    // before the current unit is converted, so their latency overlaps the conversion.
+    std::int64_t u = blockIdx.x;
+    if (u >= n_units) return;
+    PrepUnit nxt = prep_unit(u, n_chunks, bt, chunk_block);
+    typename Src::Raw raw = src.fetch(32 * nxt.g, P, nxt.snp0, nxt.live, tid);
+    for (; u < n_units; u += gridDim.x) {
+        const PrepUnit cur = nxt;
+        const int g = cur.g;
+        const std::int64_t c = cur.c;
         __syncthreads();  // the previous unit's readers are done with the stage
-        src.stage(32 * g, P, snp0, live, tid, sq, sh);
+        src.put(raw, tid, sq, sh);
         __syncthreads();
+        if (u + gridDim.x < n_units) {
+            nxt = prep_unit(u + gridDim.x, n_chunks, bt, chunk_block);
+            raw = src.fetch(32 * nxt.g, P, nxt.snp0, nxt.live, tid);
+        }
 
         // BF16 digit planes: thread -> (16-SNP step t4 of the chunk, fragment lane).
         const int r = lane & 31;
