@@ -231,38 +231,77 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq, FitShape s, int i
             double row[T];
             double diag = 0.0;
             int uc = 0, vc = 0;
-            static_for<T>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                row[c] = 0.0;
-                if (c < t) {
+            // Columns in pairs (c0, c0 + 1): both W columns in one pass (the second in
+            // the Jacobi work buffer, free after the seed), then both entries of the
+            // lane's row of C; each sum keeps its order, so the values are those of a
+            // column-by-column build.
+            static_for<T / 2>([&](auto cp) {
+                constexpr int c0 = 2 * decltype(cp)::value, c1 = c0 + 1;
+                row[c0] = 0.0;
+                row[c1] = 0.0;
+                if (c0 < t) {
+                    const bool two = c1 < t;
+                    int uc1 = uc, vc1 = vc + 1;
+                    if (vc1 == div) {
+                        vc1 = 0;
+                        ++uc1;
+                    }
                     // W[:, c] = Qsub L[:, c]
                     if (lane < m) {
-                        double acc = 0.0;
+                        double acc0 = 0.0, acc1 = 0.0;
                         if (!bstep) {
-                            const double* qq = Qr + src_of(uc, s.drop) * nr;
-                            const double* bb = W.b + vc * nr;
-                            for (int j = 0; j < nr; ++j) acc += qq[j] * bb[j];
+                            const double* qq0 = Qr + src_of(uc, s.drop) * nr;
+                            const double* qq1 = Qr + src_of(uc1, s.drop) * nr;
+                            const double* bb0 = W.b + vc * nr;
+                            const double* bb1 = W.b + vc1 * nr;
+                            for (int j = 0; j < nr; ++j) {
+                                acc0 += qq0[j] * bb0[j];
+                                if (two) acc1 += qq1[j] * bb1[j];
+                            }
                         } else {
-                            for (int i = 0; i < nl; ++i) acc += Qr[src_of(i, s.drop) * nr + vc] * W.a[i * r + uc];
+                            for (int i = 0; i < nl; ++i) {
+                                const double* qi = Qr + src_of(i, s.drop) * nr;
+                                acc0 += qi[vc] * W.a[i * r + uc];
+                                if (two) acc1 += qi[vc1] * W.a[i * r + uc1];
+                            }
                         }
-                        W.w[lane] = acc;
+                        W.w[lane] = acc0;
+                        if (two) W.ws[lane] = acc1;
                     }
                     __builtin_amdgcn_wave_barrier();
                     // C[a][c] = L[:, a]^T W[:, c]
-                    double cv = 0.0;
+                    double cv0 = 0.0, cv1 = 0.0;
                     if (lane < t) {
                         if (!bstep) {
                             const double* bb = W.b + va * nr;
-                            const double* ww = W.w + ua * nr;
-                            for (int j = 0; j < nr; ++j) cv += bb[j] * ww[j];
+                            const double* ww0 = W.w + ua * nr;
+                            const double* ww1 = W.ws + ua * nr;
+                            for (int j = 0; j < nr; ++j) {
+                                const double bj = bb[j];
+                                cv0 += bj * ww0[j];
+                                if (two) cv1 += bj * ww1[j];
+                            }
                         } else {
-                            for (int i = 0; i < nl; ++i) cv += W.a[i * r + ua] * W.w[i * nr + va];
+                            for (int i = 0; i < nl; ++i) {
+                                const double ai = W.a[i * r + ua];
+                                cv0 += ai * W.w[i * nr + va];
+                                if (two) cv1 += ai * W.ws[i * nr + va];
+                            }
                         }
                     }
                     __builtin_amdgcn_wave_barrier();
-                    row[c] = cv;
-                    if (lane == c) diag = cv;
-                    if (++vc == div) { vc = 0; ++uc; }
+                    row[c0] = cv0;
+                    if (lane == c0) diag = cv0;
+                    if (two) {
+                        row[c1] = cv1;
+                        if (lane == c1) diag = cv1;
+                    }
+                    uc = uc1;
+                    vc = vc1;
+                    if (++vc == div) {
+                        vc = 0;
+                        ++uc;
+                    }
                 }
             });
             double tr = 0.0;
