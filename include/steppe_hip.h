@@ -162,6 +162,60 @@ int stp_qpadm_batch(stp_context* ctx, const stp_f2_handle* f2, int32_t n_models,
                     const int32_t* left_offsets, const int32_t* left, const int32_t* right_offsets,
                     const int32_t* right, const stp_qpadm_options* options, stp_qpadm_result* out);
 
+/* ---- standalone f-statistics on a resident f2 handle ----------------------------------- */
+
+/* f4 statistics (reference: steppe::run_f4 over the device tensor, src/core/qpadm/f4.cpp:38-111,
+ * assemble_f4_quartets + jackknife_diag, src/device/backend.hpp; qpDstat on the f2 path is
+ * the same computation, steppe::run_qpdstat).  quartets holds n rows (p1, p2, p3, p4) of
+ * population indices; f4 = (f2(p2,p3) + f2(p1,p4) - f2(p1,p3) - f2(p2,p4)) / 2 per surviving
+ * block, the weighted block-jackknife estimate est, its standard error se (NaN when the
+ * variance is not positive), z = est / se and the two-sided normal p.  Out arrays hold n
+ * values each; all NaN when no block survives.  Out-of-range indices: STP_ERR_INVALID. */
+int stp_f4(stp_context* ctx, const stp_f2_handle* f2, const int32_t* quartets, int64_t n, double* est, double* se,
+           double* z, double* p);
+
+/* f3 statistics (reference: steppe::run_f3, src/core/qpadm/f3.cpp:37-105): rows (C, A, B),
+ * f3 = (f2(C,A) + f2(C,B) - f2(A,B)) / 2 per block, read out as stp_f4. */
+int stp_f3(stp_context* ctx, const stp_f2_handle* f2, const int32_t* triples, int64_t n, double* est, double* se,
+           double* z, double* p);
+
+/* f4-ratio (reference: steppe::run_f4ratio, src/core/qpadm/f4ratio.cpp:34-100): rows
+ * (p1, p2, p3, p4, p5), alpha = f4(p1,p2;p3,p4) / f4(p1,p2;p5,p4) by the ratio block
+ * jackknife; blocks whose denominator block value is below 1e-6 in magnitude are left
+ * out for that row.  alpha / se / z are NaN when fewer than two blocks remain. */
+int stp_f4ratio(stp_context* ctx, const stp_f2_handle* f2, const int32_t* tuples, int64_t n, double* alpha,
+                double* se, double* z);
+
+/* All-combination sweep request (reference: SweepRequest, include/steppe/fstat_sweep.hpp:25-38). */
+#define STP_SWEEP_MIN_Z 0   /* keep |z| >= min_z */
+#define STP_SWEEP_TOP_K 1   /* keep the top_k largest |z| among those with |z| >= min_z */
+#define STP_SWEEP_AUTO 0    /* method: library's choice */
+#define STP_SWEEP_DIRECT 1  /* method: each statistic from its own block series */
+#define STP_SWEEP_GRAM 2    /* method: f4 only, block-covariance Gram matrices on FP64 MFMA */
+typedef struct stp_sweep_request {
+    int32_t k;              /* 4: f4 of (c0, c1, c2, c3); 3: f3 of (c0; c1, c2), c0 < c1 < ... */
+    int32_t filter;         /* STP_SWEEP_MIN_Z or STP_SWEEP_TOP_K */
+    double min_z;           /* 3.0 */
+    int64_t top_k;          /* 100; <= 0 takes the default reservoir of 1e6 */
+    int32_t sure;           /* non-zero lifts the 1e8-combination cap */
+    int32_t method;         /* STP_SWEEP_AUTO / DIRECT / GRAM */
+    const int32_t* subset;  /* n_subset population indices to combine (NULL: all P) */
+    int32_t n_subset;
+} stp_sweep_request;
+
+/* All-combination f4 / f3 sweep (reference: steppe::run_f4_sweep / run_f3_sweep,
+ * src/core/qpadm/fstat_sweep.cpp:31-110).  Enumerates the C(range, k) combinations of the
+ * population range, keeps those passing the filter, at most K of them (top_k in top-K mode,
+ * 1e6 otherwise), ordered by |z| descending then combination order.  keys receives 4 ints
+ * per row (population indices; the fourth is 0 for f3), est / se / z / p one value per
+ * row, up to capacity rows; *n_out is the number of survivors (may exceed capacity: then
+ * only the first capacity rows are written).  *enumerated = C(range, k); when it exceeds
+ * 1e8 and sure is 0 nothing is computed, *capped = 1 and *status = 5 (invalid_config);
+ * otherwise *status = 0. */
+int stp_fstat_sweep(stp_context* ctx, const stp_f2_handle* f2, const stp_sweep_request* req, int64_t capacity,
+                    int32_t* keys, double* est, double* se, double* z, double* p, int64_t* n_out,
+                    uint64_t* enumerated, int32_t* capped, int32_t* status);
+
 /* ---- qpGraph ------------------------------------------------------------------------ */
 
 typedef struct stp_qpgraph_topology stp_qpgraph_topology;
@@ -271,9 +325,10 @@ int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_s
  * around the decode, prep and f2 contraction launches. */
 int stpx_timing(stp_context* ctx, int32_t enable);
 /* Accumulated milliseconds and launch counts per stage [decode, prep, f2, count, qpadm,
- * qpgraph] (f2 = the contraction kernel, count = the pair-count kernel, qpadm = the
+ * qpgraph, fstat] (f2 = the contraction kernel, count = the pair-count kernel, qpadm = the
  * batched fit kernels of one stp_qpadm_batch call, qpgraph = one stp_qpgraph_eval or
- * stp_qpgraph_fit launch); reset clears them. */
+ * stp_qpgraph_fit launch, fstat = the kernels of one f4 / f3 / f4-ratio / sweep call);
+ * reset clears them. */
 int stpx_timing_read(stp_context* ctx, double* ms, int64_t* count, int32_t reset);
 
 /* ---- synthetic panels (bench/test input; no counterpart in the reference) ------------ */

@@ -14,7 +14,11 @@
 //   ref_qpadm            steppe::run_qpadm (host tensor)    src/core/qpadm/qpadm_fit.cpp:216
 //   ref_qpgraph_fit      steppe::run_qpgraph (host tensor)  src/core/qpadm/qpgraph_fit.cpp:127
 //   ref_qpgraph_score    core::qpadm::qpgraph_score         src/core/qpadm/qpgraph_objective.hpp:163
+//   ref_f4               steppe::run_f4 (host tensor)       src/core/qpadm/f4.cpp:106
+//   ref_f3               steppe::run_f3 (host tensor)       src/core/qpadm/f3.cpp:105
+//   ref_f4ratio          steppe::run_f4ratio (host tensor)  src/core/qpadm/f4ratio.cpp:94
 
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -29,6 +33,9 @@
 #include "device/backend.hpp"
 #include "device/backend_factory.hpp"
 #include "device/resources.hpp"
+#include "steppe/f3.hpp"
+#include "steppe/f4.hpp"
+#include "steppe/f4ratio.hpp"
 #include "steppe/fstats.hpp"
 #include "steppe/qpadm.hpp"
 #include "steppe/qpgraph.hpp"
@@ -59,6 +66,17 @@ steppe::DecodeTileView tile_view(const std::uint8_t* packed, std::size_t bpr, st
     t.sample_ploidy = sample_ploidy;
     t.ploidy = ploidy;
     t.detect_ploidy_on_device = detect != 0;
+    return t;
+}
+
+steppe::F2BlockTensor host_tensor(const double* f2, const double* vpair, const std::int32_t* bsz, int P, int nb) {
+    steppe::F2BlockTensor t;
+    t.P = P;
+    t.n_block = nb;
+    const std::size_t tot = static_cast<std::size_t>(P) * P * nb;
+    t.f2.assign(f2, f2 + tot);
+    if (vpair) t.vpair.assign(vpair, vpair + tot);
+    t.block_sizes.assign(bsz, bsz + nb);
     return t;
 }
 
@@ -264,6 +282,61 @@ double ref_qpgraph_score(int n_edges, const char* const* from, const char* const
     if (bl) put(bl, blv, blv.size());
     if (fit) put(fit, fitv, fitv.size());
     return s;
+}
+
+// run_f4 / run_f3 over N index tuples (4 or 3 ints each); est/se/z/p per tuple.
+int ref_f4(const double* f2, const double* vpair, const std::int32_t* bsz, int P, int nb, const std::int32_t* quartets,
+           int N, double* est, double* se, double* z, double* p) {
+    try {
+        const auto t = host_tensor(f2, vpair, bsz, P, nb);
+        std::vector<std::array<int, 4>> q(N);
+        for (int k = 0; k < N; ++k) q[k] = {quartets[4 * k], quartets[4 * k + 1], quartets[4 * k + 2], quartets[4 * k + 3]};
+        auto res = cpu_resources();
+        const steppe::F4Result r = steppe::run_f4(t, q, steppe::QpAdmOptions{}, res);
+        put(est, r.est, N);
+        put(se, r.se, N);
+        put(z, r.z, N);
+        put(p, r.p, N);
+        return static_cast<int>(r.status);
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_f3(const double* f2, const double* vpair, const std::int32_t* bsz, int P, int nb, const std::int32_t* triples,
+           int N, double* est, double* se, double* z, double* p) {
+    try {
+        const auto t = host_tensor(f2, vpair, bsz, P, nb);
+        std::vector<std::array<int, 3>> q(N);
+        for (int k = 0; k < N; ++k) q[k] = {triples[3 * k], triples[3 * k + 1], triples[3 * k + 2]};
+        auto res = cpu_resources();
+        const steppe::F3Result r = steppe::run_f3(t, q, steppe::QpAdmOptions{}, res);
+        put(est, r.est, N);
+        put(se, r.se, N);
+        put(z, r.z, N);
+        put(p, r.p, N);
+        return static_cast<int>(r.status);
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_f4ratio(const double* f2, const double* vpair, const std::int32_t* bsz, int P, int nb,
+                const std::int32_t* tuples, int N, double* alpha, double* se, double* z) {
+    try {
+        const auto t = host_tensor(f2, vpair, bsz, P, nb);
+        std::vector<std::array<int, 5>> q(N);
+        for (int k = 0; k < N; ++k)
+            q[k] = {tuples[5 * k], tuples[5 * k + 1], tuples[5 * k + 2], tuples[5 * k + 3], tuples[5 * k + 4]};
+        auto res = cpu_resources();
+        const steppe::F4RatioResult r = steppe::run_f4ratio(t, q, steppe::QpAdmOptions{}, res);
+        put(alpha, r.alpha, N);
+        put(se, r.se, N);
+        put(z, r.z, N);
+        return static_cast<int>(r.status);
+    } catch (const std::exception&) {
+        return -1;
+    }
 }
 
 }  // extern "C"

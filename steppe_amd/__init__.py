@@ -15,8 +15,14 @@ from ._core import (  # noqa: F401
     read_f2,
     read_f2_host,
     run_extract_f2,
+    run_f3,
+    run_f4,
+    run_f4ratio,
+    run_fstat_sweep,
     run_qpadm,
     run_qpadm_search,
+    run_qpdstat,
+    run_qpwave,
     upload_f2,
     write_f2_dir,
 )
@@ -40,3 +46,75 @@ def qpadm(f2, target, left, right, **kw):
 def qpadm_search(f2, target, lefts, right, **kw):
     """Facade over ``_core.run_qpadm_search``."""
     return run_qpadm_search(f2, target, lefts, right, **kw)
+
+
+class FstatTable:
+    """A table of per-row statistics with list-valued columns (pop1.. then the values),
+    the shape of the reference's F4Result / F3Result / F4RatioResult
+    (bindings/steppe/__init__.py:321-441): attribute access per column, len(), and
+    ``.table`` as a pandas DataFrame."""
+
+    def __init__(self, d: dict, columns):
+        self._d = d
+        self.columns = list(columns)
+        for c in self.columns:
+            setattr(self, c, list(d[c]))
+        self.status = d["status"]
+        self.precision = d["precision"]
+
+    @property
+    def table(self):
+        import pandas as pd
+        return pd.DataFrame({c: list(getattr(self, c)) for c in self.columns})
+
+    def __len__(self) -> int:
+        return len(getattr(self, self.columns[-1]))
+
+    def __repr__(self) -> str:
+        return f"FstatTable(rows={len(self)}, status={self.status})"
+
+
+_F4_COLS = ("pop1", "pop2", "pop3", "pop4", "est", "se", "z", "p")
+_F3_COLS = ("pop1", "pop2", "pop3", "est", "se", "z", "p")
+_F4R_COLS = ("pop1", "pop2", "pop3", "pop4", "pop5", "alpha", "se", "z")
+
+
+def f4(f2, quartets, *, as_dataframe=False):
+    """f4(p1,p2;p3,p4) per (p1, p2, p3, p4) row: est / se / z / p (bindings/steppe/__init__.py:865)."""
+    res = FstatTable(run_f4(f2, quartets), _F4_COLS)
+    return res.table if as_dataframe else res
+
+
+def qpdstat(f2, quartets, *, as_dataframe=False):
+    """qpDstat over f2 data: the f4 statistic per quadruple (bindings/steppe/__init__.py:891)."""
+    res = FstatTable(run_qpdstat(f2, quartets), _F4_COLS)
+    return res.table if as_dataframe else res
+
+
+def f3(f2, triples, *, as_dataframe=False):
+    """f3(C;A,B) per (C, A, B) row: est / se / z / p (bindings/steppe/__init__.py:1045)."""
+    res = FstatTable(run_f3(f2, triples), _F3_COLS)
+    return res.table if as_dataframe else res
+
+
+def f4ratio(f2, tuples, *, as_dataframe=False):
+    """alpha = f4(p1,p2;p3,p4) / f4(p1,p2;p5,p4) per 5-tuple (bindings/steppe/__init__.py:1073)."""
+    res = FstatTable(run_f4ratio(f2, tuples), _F4R_COLS)
+    return res.table if as_dataframe else res
+
+
+def qpwave(f2, *, left, right, fudge=1e-4, rank_alpha=0.05):
+    """qpWave rank sweep, left[0] the reference row (bindings/steppe/__init__.py:632)."""
+    return run_qpwave(f2, left, right, fudge, rank_alpha)
+
+
+def f4_sweep(f2, pops=None, *, min_z=3.0, top_k=0, sure=False, all_combinations=False, method="auto"):
+    """Every f4 over the 4-subsets of pops (all populations by default), filtered by |z|
+    (src/core/qpadm/fstat_sweep.cpp run_f4_sweep; the CLI's f4-sweep)."""
+    return run_fstat_sweep(f2, 4, pops, min_z=min_z, top_k=top_k, sure=sure, all_combinations=all_combinations,
+                           method=method)
+
+
+def f3_sweep(f2, pops=None, *, min_z=3.0, top_k=0, sure=False, all_combinations=False):
+    """Every f3 over the 3-subsets of pops, filtered by |z| (run_f3_sweep)."""
+    return run_fstat_sweep(f2, 3, pops, min_z=min_z, top_k=top_k, sure=sure, all_combinations=all_combinations)

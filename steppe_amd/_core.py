@@ -107,6 +107,12 @@ class QpAdmOptions(C.Structure):
                 ("p_se_threshold", C.c_double)]
 
 
+class SweepRequest(C.Structure):
+    """stp_sweep_request (include/steppe_hip.h); reference: steppe::SweepRequest."""
+    _fields_ = [("k", C.c_int32), ("filter", C.c_int32), ("min_z", C.c_double), ("top_k", C.c_int64),
+                ("sure", C.c_int32), ("method", C.c_int32), ("subset", C.c_void_p), ("n_subset", C.c_int32)]
+
+
 # The reference's jackknife policy names (bindings/internal/bind_common.hpp:168-178).
 JACKKNIFE = {"none": 0, "feasible": 1, "feasible_only": 1, "all": 2}
 
@@ -147,6 +153,11 @@ def lib():
         "stp_f2bin_read": (C.c_int, [C.c_char_p, vp, vp, vp]),
         "stp_qpadm_batch": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, C.POINTER(QpAdmOptions),
                                       C.POINTER(QpAdmResult)]),
+        "stp_f4": (C.c_int, [vp, vp, vp, i64, vp, vp, vp, vp]),
+        "stp_f3": (C.c_int, [vp, vp, vp, i64, vp, vp, vp, vp]),
+        "stp_f4ratio": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
+        "stp_fstat_sweep": (C.c_int, [vp, vp, C.POINTER(SweepRequest), i64, vp, vp, vp, vp, vp, C.POINTER(i64),
+                                      C.POINTER(C.c_uint64), C.POINTER(i32), C.POINTER(i32)]),
         "stpx_timing": (C.c_int, [vp, i32]),
         "stpx_timing_read": (C.c_int, [vp, vp, vp, i32]),
         "stpx_synth_panel": (C.c_int, [vp, C.c_uint64, i32, i64, i64, i64, vp, vp, vp, vp, vp]),
@@ -194,7 +205,7 @@ class Context:
             cls._by_device[int(device)] = ctx
         return ctx
 
-    STAGES = ("decode", "prep", "f2", "count", "qpadm", "qpgraph")
+    STAGES = ("decode", "prep", "f2", "count", "qpadm", "qpgraph", "fstat")
 
     def timing(self, enable: bool = True):
         check(lib().stpx_timing(self.h, 1 if enable else 0))
@@ -537,3 +548,134 @@ def run_qpadm_search(f2: F2Handle, target, lefts, right, rank=-1, fudge=1e-4, al
             raise ValueError(f"qpadm_search: model {i} has an empty left source list")
     return qpadm_models(f2, [(target, list(l), list(right)) for l in lefts], rank=rank, fudge=fudge,
                         als_iterations=als_iterations, rank_alpha=rank_alpha, jackknife=jackknife)
+
+
+# ---------------------------------------------------------------------------------------
+# standalone f-statistics (bindings/bind_fstats.cpp:29-120, bindings/steppe/__init__.py:865-1102)
+# ---------------------------------------------------------------------------------------
+def _tuples(f2: F2Handle, rows, width: int, what: str) -> np.ndarray:
+    idx = np.zeros((len(rows), width), np.int32)
+    for i, r in enumerate(rows):
+        if isinstance(r, dict):
+            r = [r[f"pop{j + 1}"] for j in range(width)]
+        r = list(r)
+        if len(r) != width:
+            raise ValueError(f"{what}: row {i} needs {width} populations, got {len(r)}")
+        for j, name in enumerate(r):
+            idx[i, j] = _resolve(f2, name, what)
+    return idx
+
+
+def _names(f2: F2Handle, idx: np.ndarray, width: int) -> dict:
+    pops = f2.pops
+    return {f"pop{j + 1}": [pops[int(v)] if pops else int(v) for v in idx[:, j]] for j in range(width)}
+
+
+def _fstat(f2: F2Handle, rows, width: int, fn: str, what: str) -> dict:
+    idx = _tuples(f2, rows, width, what)
+    n = idx.shape[0]
+    est, se, z, p = (np.zeros(max(n, 1)) for _ in range(4))
+    if n:
+        ctx = Context.get(f2.device)
+        check(getattr(lib(), fn)(ctx.h, f2.raw, _ptr(idx), n, _ptr(est), _ptr(se), _ptr(z), _ptr(p)), what)
+    d = _names(f2, idx, width)
+    d.update({"est": est[:n].tolist(), "se": se[:n].tolist(), "z": z[:n].tolist(), "p": p[:n].tolist(),
+              "status": "ok", "precision": "fp64"})
+    return d
+
+
+def run_f4(f2: F2Handle, quartets) -> dict:
+    """f4(p1,p2;p3,p4) per quartet: est / se / z / p (reference: run_f4, src/core/qpadm/f4.cpp)."""
+    return _fstat(f2, quartets, 4, "stp_f4", "f4")
+
+
+def run_qpdstat(f2: F2Handle, quartets) -> dict:
+    """qpDstat on the f2 path: the same statistic as f4 (bindings/bind_fstats.cpp:52-75)."""
+    return _fstat(f2, quartets, 4, "stp_f4", "qpdstat")
+
+
+def run_f3(f2: F2Handle, triples) -> dict:
+    """f3(C;A,B) per triple: est / se / z / p (reference: run_f3, src/core/qpadm/f3.cpp)."""
+    return _fstat(f2, triples, 3, "stp_f3", "f3")
+
+
+def run_f4ratio(f2: F2Handle, tuples) -> dict:
+    """alpha = f4(p1,p2;p3,p4) / f4(p1,p2;p5,p4) per 5-tuple with the ratio block jackknife
+    (reference: run_f4ratio, src/core/qpadm/f4ratio.cpp)."""
+    idx = _tuples(f2, tuples, 5, "f4ratio")
+    n = idx.shape[0]
+    alpha, se, z = (np.zeros(max(n, 1)) for _ in range(3))
+    if n:
+        ctx = Context.get(f2.device)
+        check(lib().stp_f4ratio(ctx.h, f2.raw, _ptr(idx), n, _ptr(alpha), _ptr(se), _ptr(z)), "f4ratio")
+    d = _names(f2, idx, 5)
+    d.update({"alpha": alpha[:n].tolist(), "se": se[:n].tolist(), "z": z[:n].tolist(), "status": "ok",
+              "precision": "fp64"})
+    return d
+
+
+SWEEP_METHODS = {"auto": 0, "direct": 1, "gram": 2}
+
+
+def run_fstat_sweep(f2: F2Handle, k: int, pops=None, min_z: float = 3.0, top_k: int = 0, sure: bool = False,
+                    all_combinations: bool = False, method: str = "auto") -> dict:
+    """All-combination f4 (k=4) / f3 (k=3) sweep (reference: run_f4_sweep / run_f3_sweep,
+    src/core/qpadm/fstat_sweep.cpp, driven as cmd_fstat_sweep.cpp:96-110 drives it):
+    top_k > 0 keeps the top_k largest |z|; all_combinations keeps up to 1e6 by |z|;
+    otherwise every combination with |z| >= min_z (at most 1e6).  Rows are ordered by
+    |z| descending; keys name the populations (k columns)."""
+    if k not in (3, 4):
+        raise ValueError("fstat_sweep: k must be 3 or 4")
+    if method not in SWEEP_METHODS:
+        raise ValueError(f"fstat_sweep: method must be one of {sorted(SWEEP_METHODS)}")
+    sub = None if not pops else np.ascontiguousarray([_resolve(f2, x, "pops") for x in pops], np.int32)
+    if top_k > 0:
+        filt, K = 1, int(top_k)
+    elif all_combinations:
+        filt, K = 1, 1000000
+    else:
+        filt, K = 0, 1000000
+    from math import comb
+    rng = f2.P if sub is None else len(sub)
+    cap = max(1, min(K, comb(rng, k)))
+    req = SweepRequest(k, filt, float(min_z), K, 1 if sure else 0, SWEEP_METHODS[method],
+                       None if sub is None else sub.ctypes.data_as(C.c_void_p), 0 if sub is None else len(sub))
+    keys = np.zeros((cap, 4), np.int32)
+    est, se, z, p = (np.zeros(cap) for _ in range(4))
+    n_out, enum = C.c_int64(0), C.c_uint64(0)
+    capped, status = C.c_int32(0), C.c_int32(0)
+    ctx = Context.get(f2.device)
+    check(lib().stp_fstat_sweep(ctx.h, f2.raw, C.byref(req), cap, _ptr(keys), _ptr(est), _ptr(se), _ptr(z), _ptr(p),
+                                C.byref(n_out), C.byref(enum), C.byref(capped), C.byref(status)), "fstat_sweep")
+    n = min(int(n_out.value), cap)
+    d = _names(f2, keys[:n], k)
+    d.update({"keys": keys[:n, :k].copy(), "est": est[:n], "se": se[:n], "z": z[:n], "p": p[:n],
+              "enumerated": int(enum.value), "survivors": int(n_out.value), "capped": bool(capped.value),
+              "status": STATUS_NAMES.get(int(status.value), str(status.value)), "precision": "fp64"})
+    return d
+
+
+def run_qpwave(f2: F2Handle, left, right, fudge: float = 1e-4, rank_alpha: float = 0.05) -> dict:
+    """qpWave rank sweep with left[0] as the reference row (reference: run_qpwave,
+    src/core/qpadm/qpadm_fit.cpp:186-214).  Its f4 basis, covariance and rank sweep are
+    those of a qpAdm model with target left[0] and sources left[1:], so it runs through the
+    batched device fit without the leave-one-block-out refits."""
+    left, right = list(left), list(right)
+    if len(left) < 2:
+        raise ValueError("qpwave: left needs at least two populations")
+    if len(right) < 2:
+        raise ValueError("qpwave: right needs at least two populations")
+    r = qpadm_models(f2, [(left[0], left[1:], right)], fudge=fudge, rank_alpha=rank_alpha, jackknife="none")[0]
+    by_rank = {fr: (c, pv) for fr, c, pv in zip(r["rankdrop_f4rank"], r["rankdrop_chisq"], r["rankdrop_p"])}
+    n = len(r["rankdrop_f4rank"])
+    status = r["status"] if r["status"] in ("rank_deficient", "non_spd_covariance") else "ok"
+    return {
+        "rank_chisq": [by_rank[i][0] for i in range(n)],
+        "rank_dof": [d for _, d in sorted(zip(r["rankdrop_f4rank"], r["rankdrop_dof"]))],
+        "rank_p": [by_rank[i][1] for i in range(n)],
+        "rankdrop_f4rank": r["rankdrop_f4rank"], "rankdrop_dof": r["rankdrop_dof"],
+        "rankdrop_dofdiff": r["rankdrop_dofdiff"], "rankdrop_chisq": r["rankdrop_chisq"],
+        "rankdrop_p": r["rankdrop_p"], "rankdrop_chisqdiff": r["rankdrop_chisqdiff"],
+        "rankdrop_p_nested": r["rankdrop_p_nested"],
+        "f4rank": r["f4rank"], "est_rank": r["f4rank"], "status": status, "precision": "fp64",
+    }

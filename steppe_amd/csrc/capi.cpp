@@ -263,6 +263,67 @@ int stp_qpadm_batch(stp_context* ctx, const stp_f2_handle* f2, int32_t n_models,
     });
 }
 
+int stp_f4(stp_context* ctx, const stp_f2_handle* f2, const int32_t* quartets, int64_t n, double* est, double* se,
+           double* z, double* p) {
+    return guard([&] {
+        STP_REQUIRE(f2 && f2->blocks, "stp_f4: null f2 handle");
+        STP_REQUIRE(f2->ctx == ctx, "stp_f4: the f2 handle belongs to another context");
+        backend(ctx).fstat(*f2->blocks, 4, n, quartets, est, se, z, p);
+    });
+}
+
+int stp_f3(stp_context* ctx, const stp_f2_handle* f2, const int32_t* triples, int64_t n, double* est, double* se,
+           double* z, double* p) {
+    return guard([&] {
+        STP_REQUIRE(f2 && f2->blocks, "stp_f3: null f2 handle");
+        STP_REQUIRE(f2->ctx == ctx, "stp_f3: the f2 handle belongs to another context");
+        backend(ctx).fstat(*f2->blocks, 3, n, triples, est, se, z, p);
+    });
+}
+
+int stp_f4ratio(stp_context* ctx, const stp_f2_handle* f2, const int32_t* tuples, int64_t n, double* alpha,
+                double* se, double* z) {
+    return guard([&] {
+        STP_REQUIRE(f2 && f2->blocks, "stp_f4ratio: null f2 handle");
+        STP_REQUIRE(f2->ctx == ctx, "stp_f4ratio: the f2 handle belongs to another context");
+        backend(ctx).f4ratio(*f2->blocks, n, tuples, alpha, se, z);
+    });
+}
+
+int stp_fstat_sweep(stp_context* ctx, const stp_f2_handle* f2, const stp_sweep_request* req, int64_t capacity,
+                    int32_t* keys, double* est, double* se, double* z, double* p, int64_t* n_out,
+                    uint64_t* enumerated, int32_t* capped, int32_t* status) {
+    return guard([&] {
+        STP_REQUIRE(f2 && f2->blocks, "stp_fstat_sweep: null f2 handle");
+        STP_REQUIRE(f2->ctx == ctx, "stp_fstat_sweep: the f2 handle belongs to another context");
+        STP_REQUIRE(req && n_out && enumerated && capped && status, "stp_fstat_sweep: null argument");
+        STP_REQUIRE(capacity >= 0, "stp_fstat_sweep: negative capacity");
+        STP_REQUIRE(capacity == 0 || (keys && est && se && z && p), "stp_fstat_sweep: null output array");
+        STP_REQUIRE(req->n_subset >= 0 && (req->n_subset == 0 || req->subset), "stp_fstat_sweep: bad subset");
+        stp::FstatSweepRequest r;
+        r.k = req->k;
+        r.filter = req->filter;
+        r.min_z = req->min_z;
+        r.top_k = req->top_k;
+        r.sure = req->sure != 0;
+        r.method = req->method;
+        if (req->n_subset > 0) r.subset.assign(req->subset, req->subset + req->n_subset);
+        stp::FstatSweepOut o;
+        backend(ctx).fstat_sweep(*f2->blocks, r, o);
+        const std::int64_t ns = static_cast<std::int64_t>(o.est.size());
+        const std::int64_t w = std::min(ns, capacity);
+        std::copy(o.keys.begin(), o.keys.begin() + 4 * w, keys);
+        std::copy(o.est.begin(), o.est.begin() + w, est);
+        std::copy(o.se.begin(), o.se.begin() + w, se);
+        std::copy(o.z.begin(), o.z.begin() + w, z);
+        std::copy(o.p.begin(), o.p.begin() + w, p);
+        *n_out = ns;
+        *enumerated = o.enumerated;
+        *capped = o.capped ? 1 : 0;
+        *status = o.status;
+    });
+}
+
 int stp_qpgraph_parse(int32_t n_edges, const char* const* from, const char* const* to, int32_t n_names,
                       const char* const* names, stp_qpgraph_topology** out) {
     return guard([&] {

@@ -15,6 +15,7 @@
 
 #include "common.hpp"
 #include "f2_layout.hpp"
+#include "fstats.hpp"
 #include "qpgraph.hpp"
 #include "steppe_hip.h"
 
@@ -96,6 +97,31 @@ struct QgSet {
     DevBuf<double> in, out;
 };
 
+// An all-combination f-statistic sweep request (the reference's SweepRequest,
+// include/steppe/fstat_sweep.hpp:25-38): k = 4 (f4) or 3 (f3) over the C(range, k)
+// combinations of the population range (every population, or the subset's).
+struct FstatSweepRequest {
+    int k = 4;
+    std::vector<std::int32_t> subset;
+    int filter = 0;          // 0: |z| >= min_z; 1: the top_k largest |z| (also >= min_z)
+    double min_z = 3.0;
+    std::int64_t top_k = 100;
+    bool sure = false;       // lift the 1e8-combination cap
+    int method = 0;          // 0: automatic, 1: direct per statistic, 2: Gram (f4 only)
+};
+
+// Survivors ordered by |z| descending then combination rank; keys hold 4 population
+// indices per row (the fourth 0 for f3).  status: 0 ok, 5 invalid_config (capped).
+struct FstatSweepOut {
+    std::vector<std::int32_t> keys;
+    std::vector<double> est, se, z, p;
+    std::vector<std::int64_t> rank;
+    std::uint64_t enumerated = 0;
+    bool capped = false;
+    int status = 0;
+    std::int64_t rechecked = 0;   // Gram method: statistics recomputed directly
+};
+
 // Options of the device qpGraph fit (projected L-BFGS per instance).
 struct QgFitOptions {
     int maxit = 200;
@@ -145,6 +171,14 @@ public:
                      const std::int32_t* left, const std::int32_t* right_off, const std::int32_t* right,
                      const stp_qpadm_options& opt, stp_qpadm_result* out);
 
+    // Standalone f-statistics on a resident tensor (fstats_host.cpp): f4 (width 4) or
+    // f3 (width 3) over n index tuples, the f4-ratio over n 5-tuples, and the
+    // all-combination sweeps.  Results go to host arrays.
+    void fstat(const F2Blocks& h, int width, std::int64_t n, const std::int32_t* idx, double* est, double* se,
+               double* z, double* p);
+    void f4ratio(const F2Blocks& h, std::int64_t n, const std::int32_t* tuples, double* alpha, double* se, double* z);
+    void fstat_sweep(const F2Blocks& h, const FstatSweepRequest& req, FstatSweepOut& out);
+
     // qpGraph (qpgraph_host.cpp): build a resident set from topologies and f2 handles
     // (the f3 basis and covariance run on the device), evaluate scores / adjoints.
     QgSet* qpgraph_set_create(const std::vector<const QgTopology*>& topos, const F2Blocks* const* f2, double diag_f3);
@@ -164,7 +198,7 @@ public:
     void qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOptions& o, QgFleetOut& out);
 
     // Per-stage device timing with HIP events on this backend's stream (opt-in).
-    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kQpgraph = 5, kStages = 6 };
+    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kQpgraph = 5, kFstat = 6, kStages = 7 };
     void set_timing(bool on);
     void read_timing(double* ms, std::int64_t* count, bool reset);
 
@@ -173,6 +207,11 @@ public:
                      const std::int8_t* ind_haploid, std::uint8_t* d_packed);
 
 private:
+    // Surviving blocks of a resident tensor (blocks whose pairs are not partly missing),
+    // their sizes and total; then the survivor-transposed tensor in qpT_.
+    void survivor_list(const F2Blocks& h, std::vector<std::int32_t>& surv, std::vector<std::int32_t>& bsz,
+                       double& n_total);
+    FstatTensor survivor_tensor(const F2Blocks& h);
     void qpgraph_upload_topologies(QgSet& s, const std::vector<const QgTopology*>& topos);
     struct PlanDev {
         DevBuf<std::int64_t> s0, k0;
@@ -229,6 +268,13 @@ private:
     DevBuf<double> qpT_, qp_scratch_;
     DevBuf<std::int32_t> qp_i32_, qp_keep_;
     DevBuf<stp_qpadm_result> qp_out_;
+    // f-statistics working set (grow-only): survivor sizes / index tuples, outputs,
+    // sweep combinatorics and hit buffers.
+    DevBuf<std::int32_t> fs_i32_, fs_idx_;
+    DevBuf<double> fs_out_;
+    DevBuf<std::int64_t> fs_i64_;
+    DevBuf<SweepHit> fs_hits_;
+    DevBuf<unsigned long long> fs_count_;
     hipEvent_t ev_begin_[kStages] = {}, ev_end_[kStages] = {};
     bool pending_[kStages] = {};
     double total_ms_[kStages] = {};

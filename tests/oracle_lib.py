@@ -147,6 +147,74 @@ class _Lib:
         return out.to_dict()
 
 
+    # ---- f-statistics -------------------------------------------------------------
+    @staticmethod
+    def _tensor_args(f2, vpair, block_sizes):
+        f2 = np.asarray(f2, np.float64)
+        P, _, nb = f2.shape
+        fl = np.asfortranarray(f2).ravel(order="F")
+        vp = None if vpair is None else np.asfortranarray(np.asarray(vpair, np.float64)).ravel(order="F")
+        bs = np.ascontiguousarray(block_sizes, np.int32)
+        return (fl, vp, bs), (fl.ctypes.data_as(C.c_void_p), None if vp is None else vp.ctypes.data_as(C.c_void_p),
+                              bs.ctypes.data_as(C.c_void_p), C.c_int(P), C.c_int(nb))
+
+    def _fstat(self, name, f2, vpair, block_sizes, tuples, width, fields):
+        keep, targs = self._tensor_args(f2, vpair, block_sizes)
+        t = np.ascontiguousarray(np.asarray(tuples, np.int32).reshape(-1, width))
+        N = t.shape[0]
+        outs = [np.full(max(N, 1), -7.0) for _ in fields]
+        f = self.fn(name)
+        f.restype = C.c_int
+        rc = f(*targs, t.ctypes.data_as(C.c_void_p), C.c_int(N), *[o.ctypes.data_as(C.c_void_p) for o in outs])
+        assert rc == 0, rc
+        return {k: o[:N] for k, o in zip(fields, outs)}
+
+    def f4(self, f2, vpair, block_sizes, quartets):
+        """run_f4 over (N, 4) quartets -> dict of est / se / z / p arrays."""
+        return self._fstat("f4", f2, vpair, block_sizes, quartets, 4, ("est", "se", "z", "p"))
+
+    def f3(self, f2, vpair, block_sizes, triples):
+        """run_f3 over (N, 3) triples (C, A, B) -> dict of est / se / z / p arrays."""
+        return self._fstat("f3", f2, vpair, block_sizes, triples, 3, ("est", "se", "z", "p"))
+
+    def f4ratio(self, f2, vpair, block_sizes, tuples):
+        """run_f4ratio over (N, 5) tuples -> dict of alpha / se / z arrays."""
+        return self._fstat("f4ratio", f2, vpair, block_sizes, tuples, 5, ("alpha", "se", "z"))
+
+    def fstat_sweep(self, f2, vpair, block_sizes, k, subset=None, filter="min_z", min_z=3.0, top_k=100,
+                    sure=False):
+        """All-combination f4 (k=4) / f3 (k=3) sweep (restated oracle only)."""
+        keep, targs = self._tensor_args(f2, vpair, block_sizes)
+        sub = None if subset is None else np.ascontiguousarray(subset, np.int32)
+        P = np.asarray(f2).shape[0]
+        rng = P if sub is None else len(sub)
+        from math import comb
+        mode = 1 if filter == "top_k" else 0
+        K = (top_k if (mode == 1 and top_k > 0) else 1000000)
+        cap = max(1, min(K, comb(rng, k)))
+        keys = np.zeros((cap, 4), np.int32)
+        est, se, z, p = (np.zeros(cap) for _ in range(4))
+        n_out = C.c_int64(0)
+        enum = C.c_uint64(0)
+        capped = C.c_int(0)
+        f = self.fn("fstat_sweep")
+        f.restype = C.c_int
+        st = f(*targs, C.c_int(k), None if sub is None else sub.ctypes.data_as(C.c_void_p),
+               C.c_int(0 if sub is None else len(sub)), C.c_int(mode), C.c_double(min_z), C.c_int64(top_k),
+               C.c_int(1 if sure else 0), C.c_int64(cap), keys.ctypes.data_as(C.c_void_p),
+               est.ctypes.data_as(C.c_void_p), se.ctypes.data_as(C.c_void_p), z.ctypes.data_as(C.c_void_p),
+               p.ctypes.data_as(C.c_void_p), C.byref(n_out), C.byref(enum), C.byref(capped))
+        n = min(n_out.value, cap)
+        return {"status": st, "keys": keys[:n], "est": est[:n], "se": se[:n], "z": z[:n], "p": p[:n],
+                "enumerated": enum.value, "capped": bool(capped.value), "survivors": n_out.value}
+
+    def unrank(self, r, k):
+        c = np.zeros(4, np.int32)
+        f = self.fn("unrank")
+        f.restype = None
+        f(C.c_int64(r), C.c_int(k), c.ctypes.data_as(C.c_void_p))
+        return tuple(int(x) for x in c[:k])
+
     # ---- qpGraph ----------------------------------------------------------------
     @staticmethod
     def _strs(items):
