@@ -211,7 +211,11 @@ private:
     // their sizes and total; then the survivor-transposed tensor in qpT_.
     void survivor_list(const F2Blocks& h, std::vector<std::int32_t>& surv, std::vector<std::int32_t>& bsz,
                        double& n_total);
-    FstatTensor survivor_tensor(const F2Blocks& h);
+    FstatTensor survivor_tensor(const F2Blocks& h, std::vector<std::int32_t>* bsz_out = nullptr);
+    // The Gram form of an f4 sweep (fstats_host.cpp); appends |z| >= tau survivors.
+    void f4_sweep_gram(const FstatTensor& t, const std::vector<std::int32_t>& bsz, int range,
+                       const std::int32_t* d_subset, const std::vector<std::int64_t>& binom, std::int64_t K,
+                       int filter, double& tau, std::vector<SweepHit>& res, std::int64_t& rechecked);
     void qpgraph_upload_topologies(QgSet& s, const std::vector<const QgTopology*>& topos);
     struct PlanDev {
         DevBuf<std::int64_t> s0, k0;
@@ -275,6 +279,9 @@ private:
     DevBuf<std::int64_t> fs_i64_;
     DevBuf<SweepHit> fs_hits_;
     DevBuf<unsigned long long> fs_count_;
+    DevBuf<double> fs_mean_, fs_wroot_;
+    DevBuf<GramItem> fs_items_;
+    DevBuf<std::int64_t> fs_recheck_;
     hipEvent_t ev_begin_[kStages] = {}, ev_end_[kStages] = {};
     bool pending_[kStages] = {};
     double total_ms_[kStages] = {};

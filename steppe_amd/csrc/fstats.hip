@@ -341,10 +341,224 @@ __global__ __launch_bounds__(kThreads) void sweep_direct_kernel(SweepArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Weighted block means of every f2 series (one wavefront per series).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void f2_means_kernel(FstatTensor t, double* __restrict__ mean) {
+    const std::int64_t k = my_item(static_cast<std::int64_t>(t.P) * t.P);
+    if (k < 0) return;
+    const double* r = t.f2T + k * t.nbs;
+    dd s{0.0, 0.0};
+    for (int b = lane_id(); b < t.nbs; b += 64) s = dd_fma(s, r[b], static_cast<double>(t.bsz[b]));
+    const double m = dd_div_d(wave_sum_dd(s), t.n_total);
+    if (lane_id() == 0) mean[k] = m;
+}
+
+// ---------------------------------------------------------------------------------
+// Gram form of the f4 sweep: one workgroup per item (c2, c3, row block bi, row block bj).
+//
+// The centred, weighted series e_b(p) of the item's rows are staged 16 blocks at a time
+// into LDS as sE[k][slot] (slot < 128: rows of block bi, slot >= 128: rows of block bj);
+// each wavefront owns up to 8 16x16 tiles of S = E E^T and accumulates them with
+// v_mfma_f64_16x16x4f64 (A = rows of bi, B = rows of bj, both read from the same LDS
+// chunk).  The staging threads also accumulate the diagonal D(p) = sum_b e_b(p)^2.  The
+// next chunk's f2 values are loaded into registers before the current chunk's MFMAs.
+// Each f4 (c0 < c1) of a tile is then read out.  Its rank goes to the recheck list for the
+// direct kernel instead when the variance is formed with at least max_cancel-fold
+// cancellation, or the estimate is a difference of f2 means more than max_level times
+// larger than |est| + se (copied or nearly copied populations: there the reference's own
+// block values carry rounding noise at that level, which only its operation order
+// reproduces), or the variance is 0 / 0.
+// ---------------------------------------------------------------------------------
+constexpr int kGramWaves = 8;
+constexpr int kGramThreads = 64 * kGramWaves;
+constexpr int kGramK = 16;                                  // blocks per staged chunk
+constexpr int kGramLd = 2 * kGramRows + 1;                  // LDS stride of one block column
+constexpr int kGramSlots = 2 * kGramRows * kGramK / kGramThreads;   // staged rows per thread (8)
+constexpr int kGramMaxTiles = 8;                            // 64 tiles of a 128 x 128 block / 8 waves
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void wave_append(bool pred, unsigned long long* counter, std::int64_t cap, int lane,
+                                   unsigned long long& slot_out, bool& ok) {
+    const unsigned long long bal = __ballot(pred);
+    ok = false;
+    if (bal == 0) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(bal)));
+    base = __shfl(base, 0);
+    const unsigned long long off = __popcll(bal & ((1ULL << lane) - 1ULL));
+    slot_out = base + off;
+    ok = pred && static_cast<std::int64_t>(slot_out) < cap;
+}
+
+__global__ __launch_bounds__(kGramThreads) void f4_gram_kernel(GramArgs a) {
+    __shared__ double sE[kGramK * kGramLd];
+    __shared__ double sG[2 * kGramRows], sD[2 * kGramRows], sL[2 * kGramRows];
+    const std::int64_t item = die_major_group(a.n_items);
+    if (item < 0) return;
+    const GramItem it = a.items[item];
+    const FstatTensor& t = a.t;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int P = t.P, nbs = t.nbs;
+    const int m = it.c2;
+    const int r0i = it.bi * kGramRows, r0j = it.bj * kGramRows;
+    const int ni = min(kGramRows, m - r0i), nj = min(kGramRows, m - r0j);
+    const bool same = it.bi == it.bj;
+    const int pc2 = a.subset ? a.subset[it.c2] : it.c2;
+    const int pc3 = a.subset ? a.subset[it.c3] : it.c3;
+
+    // ---- staging slots of this thread ----
+    const int kcol = tid % kGramK;
+    std::int64_t offA[kGramSlots], offB[kGramSlots];
+    double Gs[kGramSlots], Ds[kGramSlots];
+    bool live[kGramSlots];
+#pragma unroll
+    for (int s = 0; s < kGramSlots; ++s) {
+        const int slot = tid / kGramK + (kGramThreads / kGramK) * s;
+        const bool lo = slot < kGramRows;
+        const int p = lo ? r0i + slot : r0j + (slot - kGramRows);
+        live[s] = lo ? slot < ni : (!same && slot - kGramRows < nj);
+        const int pp = live[s] ? (a.subset ? a.subset[p] : p) : 0;
+        const std::int64_t ra = static_cast<std::int64_t>(pp) + static_cast<std::int64_t>(P) * pc2;
+        const std::int64_t rb = static_cast<std::int64_t>(pp) + static_cast<std::int64_t>(P) * pc3;
+        offA[s] = ra * nbs;
+        offB[s] = rb * nbs;
+        const double ma = live[s] ? a.mean[ra] : 0.0, mb = live[s] ? a.mean[rb] : 0.0;
+        Gs[s] = ma - mb;
+        Ds[s] = 0.0;
+        if (kcol == 0) {
+            sG[slot] = Gs[s];
+            sL[slot] = fabs(ma) + fabs(mb);
+        }
+    }
+    double va[kGramSlots], vb[kGramSlots];
+    auto fetch = [&](int c) {
+        const int b = c * kGramK + kcol;
+#pragma unroll
+        for (int s = 0; s < kGramSlots; ++s) {
+            const bool ok = live[s] && b < nbs;
+            va[s] = ok ? t.f2T[offA[s] + b] : 0.0;
+            vb[s] = ok ? t.f2T[offB[s] + b] : 0.0;
+        }
+    };
+    auto stage = [&](int c) {
+        const int b = c * kGramK + kcol;
+        const double w = b < nbs ? a.wroot[b] : 0.0;
+#pragma unroll
+        for (int s = 0; s < kGramSlots; ++s) {
+            const int slot = tid / kGramK + (kGramThreads / kGramK) * s;
+            const double e = (live[s] && b < nbs) ? (va[s] - vb[s] - Gs[s]) * w : 0.0;
+            Ds[s] += e * e;
+            sE[kcol * kGramLd + slot] = e;
+        }
+    };
+
+    // ---- this wavefront's tiles ----
+    const int nti = (ni + 15) / 16, ntj = (nj + 15) / 16;
+    const int n_tiles = same ? nti * (nti + 1) / 2 : nti * ntj;
+    int colA[kGramMaxTiles], colB[kGramMaxTiles];
+    int my_tiles = 0;
+#pragma unroll
+    for (int u = 0; u < kGramMaxTiles; ++u) {
+        int tl = wave + kGramWaves * u;
+        colA[u] = colB[u] = 0;
+        if (tl < n_tiles) {
+            int ti = 0, tj = 0;
+            if (same) {
+                for (ti = 0; ti < nti; ++ti) {
+                    const int len = nti - ti;
+                    if (tl < len) { tj = ti + tl; break; }
+                    tl -= len;
+                }
+            } else {
+                ti = tl / ntj;
+                tj = tl - ti * ntj;
+            }
+            colA[u] = ti * 16;
+            colB[u] = (same ? 0 : kGramRows) + tj * 16;
+            my_tiles = u + 1;
+        }
+    }
+    dbl4 acc[kGramMaxTiles];
+#pragma unroll
+    for (int u = 0; u < kGramMaxTiles; ++u) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    // ---- stream the blocks ----
+    const int nch = (nbs + kGramK - 1) / kGramK;
+    fetch(0);
+    stage(0);
+    __syncthreads();
+    const int krow = lane >> 4, kc = lane & 15;
+    for (int c = 0; c < nch; ++c) {
+        if (c + 1 < nch) fetch(c + 1);
+#pragma unroll
+        for (int u = 0; u < kGramMaxTiles; ++u) {
+            if (u < my_tiles) {
+#pragma unroll
+                for (int ks = 0; ks < kGramK / 4; ++ks) {
+                    const double* row = sE + (4 * ks + krow) * kGramLd + kc;
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(row[colA[u]], row[colB[u]], acc[u], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+        if (c + 1 < nch) {
+            stage(c + 1);
+            __syncthreads();
+        }
+    }
+
+    // ---- diagonal: sum over the 16 lanes that staged each slot ----
+#pragma unroll
+    for (int s = 0; s < kGramSlots; ++s) {
+        double d = Ds[s];
+        d += __shfl_xor(d, 8);
+        d += __shfl_xor(d, 4);
+        d += __shfl_xor(d, 2);
+        d += __shfl_xor(d, 1);
+        if (kcol == 0) sD[tid / kGramK + (kGramThreads / kGramK) * s] = d;
+    }
+    __syncthreads();
+
+    // ---- read-out: lane holds S(row (lane/16) + 4r, col lane%16) of each tile ----
+    const double inv4nb = 1.0 / (4.0 * static_cast<double>(nbs));
+#pragma unroll
+    for (int u = 0; u < kGramMaxTiles; ++u) {
+        if (u >= my_tiles) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int sp = colA[u] + krow + 4 * r;          // slot of c0 (block bi rows)
+            const int sq = colB[u] + kc;                    // slot of c1
+            const int p = r0i + sp;
+            const int q = r0j + (sq - (same ? 0 : kGramRows));
+            const bool valid = sp < ni && q < m && p < q;
+            const double D0 = sD[sp], D1 = sD[sq];
+            const double sum = D0 + D1;
+            const double den = sum - 2.0 * acc[u][r];
+            const double var = den * inv4nb;
+            const double est = 0.5 * (sG[sq] - sG[sp]);
+            const double se = var > 0.0 ? sqrt(var) : __builtin_nan("");
+            // false for 0 / 0 and NaN
+            const bool precise = den * a.max_cancel > sum && sL[sp] + sL[sq] <= a.max_level * (fabs(est) + se);
+            const double z = est / se;
+            const std::int64_t rank = it.base + static_cast<std::int64_t>(q) * (q - 1) / 2 + p;
+            unsigned long long slot;
+            bool ok;
+            wave_append(valid && precise && fabs(z) >= a.tau, a.n_hits, a.cap, lane, slot, ok);
+            if (ok) a.hits[slot] = SweepHit{rank, est, se, z};
+            wave_append(valid && !precise, a.n_recheck, a.cap, lane, slot, ok);
+            if (ok) a.recheck[slot] = rank;
+        }
+    }
+}
+
 // Smallest cached width that holds ceil(nbs / 64) values per lane; 0 = streaming.
 int per_lane(int nbs) {
     const int need = (nbs + 63) / 64;
-    for (int k : {2, 4, 8, 16, 32})
+    if (need > kMaxPer) return 0;
+    for (int k : {2, 4, 8, 16, kMaxPer})
         if (need <= k) return k;
     return 0;
 }
@@ -386,6 +600,22 @@ void launch_f4ratio(const F4RatioArgs& a, hipStream_t st) {
     STP_REQUIRE(a.t.nbs >= 1, "f4ratio: no surviving block");
     STP_REQUIRE((a.N + kWaves - 1) / kWaves <= 0x7fffffffLL, "f4ratio: too many tuples for one launch");
     f4ratio_kernel<<<grid_for(a.N), kThreads, 0, st>>>(a);
+    STP_LAUNCH_CHECK();
+}
+
+void launch_f2_means(const FstatTensor& t, double* mean, hipStream_t st) {
+    const std::int64_t rows = static_cast<std::int64_t>(t.P) * t.P;
+    if (rows <= 0) return;
+    STP_REQUIRE(t.nbs >= 1, "f2 means: no surviving block");
+    f2_means_kernel<<<grid_for(rows), kThreads, 0, st>>>(t, mean);
+    STP_LAUNCH_CHECK();
+}
+
+void launch_f4_gram(const GramArgs& a, hipStream_t st) {
+    if (a.n_items <= 0) return;
+    STP_REQUIRE(a.t.nbs >= 1, "f4 Gram sweep: no surviving block");
+    STP_REQUIRE(a.n_items <= 0x7ffffff0LL, "f4 Gram sweep: too many items for one launch");
+    f4_gram_kernel<<<dim3(static_cast<unsigned>(round_up(a.n_items, 8))), kGramThreads, 0, st>>>(a);
     STP_LAUNCH_CHECK();
 }
 

@@ -57,4 +57,35 @@ struct SweepArgs {
 };
 void launch_sweep_direct(const SweepArgs& a, hipStream_t st);
 
+// Weighted block means of every f2 series: mean[i + P*j] = sum_s bsz[s] f2T[(i+P*j)*nbs+s] / n.
+void launch_f2_means(const FstatTensor& t, double* mean, hipStream_t st);
+
+// Gram form of the f4 sweep.  For positions c2 < c3 every f4 of (c0 < c1 < c2, c2, c3) is
+// 0.5 (g(c1) - g(c0)) per block with g(p) = f2(p,c2) - f2(p,c3), so its jackknife variance
+// is (D(c0) + D(c1) - 2 S(c0,c1)) / (4 nbs) with S the Gram matrix of the centred,
+// weighted series e_b(p) = (g_b(p) - G(p)) sqrt(w_b / (n - w_b)) and D its diagonal.
+// One item is one 128 x 128 block (bi <= bj) of S for one (c2, c3).
+constexpr int kGramRows = 128;       // rows of S per item block
+struct GramItem {
+    int c2, c3, bi, bj;
+    std::int64_t base;               // C(c2, 3) + C(c3, 4)
+};
+struct GramArgs {
+    FstatTensor t;
+    const double* mean = nullptr;    // launch_f2_means output
+    const double* wroot = nullptr;   // sqrt(w_b / (n - w_b)) per surviving block
+    const std::int32_t* subset = nullptr;
+    const GramItem* items = nullptr;
+    std::int64_t n_items = 0;
+    double tau = 0.0;
+    double max_cancel = 1e3;         // (D0 + D1) / (D0 + D1 - 2 S01) above this: recomputed directly
+    double max_level = 1e3;          // the four f2 means above this times (|est| + se): recomputed
+    SweepHit* hits = nullptr;
+    unsigned long long* n_hits = nullptr;
+    std::int64_t* recheck = nullptr;
+    unsigned long long* n_recheck = nullptr;
+    std::int64_t cap = 0;            // capacity of hits and of recheck
+};
+void launch_f4_gram(const GramArgs& a, hipStream_t st);
+
 }  // namespace stp

@@ -26,6 +26,7 @@ namespace {
 constexpr std::uint64_t kMaxComb = 100000000ULL;   // include/steppe/config.hpp:20
 constexpr std::int64_t kDefaultTopK = 1000000;      // include/steppe/config.hpp:21
 constexpr std::int64_t kDirectChunk = 1 << 22;      // combinations per direct launch
+constexpr std::int64_t kGramLaunch = 1 << 24;       // f4 statistics per Gram launch (about)
 
 std::uint64_t choose_sat(std::int64_t n, int k) {
     if (k < 0 || n < k) return 0;
@@ -62,6 +63,16 @@ bool hit_before(const SweepHit& a, const SweepHit& b) {
     return a.rank < b.rank;
 }
 
+// Keep the K best hits; in top-K mode raise tau to the K-th |z| (the reference's
+// compact_and_raise, cuda_backend_fstats_assemble.cu:181-209).
+void keep_top(std::vector<SweepHit>& res, std::int64_t K, int filter, double& tau) {
+    if (static_cast<std::int64_t>(res.size()) <= K) return;
+    std::nth_element(res.begin(), res.begin() + (K - 1), res.end(), hit_before);
+    const double kth = std::fabs(res[K - 1].z);
+    res.resize(K);
+    if (filter == 1 && kth > tau) tau = kth;
+}
+
 }  // namespace
 
 void HipBackend::survivor_list(const F2Blocks& h, std::vector<std::int32_t>& surv, std::vector<std::int32_t>& bsz,
@@ -85,10 +96,11 @@ void HipBackend::survivor_list(const F2Blocks& h, std::vector<std::int32_t>& sur
         }
 }
 
-FstatTensor HipBackend::survivor_tensor(const F2Blocks& h) {
+FstatTensor HipBackend::survivor_tensor(const F2Blocks& h, std::vector<std::int32_t>* bsz_out) {
     std::vector<std::int32_t> surv, bsz;
     double n_total = 0.0;
     survivor_list(h, surv, bsz, n_total);
+    if (bsz_out) *bsz_out = bsz;
     FstatTensor t;
     t.P = h.P;
     t.nbs = static_cast<int>(surv.size());
@@ -190,6 +202,108 @@ void HipBackend::f4ratio(const F2Blocks& h, std::int64_t n, const std::int32_t* 
     std::copy(host.begin() + 2 * n, host.end(), z);
 }
 
+// The Gram form: items are 128 x 128 row blocks of S for each (c2, c3), grouped into
+// launches of about kGramLaunch statistics (all items of a pair in one launch).
+// Statistics whose variance would be formed with too much cancellation come back as
+// ranks and are recomputed by the direct kernel in the same launch's hit buffer.
+void HipBackend::f4_sweep_gram(const FstatTensor& t, const std::vector<std::int32_t>& bsz, int range,
+                               const std::int32_t* d_subset, const std::vector<std::int64_t>& binom, std::int64_t K,
+                               int filter, double& tau, std::vector<SweepHit>& res, std::int64_t& rechecked) {
+    const std::int64_t rows = static_cast<std::int64_t>(t.P) * t.P;
+    fs_mean_.ensure(static_cast<std::size_t>(rows));
+    launch_f2_means(t, fs_mean_.get(), stream_);
+    std::vector<double> wroot(t.nbs);
+    for (int b = 0; b < t.nbs; ++b) {
+        const double w = static_cast<double>(bsz[b]);
+        wroot[b] = std::sqrt(w / (t.n_total - w));
+    }
+    fs_wroot_.ensure(wroot.size());
+    STP_HIP(hipMemcpyAsync(fs_wroot_.get(), wroot.data(), wroot.size() * sizeof(double), hipMemcpyHostToDevice,
+                           stream_));
+    auto C = [&](int v, int kk) { return binom[static_cast<std::size_t>(kk) * (range + 1) + v]; };
+    // launches: [item range, statistic count]
+    std::vector<GramItem> items;
+    std::vector<std::pair<std::size_t, std::int64_t>> cuts;   // (end item, statistics)
+    std::int64_t acc = 0, cap = 0;
+    for (int c3 = 3; c3 < range; ++c3)
+        for (int c2 = 2; c2 < c3; ++c2) {
+            const int nbk = (c2 + kGramRows - 1) / kGramRows;
+            const std::int64_t base = C(c2, 3) + C(c3, 4);
+            for (int bi = 0; bi < nbk; ++bi)
+                for (int bj = bi; bj < nbk; ++bj) items.push_back(GramItem{c2, c3, bi, bj, base});
+            acc += C(c2, 2);
+            if (acc >= kGramLaunch) {
+                cuts.emplace_back(items.size(), acc);
+                cap = std::max(cap, acc);
+                acc = 0;
+            }
+        }
+    if (acc > 0) {
+        cuts.emplace_back(items.size(), acc);
+        cap = std::max(cap, acc);
+    }
+    fs_items_.ensure(items.size());
+    STP_HIP(hipMemcpyAsync(fs_items_.get(), items.data(), items.size() * sizeof(GramItem), hipMemcpyHostToDevice,
+                           stream_));
+    fs_hits_.ensure(static_cast<std::size_t>(cap));
+    fs_recheck_.ensure(static_cast<std::size_t>(cap));
+    fs_count_.ensure(2);
+    std::vector<SweepHit> buf;
+    std::size_t i0 = 0;
+    for (const auto& [i1, nstat] : cuts) {
+        STP_HIP(hipMemsetAsync(fs_count_.get(), 0, 2 * sizeof(unsigned long long), stream_));
+        GramArgs g;
+        g.t = t;
+        g.mean = fs_mean_.get();
+        g.wroot = fs_wroot_.get();
+        g.subset = d_subset;
+        g.items = fs_items_.get() + i0;
+        g.n_items = static_cast<std::int64_t>(i1 - i0);
+        g.tau = tau;
+        g.hits = fs_hits_.get();
+        g.n_hits = fs_count_.get();
+        g.recheck = fs_recheck_.get();
+        g.n_recheck = fs_count_.get() + 1;
+        g.cap = cap;
+        launch_f4_gram(g, stream_);
+        unsigned long long cnt[2] = {0, 0};
+        STP_HIP(hipMemcpyAsync(cnt, fs_count_.get(), sizeof(cnt), hipMemcpyDeviceToHost, stream_));
+        STP_HIP(hipStreamSynchronize(stream_));
+        STP_REQUIRE(cnt[0] <= static_cast<unsigned long long>(nstat) &&
+                        cnt[1] <= static_cast<unsigned long long>(nstat),
+                    "fstat_sweep: Gram launch counts exceed its statistics");
+        if (cnt[1] > 0) {
+            SweepArgs d;
+            d.t = t;
+            d.k = 4;
+            d.range = range;
+            d.subset = d_subset;
+            d.binom = fs_i64_.get();
+            d.ranks = fs_recheck_.get();
+            d.count = static_cast<std::int64_t>(cnt[1]);
+            d.tau = tau;
+            d.hits = fs_hits_.get();
+            d.n_hits = fs_count_.get();
+            d.cap = cap;
+            launch_sweep_direct(d, stream_);
+            STP_HIP(hipMemcpyAsync(cnt, fs_count_.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   stream_));
+            STP_HIP(hipStreamSynchronize(stream_));
+            STP_REQUIRE(cnt[0] <= static_cast<unsigned long long>(nstat), "fstat_sweep: hit count exceeds the launch");
+            rechecked += static_cast<std::int64_t>(cnt[1]);
+        }
+        if (cnt[0] > 0) {
+            buf.resize(cnt[0]);
+            STP_HIP(hipMemcpyAsync(buf.data(), fs_hits_.get(), cnt[0] * sizeof(SweepHit), hipMemcpyDeviceToHost,
+                                   stream_));
+            STP_HIP(hipStreamSynchronize(stream_));
+            res.insert(res.end(), buf.begin(), buf.end());
+            keep_top(res, K, filter, tau);
+        }
+        i0 = i1;
+    }
+}
+
 void HipBackend::fstat_sweep(const F2Blocks& h, const FstatSweepRequest& req, FstatSweepOut& out) {
     activate();
     out = FstatSweepOut{};
@@ -210,7 +324,8 @@ void HipBackend::fstat_sweep(const F2Blocks& h, const FstatSweepRequest& req, Fs
         return;
     }
     STP_REQUIRE(total < (1ULL << 62), "fstat_sweep: too many combinations to enumerate");
-    const FstatTensor t = survivor_tensor(h);
+    std::vector<std::int32_t> host_bsz;
+    const FstatTensor t = survivor_tensor(h, &host_bsz);
     if (t.nbs == 0) return;
     std::int64_t K = (req.filter == 1 && req.top_k > 0) ? req.top_k : kDefaultTopK;
     K = std::min<std::int64_t>(K, static_cast<std::int64_t>(total));
@@ -227,51 +342,52 @@ void HipBackend::fstat_sweep(const F2Blocks& h, const FstatSweepRequest& req, Fs
         STP_HIP(hipMemcpyAsync(fs_idx_.get(), req.subset.data(), req.subset.size() * sizeof(std::int32_t),
                                hipMemcpyHostToDevice, stream_));
 
-    // Combinations per launch; STEPPE_FSTAT_CHUNK overrides it, as it does the reference's
-    // sweep chunk (cuda_backend_fstats_assemble.cu:128-131).
-    std::int64_t chunk = kDirectChunk;
-    if (const char* env = std::getenv("STEPPE_FSTAT_CHUNK")) {
-        const long long v = std::strtoll(env, nullptr, 10);
-        if (v > 0) chunk = v;
-    }
-    chunk = std::min<std::int64_t>(static_cast<std::int64_t>(total), chunk);
-    fs_hits_.ensure(static_cast<std::size_t>(chunk));
-    fs_count_.ensure(1);
     double tau = req.min_z;
-    std::vector<SweepHit> res, buf;
-    auto compact = [&] {
-        if (static_cast<std::int64_t>(res.size()) <= K) return;
-        std::nth_element(res.begin(), res.begin() + (K - 1), res.end(), hit_before);
-        const double kth = std::fabs(res[K - 1].z);
-        res.resize(K);
-        if (req.filter == 1 && kth > tau) tau = kth;
-    };
-    SweepArgs a;
-    a.t = t;
-    a.k = k;
-    a.range = range;
-    a.subset = req.subset.empty() ? nullptr : fs_idx_.get();
-    a.binom = fs_i64_.get();
-    a.hits = fs_hits_.get();
-    a.n_hits = fs_count_.get();
-    a.cap = chunk;
+    std::vector<SweepHit> res;
+    const bool gram = k == 4 && t.nbs >= 2 && (req.method == 2 || (req.method == 0 && range >= 24));
+    const std::int32_t* d_subset = req.subset.empty() ? nullptr : fs_idx_.get();
     mark_begin(kFstat);
-    for (std::int64_t r0 = 0; r0 < static_cast<std::int64_t>(total); r0 += chunk) {
-        a.r0 = r0;
-        a.count = std::min<std::int64_t>(chunk, static_cast<std::int64_t>(total) - r0);
-        a.tau = tau;
-        STP_HIP(hipMemsetAsync(fs_count_.get(), 0, sizeof(unsigned long long), stream_));
-        launch_sweep_direct(a, stream_);
-        unsigned long long nh = 0;
-        STP_HIP(hipMemcpyAsync(&nh, fs_count_.get(), sizeof(nh), hipMemcpyDeviceToHost, stream_));
-        STP_HIP(hipStreamSynchronize(stream_));
-        STP_REQUIRE(nh <= static_cast<unsigned long long>(a.count), "fstat_sweep: hit count exceeds the launch");
-        if (nh == 0) continue;
-        buf.resize(nh);
-        STP_HIP(hipMemcpyAsync(buf.data(), fs_hits_.get(), nh * sizeof(SweepHit), hipMemcpyDeviceToHost, stream_));
-        STP_HIP(hipStreamSynchronize(stream_));
-        res.insert(res.end(), buf.begin(), buf.end());
-        compact();
+    if (gram) {
+        f4_sweep_gram(t, host_bsz, range, d_subset, binom, K, req.filter, tau, res, out.rechecked);
+    } else {
+        // Combinations per launch; STEPPE_FSTAT_CHUNK overrides it, as it does the
+        // reference's sweep chunk (cuda_backend_fstats_assemble.cu:128-131).
+        std::int64_t chunk = kDirectChunk;
+        if (const char* env = std::getenv("STEPPE_FSTAT_CHUNK")) {
+            const long long v = std::strtoll(env, nullptr, 10);
+            if (v > 0) chunk = v;
+        }
+        chunk = std::min<std::int64_t>(static_cast<std::int64_t>(total), chunk);
+        fs_hits_.ensure(static_cast<std::size_t>(chunk));
+        fs_count_.ensure(1);
+        SweepArgs a;
+        a.t = t;
+        a.k = k;
+        a.range = range;
+        a.subset = d_subset;
+        a.binom = fs_i64_.get();
+        a.hits = fs_hits_.get();
+        a.n_hits = fs_count_.get();
+        a.cap = chunk;
+        std::vector<SweepHit> buf;
+        for (std::int64_t r0 = 0; r0 < static_cast<std::int64_t>(total); r0 += chunk) {
+            a.r0 = r0;
+            a.count = std::min<std::int64_t>(chunk, static_cast<std::int64_t>(total) - r0);
+            a.tau = tau;
+            STP_HIP(hipMemsetAsync(fs_count_.get(), 0, sizeof(unsigned long long), stream_));
+            launch_sweep_direct(a, stream_);
+            unsigned long long nh = 0;
+            STP_HIP(hipMemcpyAsync(&nh, fs_count_.get(), sizeof(nh), hipMemcpyDeviceToHost, stream_));
+            STP_HIP(hipStreamSynchronize(stream_));
+            STP_REQUIRE(nh <= static_cast<unsigned long long>(a.count), "fstat_sweep: hit count exceeds the launch");
+            if (nh == 0) continue;
+            buf.resize(nh);
+            STP_HIP(hipMemcpyAsync(buf.data(), fs_hits_.get(), nh * sizeof(SweepHit), hipMemcpyDeviceToHost,
+                                   stream_));
+            STP_HIP(hipStreamSynchronize(stream_));
+            res.insert(res.end(), buf.begin(), buf.end());
+            keep_top(res, K, req.filter, tau);
+        }
     }
     mark_end(kFstat);
     sync();
