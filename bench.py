@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2] [--precision split_bf16|fp64]
                     [--qpadm-models N] [--no-qpadm] [--qpgraph-graphs N] [--no-qpgraph]
-                    [--no-cpu-baseline]
+                    [--fstat-quartets N] [--no-fstats] [--no-cpu-baseline]
     python bench.py --calibrate-copy      # a plain 1 GiB device copy, for counter calibration
 
 One step = one pass of the hot path over the synthetic panel already resident in
@@ -34,7 +34,10 @@ resident c2 tensor, with its own single-threaded reference CPU rate on a sample 
 the same models.  ``qpgraph``: qpGraph fits/s for 5,000 random graphs (8-12 leaves, 1-4
 admixture events, each with its own simulated f2 over 300 blocks), 10 restarts each to
 convergence by batched L-BFGS on the device score and adjoint, with the reference CPU
-rate and the score agreement on a sample of the same graphs.
+rate and the score agreement on a sample of the same graphs.  ``fstats``: standalone
+f4 statistics/s for 1,000,000 random quartets on the resident c2 tensor, the
+all-combination f3 sweep (1.3M triples) and f4 sweep (64.7M quartets, Gram form on
+FP64 MFMA) at |z| >= 3, with the reference CPU run_f4 rate on a sample.
 """
 from __future__ import annotations
 
@@ -174,8 +177,13 @@ def random_models(rng, P: int, count: int):
     return models
 
 
-def qpadm_bench(ctx, n_models: int, cpu: bool):
-    """C4: n_models random models on the resident c2 tensor in one batched call."""
+_C2 = {}
+
+
+def c2_tensor(ctx):
+    """The resident c2 tensor (200 populations, FP64 f2 from the c2 panel), built once."""
+    if "h" in _C2:
+        return _C2["h"]
     import torch
     from steppe_amd import _core
     wp = workload_params("c2")
@@ -192,6 +200,15 @@ def qpadm_bench(ctx, n_models: int, cpu: bool):
     h = _core.F2Handle(raw, [f"p{i}" for i in range(P)], 0)
     del packed
     torch.cuda.empty_cache()
+    _C2["h"] = h
+    return h
+
+
+def qpadm_bench(ctx, n_models: int, cpu: bool):
+    """C4: n_models random models on the resident c2 tensor in one batched call."""
+    from steppe_amd import _core
+    h = c2_tensor(ctx)
+    P, nb = h.P, h.n_block
     models = random_models(np.random.default_rng(SEED + 4), P, n_models)
     _core.qpadm_models(h, models[:64], raw=True)          # warm-up (code load, buffers)
     ctx.timing(True)
@@ -230,6 +247,82 @@ def qpadm_bench(ctx, n_models: int, cpu: bool):
                                "sample": f"the first {n} of the same models, {cdt:.1f} s single-threaded "
                                          f"({os.path.basename(lib.path)})"}
         log(f"[bench] qpadm cpu baseline {out['cpu_baseline']}")
+    return out
+
+
+def fstats_bench(ctx, n_quartets: int, cpu: bool):
+    """Standalone f-statistics on the resident c2 tensor: run_f4 over n_quartets random
+    quartets, the all-combination f3 sweep and the all-combination f4 sweep (Gram form)
+    over its 200 populations at the default filter (|z| >= 3)."""
+    from math import comb
+    from steppe_amd import _core
+    h = c2_tensor(ctx)
+    P, nb = h.P, h.n_block
+    rng = np.random.default_rng(SEED + 6)
+    quartets = np.argsort(rng.random((n_quartets, P)), axis=1)[:, :4].astype(np.int32)
+    _core.run_f4(h, quartets[:1000])                       # warm-up
+    ctx.timing(True)
+    ctx.read_timing(reset=True)
+    t0 = time.perf_counter()
+    d = _core.run_f4(h, quartets)
+    t_f4 = time.perf_counter() - t0
+    tm = ctx.read_timing(reset=True)
+    f4_ms = tm["fstat"][0]
+    # f4 kernel: 4 f2 series of nb doubles per statistic (algorithmic bytes)
+    f4_bytes = 4.0 * nb * 8.0 * n_quartets
+    t0 = time.perf_counter()
+    s3 = _core.run_fstat_sweep(h, 3, None, min_z=3.0)
+    t_s3 = time.perf_counter() - t0
+    tm3 = ctx.read_timing(reset=True)
+    _core.run_fstat_sweep(h, 4, list(range(40)), min_z=3.0, method="gram")   # warm-up
+    ctx.read_timing(reset=True)
+    t0 = time.perf_counter()
+    s4 = _core.run_fstat_sweep(h, 4, None, min_z=3.0, method="gram")
+    t_s4 = time.perf_counter() - t0
+    tm4 = ctx.read_timing(reset=True)
+    ctx.timing(False)
+    # Gram work: per (c2, c3) the upper triangle of S (m = c2 rows) over nb blocks
+    gram_flop = float(sum(c2 * (c2 + 1) * nb * (P - 1 - c2) for c2 in range(2, P - 1)))
+    n4, n3 = comb(P, 4), comb(P, 3)
+    log(f"[bench] fstats: run_f4 {n_quartets} in {t_f4:.3f} s (kernel {f4_ms:.1f} ms); f3 sweep {n3} in "
+        f"{t_s3:.3f} s ({s3['survivors']} kept); f4 sweep {n4} in {t_s4:.3f} s (device {tm4['fstat'][0]:.1f} ms, "
+        f"{s4['survivors']} kept)")
+    out = {"metric": "f4 statistics/sec", "value": n_quartets / t_f4, "unit": "stats/s",
+           "workload": f"{n_quartets} random f4 quartets on the resident c2 tensor ({P} populations, {nb} "
+                       f"blocks, FP64): est / se / z / p by the block jackknife",
+           "seconds": t_f4, "kernel_ms": f4_ms,
+           "f4_kernel": {"algorithmic_bytes": f4_bytes, "GBs": f4_bytes / (f4_ms / 1e3) / 1e9 if f4_ms else None,
+                         "work": "4 f2 series x n_block x 8 B per statistic"},
+           "f3_sweep": {"combinations": n3, "seconds": t_s3, "device_ms": tm3["fstat"][0],
+                        "survivors": s3["survivors"], "value": n3 / t_s3, "unit": "triples/s"},
+           "f4_sweep": {"combinations": n4, "seconds": t_s4, "device_ms": tm4["fstat"][0],
+                        "survivors": s4["survivors"], "value": n4 / t_s4, "unit": "quartets/s",
+                        "method": "gram", "gram_TFLOPs": gram_flop / (tm4["fstat"][0] / 1e3) / 1e12
+                        if tm4["fstat"][0] else None,
+                        "gram_work": "sum over (c2, c3) of c2 (c2 + 1) n_block FLOP (upper triangle of S)"},
+           "cpu_baseline": None}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        lib = oracle_lib.ref()
+        kind = "reference"
+        if lib is None:
+            lib, kind = oracle_lib.cref(), "port"
+        f2h = h._f2_numpy()
+        vph = h._vpair_numpy()
+        bsz = np.asarray(h.block_sizes, np.int32)
+        n, chunk = 0, 2000
+        t0 = time.perf_counter()
+        while n < n_quartets and time.perf_counter() - t0 < 10.0:
+            o = lib.f4(f2h, vph, bsz, quartets[n:n + chunk])
+            m = np.abs(o["est"] - np.asarray(d["est"][n:n + chunk]))
+            assert np.all(m <= 1e-12 * (np.abs(o["est"]) + np.nan_to_num(o["se"]))), "f4 bench row disagrees"
+            n += chunk
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / cdt, "unit": "stats/s", "cores": 1, "kind": kind,
+                               "sample": f"the first {n} of the same quartets through run_f4, {cdt:.1f} s "
+                                         f"single-threaded ({os.path.basename(lib.path)})"}
+        log(f"[bench] fstats cpu baseline {out['cpu_baseline']}")
     return out
 
 
@@ -337,6 +430,8 @@ def main():
     ap.add_argument("--no-qpadm", action="store_true")
     ap.add_argument("--qpgraph-graphs", type=int, default=5_000)
     ap.add_argument("--no-qpgraph", action="store_true")
+    ap.add_argument("--fstat-quartets", type=int, default=1_000_000)
+    ap.add_argument("--no-fstats", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -429,11 +524,14 @@ def main():
         del packed, a2, nn, f2, vp
         torch.cuda.empty_cache()
         qp = qpadm_bench(ctx, args.qpadm_models, not args.no_cpu_baseline)
+    else:
+        del packed, a2, nn, f2, vp
+        torch.cuda.empty_cache()
+    fs = None
+    if not args.no_fstats:
+        fs = fstats_bench(ctx, args.fstat_quartets, not args.no_cpu_baseline)
     qg = None
     if not args.no_qpgraph:
-        if args.no_qpadm:
-            del packed, a2, nn, f2, vp
-            torch.cuda.empty_cache()
         qg = qpgraph_bench(ctx, args.qpgraph_graphs, not args.no_cpu_baseline)
 
     out = {
@@ -446,7 +544,7 @@ def main():
                                f"{nb} jackknife blocks, decode + f2 ({args.precision})",
                    "P": P, "M": M, "n_block": nb, "n_individuals": n_ind,
                    "precision": args.precision, "parallelism": "single GPU"},
-        "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg,
+        "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg, "fstats": fs,
     }
     print(json.dumps(out), flush=True)
 

@@ -554,6 +554,15 @@ def run_qpadm_search(f2: F2Handle, target, lefts, right, rank=-1, fudge=1e-4, al
 # standalone f-statistics (bindings/bind_fstats.cpp:29-120, bindings/steppe/__init__.py:865-1102)
 # ---------------------------------------------------------------------------------------
 def _tuples(f2: F2Handle, rows, width: int, what: str) -> np.ndarray:
+    """Rows of population names / indices / {"pop1": ..} dicts -> (n, width) int32 indices;
+    unknown names and out-of-range indices raise KeyError (bindings/bind_fstats.cpp)."""
+    if isinstance(rows, np.ndarray) and rows.dtype.kind in "iu":
+        idx = np.ascontiguousarray(rows.reshape(-1, width) if rows.size else rows.reshape(0, width), np.int32)
+        if idx.size and (idx.min() < 0 or idx.max() >= f2.P):
+            bad = int(np.argmax((idx < 0).any(axis=1) | (idx >= f2.P).any(axis=1)))
+            raise KeyError(f"{what}: index out of range in row {bad}")
+        return idx
+    lookup = {name: i for i, name in enumerate(f2.pops)}
     idx = np.zeros((len(rows), width), np.int32)
     for i, r in enumerate(rows):
         if isinstance(r, dict):
@@ -562,13 +571,22 @@ def _tuples(f2: F2Handle, rows, width: int, what: str) -> np.ndarray:
         if len(r) != width:
             raise ValueError(f"{what}: row {i} needs {width} populations, got {len(r)}")
         for j, name in enumerate(r):
-            idx[i, j] = _resolve(f2, name, what)
+            if isinstance(name, (int, np.integer)):
+                if not 0 <= int(name) < f2.P:
+                    raise KeyError(f"{what}: index {int(name)} out of range")
+                idx[i, j] = int(name)
+            else:
+                v = lookup.get(str(name))
+                if v is None:
+                    raise KeyError(f"{what}: unknown population {name!r}")
+                idx[i, j] = v
     return idx
 
 
 def _names(f2: F2Handle, idx: np.ndarray, width: int) -> dict:
-    pops = f2.pops
-    return {f"pop{j + 1}": [pops[int(v)] if pops else int(v) for v in idx[:, j]] for j in range(width)}
+    pops = np.asarray(f2.pops, dtype=object) if f2.pops else None
+    return {f"pop{j + 1}": (pops[idx[:, j]].tolist() if pops is not None else idx[:, j].tolist())
+            for j in range(width)}
 
 
 def _fstat(f2: F2Handle, rows, width: int, fn: str, what: str) -> dict:

@@ -363,12 +363,14 @@ __global__ __launch_bounds__(kThreads) void f2_means_kernel(FstatTensor t, doubl
 // v_mfma_f64_16x16x4f64 (A = rows of bi, B = rows of bj, both read from the same LDS
 // chunk).  The staging threads also accumulate the diagonal D(p) = sum_b e_b(p)^2.  The
 // next chunk's f2 values are loaded into registers before the current chunk's MFMAs.
-// Each f4 (c0 < c1) of a tile is then read out.  Its rank goes to the recheck list for the
-// direct kernel instead when the variance is formed with at least max_cancel-fold
-// cancellation, or the estimate is a difference of f2 means more than max_level times
-// larger than |est| + se (copied or nearly copied populations: there the reference's own
-// block values carry rounding noise at that level, which only its operation order
-// reproduces), or the variance is 0 / 0.
+// Each f4 (c0 < c1) of a tile is then read out.  Relative to the reference's operation
+// order the estimate differs by the rounding of the four f2 means it is formed from (a few
+// ulps of their magnitude, the level of the reference's own block-value rounding) and the
+// variance by the Gram sums' rounding amplified by the cancellation (D0 + D1) / den.  A
+// statistic goes to the recheck list for the direct kernel instead when that cancellation
+// reaches max_cancel, when the f2 means exceed max_level times |est| + se (copied or nearly
+// copied populations), or when the variance is 0 / 0; the others are then within 1e-9 of
+// the reference's se and 1e-9 se (plus the means' rounding) of its estimate.
 // ---------------------------------------------------------------------------------
 constexpr int kGramWaves = 8;
 constexpr int kGramThreads = 64 * kGramWaves;

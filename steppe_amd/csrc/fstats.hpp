@@ -78,8 +78,8 @@ struct GramArgs {
     const GramItem* items = nullptr;
     std::int64_t n_items = 0;
     double tau = 0.0;
-    double max_cancel = 1e3;         // (D0 + D1) / (D0 + D1 - 2 S01) above this: recomputed directly
-    double max_level = 1e3;          // the four f2 means above this times (|est| + se): recomputed
+    double max_cancel = 1e4;         // (D0 + D1) / (D0 + D1 - 2 S01) above this: recomputed directly
+    double max_level = 1e6;          // the four f2 means above this times (|est| + se): recomputed
     SweepHit* hits = nullptr;
     unsigned long long* n_hits = nullptr;
     std::int64_t* recheck = nullptr;
