@@ -114,6 +114,62 @@ int stp_compute_f2_blocks(stp_context* ctx, const double* Q, const double* V, co
 int stp_extract_f2(stp_context* ctx, const stp_tile* tile, const int32_t* block_id,
                    int32_t n_block, int32_t precision, stp_f2_handle** out);
 
+/* ---- genotype files ---------------------------------------------------------------- */
+
+/* Population selection of stp_extract_f2_files (reference: PopSelection,
+ * src/io/ind_reader.hpp:20-31): the populations are sorted by label (the f2 P axis). */
+#define STP_POPS_EXPLICIT 0  /* the n_pops labels in pops (every one must be in the .ind)   */
+#define STP_POPS_TOP_K 1     /* the top_k populations with most individuals (ties: first seen) */
+#define STP_POPS_MIN_N 2     /* every population with at least min_n individuals             */
+/* SNP filters and blocking (reference: FilterConfig, include/steppe/config.hpp:130-160, as
+ * bindings/bind_fstats.cpp:131-186 fills it): pooled minor allele frequency >= maf_min, at
+ * most a fraction maxmiss of populations without data (1 disables), autosomes 1..22 only,
+ * monomorphic dropped, transversions only; multiallelic or non-ACGT allele pairs are always
+ * dropped, strand-ambiguous pairs (A/T, C/G) when strand_mode is 0 (1 keep, 2 flip = keep).
+ * blgsize: jackknife block size in Morgans (0.05); the 2 Mb physical window is used when
+ * every genetic position is 0.  ploidy: 0 detect per sample, 1 pseudo-haploid, 2 diploid. */
+typedef struct stp_extract_request {
+    const char* geno;            /* packed TGENO or GENO (PACKEDANCESTRYMAP) .geno */
+    const char* snp;
+    const char* ind;
+    int32_t pop_mode;
+    const char* const* pops;
+    int32_t n_pops;
+    int64_t top_k;
+    int64_t min_n;
+    double maf_min;
+    double maxmiss;
+    int32_t autosomes_only;
+    int32_t drop_monomorphic;
+    int32_t transversions_only;
+    int32_t strand_mode;
+    double blgsize;
+    int32_t ploidy;
+    int32_t precision;           /* STP_PREC_* */
+} stp_extract_request;
+typedef struct stp_extract_info {
+    int64_t n_snp_total;         /* min(header SNPs, .snp rows) */
+    int64_t n_snp_kept;
+    int64_t n_pseudo_haploid;
+    int64_t n_diploid;
+    int32_t P;
+    int32_t n_block;
+} stp_extract_info;
+
+/* Genotype files -> resident f2 (reference: steppe::run_extract_f2,
+ * src/app/extract_f2_core.cpp:96-407, with the genotype front end of
+ * src/core/stats/genotype_front_end.cpp:9-37): reads the triple, builds the
+ * individual-major tile in HBM (GENO is transposed on the device), detects ploidy,
+ * decodes, keeps / compacts SNPs on the device, assigns blocks from the kept SNPs' .snp
+ * positions and contracts f2.  The handle carries the population labels
+ * (stp_f2_handle_label).  Unknown explicit populations, a filter that removes every SNP
+ * or unreadable files return STP_ERR_INVALID. */
+int stp_extract_f2_files(stp_context* ctx, const stp_extract_request* req, stp_f2_handle** out,
+                         stp_extract_info* info);
+/* Label of population i of a handle built from files ("" for other handles); writes at
+ * most len bytes including the terminating NUL; returns the label's length in *n. */
+int stp_f2_handle_label(const stp_f2_handle* h, int32_t i, char* buf, int32_t len, int32_t* n);
+
 /* ---- f2 handle (reference: DeviceF2Blocks, src/device/device_f2_blocks.hpp:18-53) ---- */
 
 /* Upload a host tensor (reference: upload_f2_blocks_to_device, device_f2_blocks.hpp:53).

@@ -17,6 +17,12 @@
 //   ref_f4               steppe::run_f4 (host tensor)       src/core/qpadm/f4.cpp:106
 //   ref_f3               steppe::run_f3 (host tensor)       src/core/qpadm/f3.cpp:105
 //   ref_f4ratio          steppe::run_f4ratio (host tensor)  src/core/qpadm/f4ratio.cpp:94
+//   ref_extract_*        the CPU branch of steppe::run_extract_f2 (src/app/extract_f2_core.cpp:
+//                        96-407, on_device == false) over the reference's own GenoReader,
+//                        read_ind, read_snp, ComputeBackend::transpose_to_canonical,
+//                        detect_sample_ploidy_device, decode_af, build_snp_keep_mask,
+//                        assign_blocks and compute_f2_blocks; the glue between those calls
+//                        follows that branch line by line
 
 #include <array>
 #include <cstdint>
@@ -27,6 +33,11 @@
 #include <vector>
 
 #include "core/domain/block_partition_rule.hpp"
+#include "io/filter/include_exclude.hpp"
+#include "io/filter/snp_filter.hpp"
+#include "io/geno_reader.hpp"
+#include "io/ind_reader.hpp"
+#include "io/snp_reader.hpp"
 #include "core/qpadm/qpgraph_model.hpp"
 #include "core/qpadm/qpgraph_objective.hpp"
 #include "core/internal/views.hpp"
@@ -338,5 +349,170 @@ int ref_f4ratio(const double* f2, const double* vpair, const std::int32_t* bsz, 
         return -1;
     }
 }
+
+namespace {
+struct RefExtract {
+    int P = 0, nb = 0;
+    long long total = 0, kept = 0, n_ph = 0, n_dip = 0;
+    std::vector<std::string> labels;
+    steppe::F2BlockTensor t;
+};
+}  // namespace
+
+void* ref_extract_open(const char* geno, const char* snp, const char* ind, int pop_mode, const char* const* pops,
+                       int n_pops, long long top_k, long long min_n, double maf_min, double maxmiss,
+                       int autosomes_only, int drop_monomorphic, int transversions_only, int strand_mode,
+                       double blgsize, int ploidy, char* err, int err_len) {
+    try {
+        namespace io = steppe::io;
+        namespace flt = steppe::io::filter;
+        io::GenoReader reader(geno);
+        io::PopSelection sel;
+        sel.mode = pop_mode == 0 ? io::PopSelection::Mode::Explicit
+                 : pop_mode == 1 ? io::PopSelection::Mode::AutoTopK : io::PopSelection::Mode::MinN;
+        for (int i = 0; i < n_pops; ++i) sel.labels.emplace_back(pops[i]);
+        sel.k = static_cast<std::size_t>(top_k);
+        sel.min_n = static_cast<std::size_t>(min_n);
+        const io::IndPartition part = io::read_ind(ind, sel, reader.records_present());
+        if (pop_mode == 0)
+            for (const auto& want : sel.labels) {
+                bool found = false;
+                for (const auto& g : part.groups) found = found || g.label == want;
+                if (!found) throw std::invalid_argument("unknown population " + want);
+            }
+        const io::SnpTable snptab = io::read_snp(snp, SIZE_MAX);
+        const std::size_t M0 = std::min(reader.header().n_snp, snptab.count);
+        auto be = steppe::device::make_cpu_backend();
+        io::GenotypeTile tile;
+        if (reader.header().format == io::GenoFormat::Tgeno) {
+            tile = reader.read_tile(part, 0, M0);
+        } else {
+            const io::SnpMajorTile src = reader.read_snp_major_tile(part, 0, M0);
+            steppe::SnpMajorTileView v;
+            v.snp_major = src.snp_major.data();
+            v.src_bytes_per_record = src.src_bytes_per_record;
+            v.n_snp = src.n_snp;
+            v.sel_rows = src.sel_rows.data();
+            v.n_individuals = src.n_individuals;
+            v.pop_offsets = src.pop_offsets.data();
+            v.n_pop = static_cast<int>(src.n_pop());
+            steppe::CanonicalTile c = be->transpose_to_canonical(v);
+            tile.packed = std::move(c.packed);
+            tile.bytes_per_record = c.bytes_per_record;
+            tile.n_snp = c.n_snp;
+            tile.n_individuals = c.n_individuals;
+            tile.pop_offsets = std::move(c.pop_offsets);
+            tile.pop_labels = src.pop_labels;
+        }
+        const int P = static_cast<int>(tile.n_pop());
+        const long M = static_cast<long>(tile.n_snp);
+        steppe::DecodeTileView view;
+        view.packed = tile.packed.data();
+        view.bytes_per_record = tile.bytes_per_record;
+        view.n_snp = tile.n_snp;
+        view.n_individuals = tile.n_individuals;
+        view.pop_offsets = tile.pop_offsets.data();
+        view.n_pop = P;
+        view.ploidy = 2;
+        std::vector<int> sample_ploidy;
+        if (ploidy == 0) {
+            view.detect_ploidy_on_device = true;
+            sample_ploidy = be->detect_sample_ploidy_device(view);
+        } else {
+            sample_ploidy.assign(tile.n_individuals, ploidy);
+        }
+        auto* out = new RefExtract;
+        for (int pl : sample_ploidy) (pl == 1 ? out->n_ph : out->n_dip)++;
+        view.sample_ploidy = sample_ploidy.data();
+        view.detect_ploidy_on_device = false;
+        const steppe::DecodeResult dec = be->decode_af(view);
+        std::vector<std::size_t> pop_individuals(static_cast<std::size_t>(P));
+        for (int p = 0; p < P; ++p) pop_individuals[p] = tile.pop_offsets[p + 1] - tile.pop_offsets[p];
+        flt::DecodedTileSummaryInput fin;
+        fin.q = dec.q.data();
+        fin.v = dec.v.data();
+        fin.n = dec.n.data();
+        fin.P = P;
+        fin.M = M;
+        fin.pop_individuals = pop_individuals;
+        fin.ploidy = 2;
+        steppe::FilterConfig filter;
+        filter.maf_min = maf_min;
+        filter.geno_max_missing = maxmiss;
+        filter.autosomes_only = autosomes_only != 0;
+        filter.drop_monomorphic = drop_monomorphic != 0;
+        filter.transversions_only = transversions_only != 0;
+        filter.strand_mode = strand_mode == 0 ? steppe::StrandMode::Drop
+                           : strand_mode == 1 ? steppe::StrandMode::Keep : steppe::StrandMode::Flip;
+        steppe::FilterConfig class_filter = filter;
+        class_filter.geno_max_missing = 1.0;
+        flt::SnpMembership mem(class_filter);
+        std::vector<bool> keep = flt::build_snp_keep_mask(fin, snptab, class_filter, mem);
+        if (maxmiss < 1.0)
+            for (long s = 0; s < M; ++s) {
+                if (!keep[s]) continue;
+                int n_missing = 0;
+                for (int p = 0; p < P; ++p)
+                    if (dec.n[static_cast<std::size_t>(p) + static_cast<std::size_t>(P) * s] <= 0.0) ++n_missing;
+                if (static_cast<double>(n_missing) / static_cast<double>(P) > maxmiss) keep[s] = false;
+            }
+        std::vector<double> Qk, Vk, Nk, gk, pk;
+        std::vector<int> ck;
+        for (long s = 0; s < M; ++s) {
+            if (!keep[s]) continue;
+            for (int i = 0; i < P; ++i) {
+                const std::size_t o = static_cast<std::size_t>(i) + static_cast<std::size_t>(P) * s;
+                Qk.push_back(dec.q[o]);
+                Vk.push_back(dec.v[o]);
+                Nk.push_back(dec.n[o]);
+            }
+            ck.push_back(snptab.chrom[s]);
+            gk.push_back(snptab.genpos_morgans[s]);
+            pk.push_back(snptab.physpos[s]);
+        }
+        const long Mk = static_cast<long>(ck.size());
+        if (Mk <= 0) throw std::invalid_argument("every SNP was filtered out");
+        const auto part_b = steppe::core::assign_blocks(ck, gk, blgsize, pk);
+        if (part_b.n_block <= 0) throw std::invalid_argument("no blocks");
+        out->t = be->compute_f2_blocks(steppe::core::MatView{Qk.data(), P, Mk}, steppe::core::MatView{Vk.data(), P, Mk},
+                                       steppe::core::MatView{Nk.data(), P, Mk}, part_b.block_id.data(),
+                                       part_b.n_block, steppe::Precision{});
+        out->P = P;
+        out->nb = part_b.n_block;
+        out->total = M;
+        out->kept = Mk;
+        out->labels = tile.pop_labels;
+        return out;
+    } catch (const std::exception& e) {
+        if (err && err_len > 0) {
+            std::strncpy(err, e.what(), static_cast<std::size_t>(err_len - 1));
+            err[err_len - 1] = 0;
+        }
+        return nullptr;
+    }
+}
+
+int ref_extract_dims(void* h, int* P, int* nb, long long* info) {
+    const auto* x = static_cast<RefExtract*>(h);
+    *P = x->P;
+    *nb = x->nb;
+    info[0] = x->total;
+    info[1] = x->kept;
+    info[2] = x->n_ph;
+    info[3] = x->n_dip;
+    return 0;
+}
+
+int ref_extract_fetch(void* h, double* f2, double* vpair, std::int32_t* bsz) {
+    const auto* x = static_cast<RefExtract*>(h);
+    std::memcpy(f2, x->t.f2.data(), x->t.f2.size() * sizeof(double));
+    std::memcpy(vpair, x->t.vpair.data(), x->t.vpair.size() * sizeof(double));
+    for (int b = 0; b < x->nb; ++b) bsz[b] = x->t.block_sizes[static_cast<std::size_t>(b)];
+    return 0;
+}
+
+const char* ref_extract_label(void* h, int i) { return static_cast<RefExtract*>(h)->labels[i].c_str(); }
+
+void ref_extract_free(void* h) { delete static_cast<RefExtract*>(h); }
 
 }  // extern "C"

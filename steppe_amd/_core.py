@@ -107,6 +107,20 @@ class QpAdmOptions(C.Structure):
                 ("p_se_threshold", C.c_double)]
 
 
+class ExtractRequest(C.Structure):
+    """stp_extract_request (include/steppe_hip.h)."""
+    _fields_ = [("geno", C.c_char_p), ("snp", C.c_char_p), ("ind", C.c_char_p), ("pop_mode", C.c_int32),
+                ("pops", C.POINTER(C.c_char_p)), ("n_pops", C.c_int32), ("top_k", C.c_int64), ("min_n", C.c_int64),
+                ("maf_min", C.c_double), ("maxmiss", C.c_double), ("autosomes_only", C.c_int32),
+                ("drop_monomorphic", C.c_int32), ("transversions_only", C.c_int32), ("strand_mode", C.c_int32),
+                ("blgsize", C.c_double), ("ploidy", C.c_int32), ("precision", C.c_int32)]
+
+
+class ExtractInfo(C.Structure):
+    _fields_ = [("n_snp_total", C.c_int64), ("n_snp_kept", C.c_int64), ("n_pseudo_haploid", C.c_int64),
+                ("n_diploid", C.c_int64), ("P", C.c_int32), ("n_block", C.c_int32)]
+
+
 class SweepRequest(C.Structure):
     """stp_sweep_request (include/steppe_hip.h); reference: steppe::SweepRequest."""
     _fields_ = [("k", C.c_int32), ("filter", C.c_int32), ("min_z", C.c_double), ("top_k", C.c_int64),
@@ -153,6 +167,8 @@ def lib():
         "stp_f2bin_read": (C.c_int, [C.c_char_p, vp, vp, vp]),
         "stp_qpadm_batch": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, C.POINTER(QpAdmOptions),
                                       C.POINTER(QpAdmResult)]),
+        "stp_extract_f2_files": (C.c_int, [vp, C.POINTER(ExtractRequest), C.POINTER(vp), C.POINTER(ExtractInfo)]),
+        "stp_f2_handle_label": (C.c_int, [vp, i32, C.c_char_p, i32, C.POINTER(i32)]),
         "stp_f4": (C.c_int, [vp, vp, vp, i64, vp, vp, vp, vp]),
         "stp_f3": (C.c_int, [vp, vp, vp, i64, vp, vp, vp, vp]),
         "stp_f4ratio": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
@@ -428,16 +444,97 @@ def parse_precision(precision) -> int:
     return _PRECISIONS[precision]
 
 
-def run_extract_f2(packed, n_snp: int, pop_offsets, block_id, n_block: int, pops=None,
-                   out: str = "", device: int = 0, ploidy="auto", precision=None,
-                   sample_ploidy=None):
-    """Genotype tile -> f2 (the compute core of bindings/bind_fstats.cpp:131-215).
+_PLOIDY = {"auto": 0, "1": 1, "pseudo": 1, "pseudo_haploid": 1, "2": 2, "diploid": 2}
+_STRAND = {"drop": 0, "keep": 1, "flip": 2}
 
-    The reference's entry reads GENO files by prefix; this one takes the decoded
-    front-end's product directly: a packed individual-major tile, population
-    offsets and the jackknife block partition.  ploidy: "auto" (detect), 1 or 2,
-    or "sample" with sample_ploidy.  Returns an F2Handle, or the directory path
-    when ``out`` is set."""
+
+def extract_f2_files(geno: str, snp: str, ind: str, pops=None, *, top_k: int = 0, min_n: int = 0,
+                     out: str = "", device: int = 0, blgsize: float = 0.05, maf: float = 0.0,
+                     maxmiss: float = 0.0, autosomes_only: bool = True, drop_monomorphic: bool = False,
+                     transversions_only: bool = False, ploidy="auto", strand_mode: str = "drop",
+                     precision=None):
+    """Packed genotype triple -> resident f2 (src/app/extract_f2_core.cpp:96-407): pops
+    (explicit labels), or the top_k largest populations, or every population with at least
+    min_n individuals.  Returns (F2Handle, info dict), or (out path, info) when ``out`` is
+    set (an STPF2BK1 directory with the reference's meta fields)."""
+    if pops:
+        mode, labels = 0, [str(x) for x in pops]
+    elif top_k > 0:
+        mode, labels = 1, []
+    elif min_n > 0:
+        mode, labels = 2, []
+    else:
+        raise ValueError("extract_f2: pops needs at least one population name")
+    pl = _PLOIDY.get(str(ploidy))
+    if pl is None:
+        raise ValueError(f"extract_f2: ploidy must be one of 'auto', '1'/'pseudo', '2'/'diploid' (got {ploidy!r})")
+    if strand_mode not in _STRAND:
+        raise ValueError(f"extract_f2: strand_mode must be one of 'drop', 'keep', 'flip' (got {strand_mode!r})")
+    for path in (geno, snp, ind):
+        if not os.path.exists(path):
+            raise ValueError(f"extract_f2: missing genotype file {path}")
+    prec = parse_precision(precision)
+    arr = (C.c_char_p * max(len(labels), 1))(*[x.encode() for x in labels])
+    req = ExtractRequest(str(geno).encode(), str(snp).encode(), str(ind).encode(), mode, arr, len(labels),
+                         int(top_k), int(min_n), float(maf), float(maxmiss), int(bool(autosomes_only)),
+                         int(bool(drop_monomorphic)), int(bool(transversions_only)), _STRAND[strand_mode],
+                         float(blgsize), pl, prec)
+    ctx = Context.get(device)
+    raw = C.c_void_p()
+    info = ExtractInfo()
+    check(lib().stp_extract_f2_files(ctx.h, C.byref(req), C.byref(raw), C.byref(info)), "extract_f2")
+    names = []
+    buf = C.create_string_buffer(4096)
+    n = C.c_int32(0)
+    for i in range(info.P):
+        check(lib().stp_f2_handle_label(raw, i, buf, 4096, C.byref(n)), "extract_f2")
+        names.append(buf.value.decode())
+    h = F2Handle(raw, names, device)
+    inf = {"n_snp_total": int(info.n_snp_total), "n_snp_kept": int(info.n_snp_kept),
+           "n_pseudo_haploid": int(info.n_pseudo_haploid), "n_diploid": int(info.n_diploid),
+           "P": int(info.P), "n_block": int(info.n_block)}
+    if out:
+        meta = {"precision_tag": PRECISION_NAMES[prec], "precision_mantissa_bits": 53 if prec == STP_PREC_FP64 else 48,
+                "blgsize_cm": blgsize * 100.0, "n_snp_total": inf["n_snp_total"], "n_snp_kept": inf["n_snp_kept"],
+                "filters": {"maf_min": maf, "maxmiss": maxmiss, "mind_max_missing": 1.0,
+                            "autosomes_only": bool(autosomes_only), "drop_monomorphic": bool(drop_monomorphic),
+                            "transversions_only": bool(transversions_only)},
+                "pop_selection": "explicit" if mode == 0 else ("top_k" if mode == 1 else "min_n"),
+                "source": {"geno": str(geno), "snp": str(snp), "ind": str(ind), "source_hash_computed": False,
+                           "geno_sha256": "", "snp_sha256": "", "ind_sha256": ""}}
+        write_f2_dir(out, h._f2_numpy(), h._vpair_numpy(), h.block_sizes, h.pops, meta)
+        return out, inf
+    return h, inf
+
+
+def run_extract_f2(packed, *args, **kw):
+    """Genotypes -> f2.  With a path prefix as the first argument this is the reference's
+    binding (bindings/bind_fstats.cpp:131-215): run_extract_f2(prefix, pops, out="",
+    device=0, blgsize=0.05, maf=0.0, maxmiss=0.0, autosomes_only=True,
+    drop_monomorphic=False, transversions_only=False, ploidy="auto", strand_mode="drop",
+    precision=None) reads <prefix>.{geno,snp,ind} and returns an F2Handle (or the out path).
+    Otherwise it takes an in-memory tile (run_extract_f2_tile)."""
+    if isinstance(packed, (str, os.PathLike)):
+        names = ("pops", "out", "device", "blgsize", "maf", "maxmiss", "autosomes_only", "drop_monomorphic",
+                 "transversions_only", "ploidy", "strand_mode", "precision")
+        kw.update(dict(zip(names, args)))
+        prefix = str(packed)
+        pops = kw.pop("pops", None)
+        if not pops:
+            raise ValueError("extract_f2: pops needs at least one population name")
+        h, _ = extract_f2_files(prefix + ".geno", prefix + ".snp", prefix + ".ind", pops,
+                                **{k: v for k, v in kw.items() if v is not None or k == "precision"})
+        return h
+    return run_extract_f2_tile(packed, *args, **kw)
+
+
+def run_extract_f2_tile(packed, n_snp: int, pop_offsets, block_id, n_block: int, pops=None,
+                        out: str = "", device: int = 0, ploidy="auto", precision=None,
+                        sample_ploidy=None):
+    """Genotype tile -> f2 (the compute core of bindings/bind_fstats.cpp:131-215): a packed
+    individual-major tile, population offsets and the jackknife block partition.  ploidy:
+    "auto" (detect), 1 or 2, or "sample" with sample_ploidy.  Returns an F2Handle, or the
+    directory path when ``out`` is set."""
     n_pop = len(pop_offsets) - 1
     pops = list(pops) if pops is not None else [f"pop{i}" for i in range(n_pop)]
     if sample_ploidy is not None:

@@ -196,6 +196,66 @@ int stp_extract_f2(stp_context* ctx, const stp_tile* tile, const int32_t* block_
     });
 }
 
+int stp_extract_f2_files(stp_context* ctx, const stp_extract_request* req, stp_f2_handle** out,
+                         stp_extract_info* info) {
+    return guard([&] {
+        STP_REQUIRE(req && out, "stp_extract_f2_files: null argument");
+        STP_REQUIRE(req->geno && req->snp && req->ind, "stp_extract_f2_files: the .geno/.snp/.ind paths are required");
+        stp::ExtractRequest r;
+        r.geno = req->geno;
+        r.snp = req->snp;
+        r.ind = req->ind;
+        STP_REQUIRE(req->pop_mode >= 0 && req->pop_mode <= 2, "stp_extract_f2_files: unknown pop_mode");
+        r.pops.mode = static_cast<stp::PopSelect::Mode>(req->pop_mode);
+        if (req->pop_mode == STP_POPS_EXPLICIT) {
+            STP_REQUIRE(req->n_pops > 0 && req->pops, "extract_f2: pops needs at least one population name");
+            for (int i = 0; i < req->n_pops; ++i) {
+                STP_REQUIRE(req->pops[i], "stp_extract_f2_files: null population name");
+                r.pops.labels.emplace_back(req->pops[i]);
+            }
+        }
+        STP_REQUIRE(req->top_k >= 0 && req->min_n >= 0, "stp_extract_f2_files: negative top_k / min_n");
+        r.pops.k = static_cast<std::size_t>(req->top_k);
+        r.pops.min_n = static_cast<std::size_t>(req->min_n);
+        r.maf_min = req->maf_min;
+        r.maxmiss = req->maxmiss;
+        r.autosomes_only = req->autosomes_only != 0;
+        r.drop_monomorphic = req->drop_monomorphic != 0;
+        r.transversions_only = req->transversions_only != 0;
+        r.strand_mode = req->strand_mode;
+        r.blgsize = req->blgsize;
+        r.ploidy = req->ploidy;
+        r.precision = req->precision;
+        stp::ExtractInfo inf;
+        auto h = std::make_unique<stp_f2_handle>();
+        h->ctx = ctx;
+        h->blocks.reset(backend(ctx).extract_f2_files(r, inf));
+        if (info) {
+            info->n_snp_total = inf.n_snp_total;
+            info->n_snp_kept = inf.n_snp_kept;
+            info->n_pseudo_haploid = inf.n_pseudo_haploid;
+            info->n_diploid = inf.n_diploid;
+            info->P = h->blocks->P;
+            info->n_block = h->blocks->n_block;
+        }
+        *out = h.release();
+    });
+}
+
+int stp_f2_handle_label(const stp_f2_handle* h, int32_t i, char* buf, int32_t len, int32_t* n) {
+    return guard([&] {
+        STP_REQUIRE(h && h->blocks, "stp_f2_handle_label: null handle");
+        STP_REQUIRE(i >= 0 && i < h->blocks->P, "stp_f2_handle_label: index out of range");
+        const std::string s = i < static_cast<int32_t>(h->blocks->labels.size()) ? h->blocks->labels[i] : std::string();
+        if (n) *n = static_cast<int32_t>(s.size());
+        if (buf && len > 0) {
+            const std::size_t k = std::min<std::size_t>(s.size(), static_cast<std::size_t>(len - 1));
+            std::memcpy(buf, s.data(), k);
+            buf[k] = '\0';
+        }
+    });
+}
+
 int stp_f2_handle_upload(stp_context* ctx, const double* f2, const double* vpair, const int32_t* block_sizes,
                          int32_t P, int32_t n_block, stp_f2_handle** out) {
     return guard([&] {

@@ -15,7 +15,9 @@
 
 #include "common.hpp"
 #include "f2_layout.hpp"
+#include "frontend.hpp"
 #include "fstats.hpp"
+#include "geno_io.hpp"
 #include "qpgraph.hpp"
 #include "steppe_hip.h"
 
@@ -80,6 +82,23 @@ public:
     int P = 0, n_block = 0;
     DevBuf<double> f2, vpair;
     std::vector<std::int32_t> block_sizes;
+    std::vector<std::string> labels;   // population labels when built from files
+};
+
+// Genotype files -> f2 (the reference's run_extract_f2 arguments,
+// src/app/extract_f2_core.cpp:96-104 and bindings/bind_fstats.cpp:131-186).
+struct ExtractRequest {
+    std::string geno, snp, ind;
+    PopSelect pops;
+    double maf_min = 0.0, maxmiss = 1.0;
+    bool autosomes_only = false, drop_monomorphic = false, transversions_only = false;
+    int strand_mode = 0;          // 0 drop strand-ambiguous SNPs, 1 keep, 2 flip (= keep)
+    double blgsize = 0.05;        // jackknife block size in Morgans
+    int ploidy = 0;               // 0 detect per sample, 1 pseudo-haploid, 2 diploid
+    int precision = 0;
+};
+struct ExtractInfo {
+    std::int64_t n_snp_total = 0, n_snp_kept = 0, n_pseudo_haploid = 0, n_diploid = 0;
 };
 
 // A resident set of qpGraph graphs: topology arena plus each graph's f3 basis (f_obs),
@@ -164,6 +183,9 @@ public:
                      const std::int32_t* block_id, int n_block, int precision, double* f2,
                      double* vpair, std::int32_t* block_sizes);
     F2Blocks* extract_f2(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision);
+    // Genotype files -> resident f2 (frontend_host.cpp): read the packed triple, build the
+    // individual-major tile in HBM, decode, keep / compact SNPs, block, contract.
+    F2Blocks* extract_f2_files(const ExtractRequest& req, ExtractInfo& info);
     F2Blocks* upload(const double* f2, const double* vpair, const std::int32_t* block_sizes, int P,
                      int n_block);
     // Batched qpAdm over models on a resident tensor; results to host (qpadm_host.cpp).
@@ -282,6 +304,12 @@ private:
     DevBuf<double> fs_mean_, fs_wroot_;
     DevBuf<GramItem> fs_items_;
     DevBuf<std::int64_t> fs_recheck_;
+    // Front-end working set: the canonical tile, a GENO staging block, the selection,
+    // SNP metadata, keep flags and kept columns, and the compacted counts.
+    DevBuf<std::uint8_t> fe_tile_, fe_stage_, fe_flags_;
+    DevBuf<std::int32_t> fe_i32_;
+    DevBuf<char> fe_alleles_;
+    DevBuf<std::uint16_t> fe_a2_, fe_n_;
     hipEvent_t ev_begin_[kStages] = {}, ev_end_[kStages] = {};
     bool pending_[kStages] = {};
     double total_ms_[kStages] = {};

@@ -1,0 +1,247 @@
+// steppe_amd/csrc/geno_io.cpp — host readers of the packed EIGENSTRAT triple (see geno_io.hpp).
+#include "geno_io.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <charconv>
+#include <cmath>
+#include <fstream>
+#include <limits>
+#include <map>
+#include <sstream>
+#include <unordered_map>
+#include <unordered_set>
+
+#include "common.hpp"
+
+namespace stp {
+
+namespace {
+
+constexpr std::size_t kHeaderBytes = 48;
+
+std::size_t packed_bytes(std::size_t n) { return (n + 3) / 4; }
+
+[[noreturn]] void io_fail(const std::string& msg) { throw ArgError(msg); }
+
+template <class T>
+bool parse_all(const std::string& tok, T& out) {
+    const char* b = tok.data();
+    const char* e = b + tok.size();
+    const auto [p, ec] = std::from_chars(b, e, out);
+    return ec == std::errc{} && p == e;
+}
+
+std::vector<std::string> words(const std::string& line) {
+    std::vector<std::string> t;
+    std::istringstream ls(line);
+    std::string w;
+    while (ls >> w) t.push_back(w);
+    return t;
+}
+
+}  // namespace
+
+GenoHeader parse_geno_header(const char* head, std::size_t len) {
+    GenoHeader h;
+    std::string text;
+    for (std::size_t i = 0; i < len && i < kHeaderBytes && head[i] != '\0'; ++i) text.push_back(head[i]);
+    const std::size_t b = text.find_first_not_of(" \t");
+    if (b == std::string::npos) return h;
+    const std::size_t e = text.find_first_of(" \t", b);
+    const std::string magic = text.substr(b, e == std::string::npos ? std::string::npos : e - b);
+    GenoFormat fmt;
+    if (magic == "TGENO") fmt = GenoFormat::Tgeno;
+    else if (magic == "GENO") fmt = GenoFormat::Geno;
+    else return h;
+    std::size_t counts[2] = {0, 0};
+    int got = 0;
+    std::size_t i = e == std::string::npos ? text.size() : e;
+    while (i < text.size() && got < 2) {
+        while (i < text.size() && !std::isdigit(static_cast<unsigned char>(text[i]))) ++i;
+        if (i >= text.size()) break;
+        std::size_t v = 0;
+        while (i < text.size() && std::isdigit(static_cast<unsigned char>(text[i]))) {
+            const std::size_t d = static_cast<std::size_t>(text[i] - '0');
+            if (v > (std::numeric_limits<std::size_t>::max() - d) / 10) return h;   // overflow
+            v = v * 10 + d;
+            ++i;
+        }
+        counts[got++] = v;
+    }
+    if (got < 2) return h;
+    h.format = fmt;
+    h.n_ind = counts[0];
+    h.n_snp = counts[1];
+    if (fmt == GenoFormat::Tgeno) {
+        h.header_bytes = kHeaderBytes;
+        h.n_records = h.n_ind;
+        h.bytes_per_record = packed_bytes(h.n_snp);
+    } else {
+        h.bytes_per_record = std::max(kHeaderBytes, packed_bytes(h.n_ind));
+        h.header_bytes = h.bytes_per_record;
+        h.n_records = h.n_snp;
+    }
+    return h;
+}
+
+GenoFile open_geno(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) io_fail("geno: cannot open " + path);
+    char head[kHeaderBytes] = {};
+    in.read(head, kHeaderBytes);
+    GenoFile f;
+    f.path = path;
+    if (in.gcount() == static_cast<std::streamsize>(kHeaderBytes)) f.header = parse_geno_header(head, kHeaderBytes);
+    if (f.header.format == GenoFormat::Unknown)
+        io_fail("geno: not a packed TGENO / GENO file (48-byte header with magic TGENO or GENO): " + path);
+    if (f.header.bytes_per_record == 0 || f.header.n_records == 0)
+        io_fail("geno: degenerate header (zero records or stride): " + path);
+    in.seekg(0, std::ios::end);
+    const std::streamoff size = in.tellg();
+    if (size < static_cast<std::streamoff>(f.header.header_bytes)) io_fail("geno: file smaller than its header: " + path);
+    f.records_present = (static_cast<std::size_t>(size) - f.header.header_bytes) / f.header.bytes_per_record;
+    if (f.records_present == 0) io_fail("geno: no complete record on disk: " + path);
+    f.records_present = std::min(f.records_present, f.header.n_records);
+    return f;
+}
+
+std::vector<PopGroup> read_ind(const std::string& path, const PopSelect& sel, std::size_t n_records_present,
+                               std::size_t* n_individuals_total) {
+    std::ifstream in(path);
+    if (!in) io_fail("ind: cannot open " + path);
+    struct Raw {
+        std::string label;
+        std::size_t first;
+        std::vector<std::size_t> rows;
+    };
+    std::vector<Raw> groups;
+    std::unordered_map<std::string, std::size_t> at;
+    std::size_t row = 0;
+    std::string line;
+    while (std::getline(in, line)) {
+        const auto t = words(line);
+        if (t.size() < 3) continue;
+        if (row >= n_records_present) continue;
+        const auto it = at.find(t[2]);
+        if (it == at.end()) {
+            at.emplace(t[2], groups.size());
+            groups.push_back(Raw{t[2], groups.size(), {row}});
+        } else {
+            groups[it->second].rows.push_back(row);
+        }
+        ++row;
+    }
+    if (n_individuals_total) *n_individuals_total = row;
+    if (groups.empty()) io_fail("ind: no individuals parsed from " + path);
+    std::vector<const Raw*> chosen;
+    if (sel.mode == PopSelect::kExplicit) {
+        const std::unordered_set<std::string> want(sel.labels.begin(), sel.labels.end());
+        for (const auto& g : groups)
+            if (want.count(g.label)) chosen.push_back(&g);
+    } else if (sel.mode == PopSelect::kTopK) {
+        for (const auto& g : groups) chosen.push_back(&g);
+        std::stable_sort(chosen.begin(), chosen.end(), [](const Raw* a, const Raw* b) {
+            if (a->rows.size() != b->rows.size()) return a->rows.size() > b->rows.size();
+            return a->first < b->first;
+        });
+        chosen.resize(std::min(sel.k, chosen.size()));
+    } else {
+        for (const auto& g : groups)
+            if (g.rows.size() >= sel.min_n) chosen.push_back(&g);
+    }
+    if (chosen.empty()) io_fail("ind: population selection is empty for " + path);
+    std::sort(chosen.begin(), chosen.end(), [](const Raw* a, const Raw* b) { return a->label < b->label; });
+    std::vector<PopGroup> out;
+    out.reserve(chosen.size());
+    for (const Raw* g : chosen) out.push_back(PopGroup{g->label, g->rows});
+    return out;
+}
+
+SnpTable read_snp(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) io_fail("snp: cannot open " + path);
+    SnpTable t;
+    std::map<std::string, int> other;
+    int next_other = -1;
+    std::string line;
+    std::size_t line_no = 0;
+    while (std::getline(in, line)) {
+        ++line_no;
+        const auto f = words(line);
+        if (f.empty()) {
+            if (in.peek() == std::char_traits<char>::eof()) break;
+            io_fail("snp: blank line at line " + std::to_string(line_no) + " of " + path);
+        }
+        if (f.size() < 3)
+            io_fail("snp: expected <id> <chrom> <genpos> at line " + std::to_string(line_no) + " of " + path);
+        int chrom = 0;
+        bool numeric = !f[1].empty() && std::all_of(f[1].begin(), f[1].end(),
+                                                     [](char c) { return std::isdigit(static_cast<unsigned char>(c)); });
+        if (!(numeric && parse_all(f[1], chrom))) {
+            if (f[1] == "X" || f[1] == "x") chrom = 23;
+            else if (f[1] == "Y" || f[1] == "y") chrom = 24;
+            else if (f[1] == "MT" || f[1] == "mt" || f[1] == "M") chrom = 90;
+            else {
+                const auto it = other.find(f[1]);
+                if (it != other.end()) chrom = it->second;
+                else {
+                    chrom = next_other--;
+                    other.emplace(f[1], chrom);
+                }
+            }
+        }
+        double g = 0.0;
+        if (!parse_all(f[2], g) || !std::isfinite(g))
+            io_fail("snp: malformed genetic position \"" + f[2] + "\" at line " + std::to_string(line_no));
+        double ph = 0.0;
+        if (f.size() > 3 && !(parse_all(f[3], ph) && std::isfinite(ph))) ph = 0.0;
+        const bool alleles = f.size() >= 6;
+        t.id.push_back(f[0]);
+        t.chrom.push_back(chrom);
+        t.genpos.push_back(g);
+        t.physpos.push_back(ph);
+        t.ref.push_back(alleles && !f[4].empty() ? f[4][0] : 'N');
+        t.alt.push_back(alleles && !f[5].empty() ? f[5][0] : 'N');
+    }
+    return t;
+}
+
+void read_tgeno_selected(const GenoFile& f, const std::vector<PopGroup>& groups, std::size_t n_snp,
+                         std::vector<std::uint8_t>& out) {
+    STP_REQUIRE(f.header.format == GenoFormat::Tgeno, "geno: not a TGENO file");
+    STP_REQUIRE(n_snp >= 1 && n_snp <= f.header.n_snp, "geno: SNP range out of bounds");
+    STP_REQUIRE(!groups.empty(), "geno: empty population selection");
+    const std::size_t bpr = packed_bytes(n_snp);
+    std::size_t n = 0;
+    for (const auto& g : groups)
+        for (std::size_t r : g.rows) {
+            STP_REQUIRE(r < f.records_present, "geno: individual row " + std::to_string(r) + " beyond the records on disk");
+            ++n;
+        }
+    out.assign(n * bpr, 0);
+    std::ifstream in(f.path, std::ios::binary);
+    if (!in) io_fail("geno: cannot reopen " + f.path);
+    std::size_t k = 0;
+    for (const auto& g : groups)
+        for (std::size_t r : g.rows) {
+            in.seekg(static_cast<std::streamoff>(f.header.header_bytes + r * f.header.bytes_per_record));
+            in.read(reinterpret_cast<char*>(out.data() + k * bpr), static_cast<std::streamsize>(bpr));
+            if (in.gcount() != static_cast<std::streamsize>(bpr)) io_fail("geno: short read in " + f.path);
+            ++k;
+        }
+}
+
+void read_geno_records(const GenoFile& f, std::size_t s0, std::size_t s1, std::vector<std::uint8_t>& out) {
+    STP_REQUIRE(f.header.format == GenoFormat::Geno, "geno: not a GENO (SNP-major) file");
+    STP_REQUIRE(s0 < s1 && s1 <= f.records_present, "geno: SNP records beyond the file");
+    const std::size_t bpr = f.header.bytes_per_record;
+    out.resize((s1 - s0) * bpr);
+    std::ifstream in(f.path, std::ios::binary);
+    if (!in) io_fail("geno: cannot reopen " + f.path);
+    in.seekg(static_cast<std::streamoff>(f.header.header_bytes + s0 * bpr));
+    in.read(reinterpret_cast<char*>(out.data()), static_cast<std::streamsize>(out.size()));
+    if (in.gcount() != static_cast<std::streamsize>(out.size())) io_fail("geno: short read in " + f.path);
+}
+
+}  // namespace stp

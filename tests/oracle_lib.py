@@ -147,6 +147,51 @@ class _Lib:
         return out.to_dict()
 
 
+    # ---- genotype files -> f2 ------------------------------------------------------
+    def extract(self, geno, snp, ind, pops=None, top_k=0, min_n=0, maf=0.0, maxmiss=0.0, autosomes_only=True,
+                drop_monomorphic=False, transversions_only=False, strand_mode=0, blgsize=0.05, ploidy=0):
+        """The extract over a genotype triple; returns a dict (f2, vpair as (P, P, nb),
+        block_sizes, labels, info) or raises ValueError with the library's message."""
+        mode = 0 if pops else (1 if top_k > 0 else 2)
+        labels = [str(x).encode() for x in (pops or [])]
+        arr = (C.c_char_p * max(len(labels), 1))(*labels)
+        err = C.create_string_buffer(512)
+        f = self.fn("extract_open")
+        f.restype = C.c_void_p
+        f.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_void_p, C.c_int, C.c_longlong, C.c_longlong,
+                      C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_char_p,
+                      C.c_int]
+        h = f(str(geno).encode(), str(snp).encode(), str(ind).encode(), mode, C.cast(arr, C.c_void_p), len(labels),
+              top_k, min_n, maf, maxmiss, int(autosomes_only), int(drop_monomorphic), int(transversions_only),
+              strand_mode, blgsize, ploidy, err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        try:
+            P, nb = C.c_int(), C.c_int()
+            info = (C.c_longlong * 4)()
+            d = self.fn("extract_dims")
+            d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+            d(h, C.byref(P), C.byref(nb), info)
+            P, nb = P.value, nb.value
+            f2 = np.zeros(P * P * nb)
+            vp = np.zeros_like(f2)
+            bs = np.zeros(nb, np.int32)
+            g = self.fn("extract_fetch")
+            g.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+            g(h, f2.ctypes.data_as(C.c_void_p), vp.ctypes.data_as(C.c_void_p), bs.ctypes.data_as(C.c_void_p))
+            lab = self.fn("extract_label")
+            lab.restype = C.c_char_p
+            lab.argtypes = [C.c_void_p, C.c_int]
+            labels = [lab(h, i).decode() for i in range(P)]
+        finally:
+            fr = self.fn("extract_free")
+            fr.argtypes = [C.c_void_p]
+            fr(h)
+        return {"f2": f2.reshape((P, P, nb), order="F"), "vpair": vp.reshape((P, P, nb), order="F"),
+                "block_sizes": bs, "labels": labels,
+                "info": {"n_snp_total": info[0], "n_snp_kept": info[1], "n_pseudo_haploid": info[2],
+                         "n_diploid": info[3]}}
+
     # ---- f-statistics -------------------------------------------------------------
     @staticmethod
     def _tensor_args(f2, vpair, block_sizes):

@@ -1,0 +1,101 @@
+"""Pin the restated genotype-file extract (oracle/cpu_ref_frontend.cpp).
+
+* bit-for-bit against the reference's own chain (oracle/_ref: GenoReader, read_ind,
+  read_snp, transpose_to_canonical, detect_sample_ploidy_device, decode_af,
+  build_snp_keep_mask, assign_blocks, compute_f2_blocks, glued as the CPU branch of
+  run_extract_f2) on synthetic TGENO and GENO triples, for explicit / top-K / min-N
+  selections, every filter, explicit ploidy, an all-zero genetic map (the 2 Mb physical
+  fallback) and a file shorter than its header;
+* the TGENO and GENO encodings of one panel give bit-identical tensors (the transpose);
+* the header geometry and the error cases the reference raises on.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from genofiles import header, synthetic_triple
+
+CASES = [
+    dict(pops=["Pop_A", "Pop_C", "Pop_F", "Pop_H"]),
+    dict(top_k=5, maxmiss=1.0, autosomes_only=False),
+    dict(min_n=6, maf=0.05, drop_monomorphic=True, transversions_only=True, strand_mode=1),
+    dict(pops=["Pop_B", "Pop_I", "Pop_E"], maxmiss=0.34, strand_mode=2, ploidy=2, blgsize=0.02),
+    dict(pops=["Pop_D", "Pop_G"], ploidy=1, maxmiss=1.0),
+]
+
+
+def _paths(pre):
+    return pre + ".geno", pre + ".snp", pre + ".ind"
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.int64)
+
+
+def _same(a, b):
+    assert a["labels"] == b["labels"] and a["info"] == b["info"]
+    assert np.array_equal(a["block_sizes"], b["block_sizes"])
+    assert np.array_equal(_bits(a["f2"]), _bits(b["f2"])) and np.array_equal(_bits(a["vpair"]), _bits(b["vpair"]))
+
+
+@pytest.mark.parametrize("fmt", ["tgeno", "geno"])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_oracle_bit_identical_to_reference(cref, ref, tmp_path, fmt, case):
+    pre = str(tmp_path / f"p_{fmt}")
+    synthetic_triple(pre, 11, n_snp=2400, fmt=fmt)
+    a = cref.extract(*_paths(pre), **CASES[case])
+    b = ref.extract(*_paths(pre), **CASES[case])
+    _same(a, b)
+    assert a["info"]["n_snp_kept"] > 100
+
+
+def test_zero_map_and_short_file(cref, ref, tmp_path):
+    pre = str(tmp_path / "z")
+    synthetic_triple(pre, 12, n_snp=1600, fmt="geno", zero_map=True)
+    a = cref.extract(*_paths(pre), pops=["Pop_A", "Pop_E"], maxmiss=1.0)
+    b = ref.extract(*_paths(pre), pops=["Pop_A", "Pop_E"], maxmiss=1.0)
+    _same(a, b)
+    assert a["info"]["n_snp_total"] == 1600 and len(a["block_sizes"]) > 20
+    # a GENO file with fewer SNP records than its header and .snp is an error in both
+    synthetic_triple(pre, 12, n_snp=1600, fmt="geno", records_missing=37)
+    for lib in (cref, ref):
+        with pytest.raises(ValueError):
+            lib.extract(*_paths(pre), pops=["Pop_A", "Pop_E"], maxmiss=1.0)
+    pre = str(tmp_path / "s")
+    synthetic_triple(pre, 13, n_snp=1200, fmt="tgeno", sizes=[3] * 9, records_missing=4)
+    a = cref.extract(*_paths(pre), top_k=9, maxmiss=1.0)
+    b = ref.extract(*_paths(pre), top_k=9, maxmiss=1.0)
+    _same(a, b)
+    assert a["info"]["n_pseudo_haploid"] + a["info"]["n_diploid"] == 27 - 4
+
+
+def test_transpose_invariance(cref, tmp_path):
+    t, g = str(tmp_path / "t"), str(tmp_path / "g")
+    synthetic_triple(t, 21, n_snp=1800, fmt="tgeno")
+    synthetic_triple(g, 21, n_snp=1800, fmt="geno")
+    for kw in CASES[:3]:
+        _same(cref.extract(*_paths(t), **kw), cref.extract(*_paths(g), **kw))
+
+
+def test_header_geometry(tmp_path):
+    """GENO records are max(48, ceil(n_ind/4)) bytes with the header in the first record."""
+    from genofiles import write_triple
+    for n_ind, want in ((10, 48), (300, 75)):
+        codes = np.zeros((n_ind, 5), np.uint8)
+        snp = {"id": list("abcde"), "chrom": ["1"] * 5, "genpos": [0.0, 0.1, 0.2, 0.3, 0.4],
+               "physpos": [1.0] * 5, "ref": ["A"] * 5, "alt": ["C"] * 5}
+        pre = str(tmp_path / f"h{n_ind}")
+        write_triple(pre, codes, ["P"] * n_ind, snp, "geno")
+        assert os.path.getsize(pre + ".geno") == want * 6
+    assert header("TGENO", 3, 7)[:11] == b"TGENO 3 7 1" and len(header("TGENO", 3, 7)) == 48
+
+
+def test_errors(cref, ref, tmp_path):
+    pre = str(tmp_path / "e")
+    synthetic_triple(pre, 31, n_snp=800)
+    for lib in (cref, ref):
+        with pytest.raises(ValueError):
+            lib.extract(*_paths(pre), pops=["Pop_A", "Nowhere"])
+        with pytest.raises(ValueError):
+            lib.extract(*_paths(pre), pops=["Pop_A"], maf=0.6)        # folded MAF <= 0.5: nothing kept
