@@ -2,7 +2,8 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2] [--precision split_bf16|fp64]
                     [--qpadm-models N] [--no-qpadm] [--qpgraph-graphs N] [--no-qpgraph]
-                    [--fstat-quartets N] [--no-fstats] [--no-cpu-baseline]
+                    [--fstat-quartets N] [--no-fstats] [--frontend-snps N] [--no-frontend]
+                    [--no-cpu-baseline]
     python bench.py --calibrate-copy      # a plain 1 GiB device copy, for counter calibration
 
 One step = one pass of the hot path over the synthetic panel already resident in
@@ -37,7 +38,9 @@ convergence by batched L-BFGS on the device score and adjoint, with the referenc
 rate and the score agreement on a sample of the same graphs.  ``fstats``: standalone
 f4 statistics/s for 1,000,000 random quartets on the resident c2 tensor, the
 all-combination f3 sweep (1.3M triples) and f4 sweep (64.7M quartets, Gram form on
-FP64 MFMA) at |z| >= 3, with the reference CPU run_f4 rate on a sample.
+FP64 MFMA) at |z| >= 3, with the reference CPU run_f4 rate on a sample.  ``frontend``:
+genotype files -> resident f2 for GENO and TGENO triples of the c2 shape (random codes,
+1.2M SNPs), with the reference's own reader / filter / f2 chain on a 40k-SNP sample.
 """
 from __future__ import annotations
 
@@ -326,6 +329,95 @@ def fstats_bench(ctx, n_quartets: int, cpu: bool):
     return out
 
 
+def write_random_triple(prefix: str, fmt: str, pop_of, n_snp: int, seed: int):
+    """A packed triple of uniformly random 2-bit codes (1/4 missing) with random ACGT allele
+    columns and 22 autosomes of 0.05-Morgan-spaced SNPs: input for timing the file path."""
+    rng = np.random.default_rng(seed)
+    n_ind = len(pop_of)
+    if fmt == "tgeno":
+        bpr = (n_snp + 3) // 4
+        head = f"TGENO {n_ind} {n_snp} 0 0".encode().ljust(48, b"\0")
+        n_rec = n_ind
+    else:
+        bpr = max(48, (n_ind + 3) // 4)
+        head = f"GENO {n_ind} {n_snp} 0 0".encode().ljust(bpr, b"\0")
+        n_rec = n_snp
+    with open(prefix + ".geno", "wb") as f:
+        f.write(head)
+        step = max(1, (64 << 20) // bpr)
+        for r0 in range(0, n_rec, step):
+            f.write(rng.integers(0, 256, (min(step, n_rec - r0), bpr), dtype=np.uint8).tobytes())
+    with open(prefix + ".ind", "w") as f:
+        f.write("".join(f"I{g} U p{pop_of[g]:03d}\n" for g in range(n_ind)))
+    chrom = np.minimum(np.arange(n_snp) * 22 // n_snp + 1, 22)
+    gpos = np.arange(n_snp) * (0.05 * 22 * 30 / n_snp)
+    alle = np.array(list("ACGT"))
+    ra, aa = alle[rng.integers(0, 4, n_snp)], alle[rng.integers(0, 4, n_snp)]
+    with open(prefix + ".snp", "w") as f:
+        f.write("".join(f"s{k} {chrom[k]} {gpos[k]:.6f} {k * 100} {ra[k]} {aa[k]}\n" for k in range(n_snp)))
+
+
+def frontend_bench(ctx, n_snp: int, cpu: bool):
+    """Genotype files -> resident f2: GENO (SNP-major, transposed on the device) and TGENO
+    triples of the c2 shape (200 populations, its individuals, n_snp SNPs), explicit
+    selection of every population, autosomes only, strand-ambiguous and non-ACGT pairs
+    dropped, no population-missingness cap (random codes leave some single-sample
+    population without data at almost every SNP), FP64 f2."""
+    import tempfile
+    from steppe_amd import _core
+    wp = workload_params("c2")
+    sizes = wp["sizes"]
+    pop_of = np.repeat(np.arange(len(sizes)), sizes)
+    pops = [f"p{i:03d}" for i in range(len(sizes))]
+    out = {"metric": "genotype-file extract SNPs/sec", "unit": "SNPs/s", "cpu_baseline": None}
+    with tempfile.TemporaryDirectory() as d:
+        for fmt in ("geno", "tgeno"):
+            pre = os.path.join(d, fmt)
+            t0 = time.perf_counter()
+            write_random_triple(pre, fmt, pop_of, n_snp, SEED + 7)
+            t_write = time.perf_counter() - t0
+            _core.extract_f2_files(pre + ".geno", pre + ".snp", pre + ".ind", pops[:3], maxmiss=1.0,
+                                   precision="fp64")                      # warm-up (code load)
+            ctx.timing(True)
+            ctx.read_timing(reset=True)
+            t0 = time.perf_counter()
+            h, info = _core.extract_f2_files(pre + ".geno", pre + ".snp", pre + ".ind", pops, maxmiss=1.0,
+                                             precision="fp64")
+            dt = time.perf_counter() - t0
+            tm = ctx.read_timing(reset=True)
+            ctx.timing(False)
+            geno_bytes = os.path.getsize(pre + ".geno")
+            log(f"[bench] frontend {fmt}: {n_snp} SNPs x {len(pop_of)} individuals ({geno_bytes / 1e9:.2f} GB) in "
+                f"{dt:.3f} s (files written in {t_write:.1f} s); kept {info['n_snp_kept']}; stages {tm}")
+            out[fmt] = {"seconds": dt, "value": n_snp / dt, "geno_bytes": geno_bytes, "n_snp_kept": info["n_snp_kept"],
+                        "n_block": info["n_block"], "frontend_ms": tm["frontend"][0], "decode_ms": tm["decode"][0],
+                        "f2_ms": tm["f2"][0] + tm["prep"][0] + tm["count"][0]}
+            del h
+        if cpu:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_lib
+            lib = oracle_lib.ref()
+            kind = "reference"
+            if lib is None:
+                lib, kind = oracle_lib.cref(), "port"
+            m = 40_000
+            pre = os.path.join(d, "sample")
+            write_random_triple(pre, "geno", pop_of, m, SEED + 8)
+            t0 = time.perf_counter()
+            lib.extract(pre + ".geno", pre + ".snp", pre + ".ind", pops=pops, maxmiss=1.0, autosomes_only=True)
+            cdt = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": m / cdt, "unit": "SNPs/s", "cores": 1, "kind": kind,
+                                   "sample": f"a GENO triple of the same shape with {m} SNPs, {cdt:.1f} s single-"
+                                             f"threaded ({os.path.basename(lib.path)})"}
+            log(f"[bench] frontend cpu baseline {out['cpu_baseline']}")
+    out["value"] = out["geno"]["value"]
+    out["workload"] = (f"GENO and TGENO triples of random codes, {len(sizes)} populations x {len(pop_of)} "
+                       f"individuals x {n_snp} SNPs, all populations selected, autosomes, strand-ambiguous / non-ACGT "
+                       f"pairs dropped, maxmiss 1, FP64 f2; "
+                       f"value = SNPs/s of the GENO file (read, transpose, ploidy, decode, keep, compact, f2)")
+    return out
+
+
 def random_graphs(rng, count: int):
     """count random admixture graphs, 8-12 leaves and 1-4 admixture events each, with
     their simulated f2 over 300 blocks."""
@@ -432,6 +524,8 @@ def main():
     ap.add_argument("--no-qpgraph", action="store_true")
     ap.add_argument("--fstat-quartets", type=int, default=1_000_000)
     ap.add_argument("--no-fstats", action="store_true")
+    ap.add_argument("--frontend-snps", type=int, default=1_200_000)
+    ap.add_argument("--no-frontend", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -530,6 +624,9 @@ def main():
     fs = None
     if not args.no_fstats:
         fs = fstats_bench(ctx, args.fstat_quartets, not args.no_cpu_baseline)
+    fe = None
+    if not args.no_frontend:
+        fe = frontend_bench(ctx, args.frontend_snps, not args.no_cpu_baseline)
     qg = None
     if not args.no_qpgraph:
         qg = qpgraph_bench(ctx, args.qpgraph_graphs, not args.no_cpu_baseline)
@@ -544,7 +641,7 @@ def main():
                                f"{nb} jackknife blocks, decode + f2 ({args.precision})",
                    "P": P, "M": M, "n_block": nb, "n_individuals": n_ind,
                    "precision": args.precision, "parallelism": "single GPU"},
-        "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg, "fstats": fs,
+        "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg, "fstats": fs, "frontend": fe,
     }
     print(json.dumps(out), flush=True)
 

@@ -381,10 +381,11 @@ int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_s
  * around the decode, prep and f2 contraction launches. */
 int stpx_timing(stp_context* ctx, int32_t enable);
 /* Accumulated milliseconds and launch counts per stage [decode, prep, f2, count, qpadm,
- * qpgraph, fstat] (f2 = the contraction kernel, count = the pair-count kernel, qpadm = the
- * batched fit kernels of one stp_qpadm_batch call, qpgraph = one stp_qpgraph_eval or
- * stp_qpgraph_fit launch, fstat = the kernels of one f4 / f3 / f4-ratio / sweep call);
- * reset clears them. */
+ * qpgraph, fstat, frontend] (f2 = the contraction kernel, count = the pair-count kernel,
+ * qpadm = the batched fit kernels of one stp_qpadm_batch call, qpgraph = one
+ * stp_qpgraph_eval or stp_qpgraph_fit launch, fstat = the kernels of one f4 / f3 /
+ * f4-ratio / sweep call, frontend = each GENO transpose block, the keep mask and the
+ * compaction of stp_extract_f2_files); reset clears them. */
 int stpx_timing_read(stp_context* ctx, double* ms, int64_t* count, int32_t reset);
 
 /* ---- synthetic panels (bench/test input; no counterpart in the reference) ------------ */

@@ -221,7 +221,7 @@ class Context:
             cls._by_device[int(device)] = ctx
         return ctx
 
-    STAGES = ("decode", "prep", "f2", "count", "qpadm", "qpgraph", "fstat")
+    STAGES = ("decode", "prep", "f2", "count", "qpadm", "qpgraph", "fstat", "frontend")
 
     def timing(self, enable: bool = True):
         check(lib().stpx_timing(self.h, 1 if enable else 0))

@@ -220,7 +220,7 @@ public:
     void qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOptions& o, QgFleetOut& out);
 
     // Per-stage device timing with HIP events on this backend's stream (opt-in).
-    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kQpgraph = 5, kFstat = 6, kStages = 7 };
+    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kQpgraph = 5, kFstat = 6, kFront = 7, kStages = 8 };
     void set_timing(bool on);
     void read_timing(double* ms, std::int64_t* count, bool reset);
 

@@ -126,9 +126,11 @@ F2Blocks* HipBackend::extract_f2_files(const ExtractRequest& req, ExtractInfo& i
             const std::size_t s1 = std::min(M0, s0 + blk);
             read_geno_records(f, s0, s1, host);
             STP_HIP(hipMemcpyAsync(fe_stage_.get(), host.data(), host.size(), hipMemcpyHostToDevice, stream_));
+            mark_begin(kFront);
             launch_transpose_gather(fe_stage_.get(), static_cast<std::int64_t>(bpr),
                                     static_cast<std::int64_t>(s1 - s0), fe_i32_.get(), n_sel, fe_tile_.get(), out_bpr,
                                     static_cast<std::int64_t>(s0 / 4), stream_);
+            mark_end(kFront);
             sync();   // host is reused by the next block
         }
     }
@@ -179,8 +181,10 @@ F2Blocks* HipBackend::extract_f2_files(const ExtractRequest& req, ExtractInfo& i
     rule.drop_monomorphic = req.drop_monomorphic;
     rule.transversions_only = req.transversions_only;
     rule.strand_drop = req.strand_mode == 0;
+    mark_begin(kFront);
     launch_snp_keep(a2_.get(), n_.get(), ld, P, M, fe_alleles_.get(), fe_alleles_.get() + M0, fe_i32_.get(), rule,
                     fe_flags_.get(), stream_);
+    mark_end(kFront);
     std::vector<std::uint8_t> flags(M0);
     STP_HIP(hipMemcpyAsync(flags.data(), fe_flags_.get(), M0, hipMemcpyDeviceToHost, stream_));
     sync();
@@ -211,7 +215,9 @@ F2Blocks* HipBackend::extract_f2_files(const ExtractRequest& req, ExtractInfo& i
                                stream_));
         fe_a2_.ensure(static_cast<std::size_t>(P) * ldk);
         fe_n_.ensure(static_cast<std::size_t>(P) * ldk);
+        mark_begin(kFront);
         launch_compact_counts(a2_.get(), n_.get(), ld, fe_i32_.get(), Mk, P, fe_a2_.get(), fe_n_.get(), ldk, stream_);
+        mark_end(kFront);
         a2 = fe_a2_.get();
         nn = fe_n_.get();
     }
