@@ -418,6 +418,38 @@ def frontend_bench(ctx, n_snp: int, cpu: bool):
     return out
 
 
+def tier_bench(ctx, tile, block_id, nb: int, P: int, prec: int):
+    """The same decode + f2 extract with the result kept in HBM, streamed into host memory,
+    and streamed into an STPF2BK1 f2.bin file (in a temporary directory, removed after):
+    wall time per tier and the rate at which the result leaves the device."""
+    import shutil
+    import tempfile
+    from steppe_amd import _core
+    L = _core.lib()
+    result_bytes = 2 * P * P * nb * 8
+    out = {"result_GB": result_bytes / 1e9, "chunk_MiB": 256}
+    tmp = tempfile.mkdtemp(prefix="stp_tier_")
+    try:
+        for name, tier in (("resident", 0), ("host", 1), ("disk", 2)):
+            path = os.path.join(tmp, "f2.bin")
+            if tier == 2 and shutil.disk_usage(tmp).free < 3 * result_bytes:
+                out[name] = {"skipped": "not enough free disk space"}
+                continue
+            raw = C.c_void_p()
+            t0 = time.perf_counter()
+            _core.check(L.stp_extract_f2_tiered(ctx.h, C.byref(tile), _core._ptr(block_id), nb, prec, tier,
+                                                path.encode(), 0, C.byref(raw)), "tier")
+            dt = time.perf_counter() - t0
+            _core.check(L.stp_f2_handle_free(raw))
+            out[name] = {"s": dt, "GBs": result_bytes / dt / 1e9}
+            log(f"[bench] tier {name}: {dt:.3f} s ({result_bytes / dt / 1e9:.1f} GB/s of result)")
+            if os.path.exists(path):
+                os.remove(path)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 def random_graphs(rng, count: int):
     """count random admixture graphs, 8-12 leaves and 1-4 admixture events each, with
     their simulated f2 over 300 blocks."""
@@ -526,6 +558,7 @@ def main():
     ap.add_argument("--no-fstats", action="store_true")
     ap.add_argument("--frontend-snps", type=int, default=1_200_000)
     ap.add_argument("--no-frontend", action="store_true")
+    ap.add_argument("--no-tiers", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -608,6 +641,9 @@ def main():
     stages["decode"]["hbm_GBs"] = dec_bytes / (avg["decode"] / 1e3) / 1e9 if avg["decode"] else None
     stages["decode"]["algorithmic_bytes"] = dec_bytes
 
+    tiers = None
+    if not args.no_tiers:
+        tiers = tier_bench(ctx, tile, block_id, nb, P, prec)
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -642,6 +678,7 @@ def main():
                    "P": P, "M": M, "n_block": nb, "n_individuals": n_ind,
                    "precision": args.precision, "parallelism": "single GPU"},
         "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg, "fstats": fs, "frontend": fe,
+        "tiers": tiers,
     }
     print(json.dumps(out), flush=True)
 

@@ -114,6 +114,30 @@ int stp_compute_f2_blocks(stp_context* ctx, const double* Q, const double* V, co
 int stp_extract_f2(stp_context* ctx, const stp_tile* tile, const int32_t* block_id,
                    int32_t n_block, int32_t precision, stp_f2_handle** out);
 
+/* Where an f2 result lives (reference: DeviceConfig::ForceTier and ExtractTier,
+ * include/steppe/config.hpp:121-124, include/steppe/extract.hpp:33; the automatic choice of
+ * src/device/tier_select.hpp:57-80).  AUTO keeps it in HBM when it takes at most 0.70 of the
+ * free device memory, else in host memory when it takes at most 0.60 of the available host
+ * memory, else in an STPF2BK1 f2.bin file at disk_path.  The host and disk tiers contract
+ * chunk_blocks blocks at a time into one device slab (0: 256 MiB of f2 + vpair, at most a
+ * quarter of free device memory; the environment variable STEPPE_F2_CHUNK_BLOCKS sets it
+ * when chunk_blocks is 0) and copy
+ * each chunk out while the next is computed.  Every tier holds the same values.  Fits
+ * (qpAdm, qpGraph, f-statistics) need a resident handle and return STP_ERR_INVALID for
+ * the others; stp_f2_handle_download reads every tier. */
+#define STP_TIER_AUTO (-1)
+#define STP_TIER_RESIDENT 0
+#define STP_TIER_HOST 1
+#define STP_TIER_DISK 2
+/* stp_extract_f2 with a tier choice (reference: the tiered engines of
+ * src/device/cuda/cuda_backend_f2_blocks.cu:349-603 and the disk sink of
+ * src/device/cuda/block_sink.cu).  disk_path may be NULL unless the disk tier is used. */
+int stp_extract_f2_tiered(stp_context* ctx, const stp_tile* tile, const int32_t* block_id,
+                          int32_t n_block, int32_t precision, int32_t tier, const char* disk_path,
+                          int64_t chunk_blocks, stp_f2_handle** out);
+/* The tier a handle's tensor lives in (reference: F2ExtractResult::tier). */
+int stp_f2_handle_tier(const stp_f2_handle* h, int32_t* tier);
+
 /* ---- genotype files ---------------------------------------------------------------- */
 
 /* Population selection of stp_extract_f2_files (reference: PopSelection,
@@ -146,6 +170,9 @@ typedef struct stp_extract_request {
     double blgsize;
     int32_t ploidy;
     int32_t precision;           /* STP_PREC_* */
+    int32_t tier;                /* STP_TIER_*; 0 is resident, as in stp_extract_f2 */
+    const char* disk_path;       /* f2.bin path of the disk tier (may be NULL otherwise) */
+    int64_t chunk_blocks;        /* blocks per streamed chunk of the host / disk tiers (0: auto) */
 } stp_extract_request;
 typedef struct stp_extract_info {
     int64_t n_snp_total;         /* min(header SNPs, .snp rows) */

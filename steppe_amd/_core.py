@@ -113,7 +113,8 @@ class ExtractRequest(C.Structure):
                 ("pops", C.POINTER(C.c_char_p)), ("n_pops", C.c_int32), ("top_k", C.c_int64), ("min_n", C.c_int64),
                 ("maf_min", C.c_double), ("maxmiss", C.c_double), ("autosomes_only", C.c_int32),
                 ("drop_monomorphic", C.c_int32), ("transversions_only", C.c_int32), ("strand_mode", C.c_int32),
-                ("blgsize", C.c_double), ("ploidy", C.c_int32), ("precision", C.c_int32)]
+                ("blgsize", C.c_double), ("ploidy", C.c_int32), ("precision", C.c_int32), ("tier", C.c_int32),
+                ("disk_path", C.c_char_p), ("chunk_blocks", C.c_int64)]
 
 
 class ExtractInfo(C.Structure):
@@ -155,6 +156,8 @@ def lib():
         "stp_f2_blocks_from_counts": (C.c_int, [vp, vp, vp, i64, i32, i64, vp, i32, i32, vp, vp]),
         "stp_compute_f2_blocks": (C.c_int, [vp, vp, vp, vp, i32, i64, vp, i32, i32, vp, vp, vp]),
         "stp_extract_f2": (C.c_int, [vp, C.POINTER(Tile), vp, i32, i32, C.POINTER(vp)]),
+        "stp_extract_f2_tiered": (C.c_int, [vp, C.POINTER(Tile), vp, i32, i32, i32, C.c_char_p, i64, C.POINTER(vp)]),
+        "stp_f2_handle_tier": (C.c_int, [vp, C.POINTER(i32)]),
         "stp_f2_handle_upload": (C.c_int, [vp, vp, vp, vp, i32, i32, C.POINTER(vp)]),
         "stp_f2_handle_shape": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32)]),
         "stp_f2_handle_block_sizes": (C.c_int, [vp, vp]),
@@ -331,9 +334,16 @@ class F2Handle:
     def raw(self):
         return self._h
 
+    @property
+    def tier(self) -> str:
+        """Where the tensor lives: "resident" (HBM), "host" or "disk" (an f2.bin file)."""
+        t = C.c_int32()
+        check(lib().stp_f2_handle_tier(self._h, C.byref(t)))
+        return TIER_NAMES[int(t.value)]
+
     def device_pointers(self):
         L = lib()
-        return int(L.stp_f2_handle_f2_device(self._h)), int(L.stp_f2_handle_vpair_device(self._h))
+        return int(L.stp_f2_handle_f2_device(self._h) or 0), int(L.stp_f2_handle_vpair_device(self._h) or 0)
 
     def _download(self, which: str) -> np.ndarray:
         n = self._P * self._P * self._nb
@@ -444,6 +454,20 @@ def parse_precision(precision) -> int:
     return _PRECISIONS[precision]
 
 
+# Output tiers (include/steppe/config.hpp:121 ForceTier, include/steppe/extract.hpp:33).
+TIERS = {"auto": -1, "resident": 0, "host": 1, "hostram": 1, "disk": 2}
+TIER_NAMES = {0: "resident", 1: "host", 2: "disk"}
+
+
+def _tier(tier, disk_path):
+    t = TIERS.get(str(tier).lower())
+    if t is None:
+        raise ValueError(f"tier must be one of 'auto', 'resident', 'host', 'disk' (got {tier!r})")
+    if t == 2 and not disk_path:
+        raise ValueError("the disk tier needs disk_path (the f2.bin file to write)")
+    return t
+
+
 _PLOIDY = {"auto": 0, "1": 1, "pseudo": 1, "pseudo_haploid": 1, "2": 2, "diploid": 2}
 _STRAND = {"drop": 0, "keep": 1, "flip": 2}
 
@@ -452,11 +476,14 @@ def extract_f2_files(geno: str, snp: str, ind: str, pops=None, *, top_k: int = 0
                      out: str = "", device: int = 0, blgsize: float = 0.05, maf: float = 0.0,
                      maxmiss: float = 0.0, autosomes_only: bool = True, drop_monomorphic: bool = False,
                      transversions_only: bool = False, ploidy="auto", strand_mode: str = "drop",
-                     precision=None):
-    """Packed genotype triple -> resident f2 (src/app/extract_f2_core.cpp:96-407): pops
+                     precision=None, tier="auto", disk_path: str = "", chunk_blocks: int = 0):
+    """Packed genotype triple -> f2 (src/app/extract_f2_core.cpp:96-407): pops
     (explicit labels), or the top_k largest populations, or every population with at least
-    min_n individuals.  Returns (F2Handle, info dict), or (out path, info) when ``out`` is
-    set (an STPF2BK1 directory with the reference's meta fields)."""
+    min_n individuals.  tier: "auto" (HBM if the tensor fits 0.70 of free device memory,
+    else host memory, else the f2.bin file disk_path), "resident", "host" or "disk";
+    chunk_blocks: blocks per streamed chunk of the host / disk tiers (0 = auto).  Returns
+    (F2Handle, info dict), or (out path, info) when ``out`` is set (an STPF2BK1 directory
+    with the reference's meta fields)."""
     if pops:
         mode, labels = 0, [str(x) for x in pops]
     elif top_k > 0:
@@ -474,11 +501,12 @@ def extract_f2_files(geno: str, snp: str, ind: str, pops=None, *, top_k: int = 0
         if not os.path.exists(path):
             raise ValueError(f"extract_f2: missing genotype file {path}")
     prec = parse_precision(precision)
+    tr = _tier(tier, disk_path)
     arr = (C.c_char_p * max(len(labels), 1))(*[x.encode() for x in labels])
     req = ExtractRequest(str(geno).encode(), str(snp).encode(), str(ind).encode(), mode, arr, len(labels),
                          int(top_k), int(min_n), float(maf), float(maxmiss), int(bool(autosomes_only)),
                          int(bool(drop_monomorphic)), int(bool(transversions_only)), _STRAND[strand_mode],
-                         float(blgsize), pl, prec)
+                         float(blgsize), pl, prec, tr, str(disk_path).encode(), int(chunk_blocks))
     ctx = Context.get(device)
     raw = C.c_void_p()
     info = ExtractInfo()
@@ -530,11 +558,12 @@ def run_extract_f2(packed, *args, **kw):
 
 def run_extract_f2_tile(packed, n_snp: int, pop_offsets, block_id, n_block: int, pops=None,
                         out: str = "", device: int = 0, ploidy="auto", precision=None,
-                        sample_ploidy=None):
+                        sample_ploidy=None, tier="resident", disk_path: str = "", chunk_blocks: int = 0):
     """Genotype tile -> f2 (the compute core of bindings/bind_fstats.cpp:131-215): a packed
     individual-major tile, population offsets and the jackknife block partition.  ploidy:
-    "auto" (detect), 1 or 2, or "sample" with sample_ploidy.  Returns an F2Handle, or the
-    directory path when ``out`` is set."""
+    "auto" (detect), 1 or 2, or "sample" with sample_ploidy.  tier / disk_path /
+    chunk_blocks choose where the tensor lives, as in extract_f2_files.  Returns an
+    F2Handle, or the directory path when ``out`` is set."""
     n_pop = len(pop_offsets) - 1
     pops = list(pops) if pops is not None else [f"pop{i}" for i in range(n_pop)]
     if sample_ploidy is not None:
@@ -551,7 +580,9 @@ def run_extract_f2_tile(packed, n_snp: int, pop_offsets, block_id, n_block: int,
         raise ValueError("block_id must have one entry per SNP")
     ctx = Context.get(device)
     raw = C.c_void_p()
-    check(lib().stp_extract_f2(ctx.h, C.byref(tile), _ptr(bid), int(n_block), prec, C.byref(raw)),
+    tr = _tier(tier, disk_path)
+    check(lib().stp_extract_f2_tiered(ctx.h, C.byref(tile), _ptr(bid), int(n_block), prec, tr,
+                                      str(disk_path).encode(), int(chunk_blocks), C.byref(raw)),
           "run_extract_f2")
     del keep
     h = F2Handle(raw, pops, device)

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "engine.hpp"
+#include "f2bin.hpp"
 #include "steppe_hip.h"
 
 struct stp_context {
@@ -83,19 +84,8 @@ const stp::F2Blocks& blocks_of(const stp_f2_handle* h) {
 }
 
 // STPF2BK1 header (src/device/f2_disk_format.hpp:17-50): 64 bytes, little-endian.
-struct F2BinHeader {
-    char magic[8];
-    std::uint32_t version;
-    std::uint32_t dtype;
-    std::int32_t P;
-    std::int32_t n_block;
-    std::uint64_t f2_offset;
-    std::uint64_t vpair_offset;
-    std::uint64_t block_sizes_offset;
-    std::uint8_t reserved[16];
-};
-static_assert(sizeof(F2BinHeader) == 64, "STPF2BK1 header is 64 bytes");
-constexpr char kMagic[8] = {'S', 'T', 'P', 'F', '2', 'B', 'K', '1'};
+using stp::F2BinHeader;
+constexpr const char* kMagic = stp::kF2BinMagic;
 
 F2BinHeader read_header(std::ifstream& f, const std::string& path) {
     F2BinHeader h{};
@@ -196,6 +186,29 @@ int stp_extract_f2(stp_context* ctx, const stp_tile* tile, const int32_t* block_
     });
 }
 
+int stp_extract_f2_tiered(stp_context* ctx, const stp_tile* tile, const int32_t* block_id, int32_t n_block,
+                          int32_t precision, int32_t tier, const char* disk_path, int64_t chunk_blocks,
+                          stp_f2_handle** out) {
+    return guard([&] {
+        STP_REQUIRE(tile && out, "stp_extract_f2_tiered: null argument");
+        stp::TierRequest tr;
+        tr.tier = tier;
+        tr.disk_path = disk_path ? disk_path : "";
+        tr.chunk_blocks = chunk_blocks;
+        auto h = std::make_unique<stp_f2_handle>();
+        h->ctx = ctx;
+        h->blocks.reset(backend(ctx).extract_f2(*tile, block_id, n_block, precision, tr));
+        *out = h.release();
+    });
+}
+
+int stp_f2_handle_tier(const stp_f2_handle* h, int32_t* tier) {
+    return guard([&] {
+        STP_REQUIRE(tier, "null output");
+        *tier = blocks_of(h).tier;
+    });
+}
+
 int stp_extract_f2_files(stp_context* ctx, const stp_extract_request* req, stp_f2_handle** out,
                          stp_extract_info* info) {
     return guard([&] {
@@ -226,6 +239,9 @@ int stp_extract_f2_files(stp_context* ctx, const stp_extract_request* req, stp_f
         r.blgsize = req->blgsize;
         r.ploidy = req->ploidy;
         r.precision = req->precision;
+        r.tier.tier = req->tier;
+        r.tier.disk_path = req->disk_path ? req->disk_path : "";
+        r.tier.chunk_blocks = req->chunk_blocks;
         stp::ExtractInfo inf;
         auto h = std::make_unique<stp_f2_handle>();
         h->ctx = ctx;
@@ -284,23 +300,16 @@ int stp_f2_handle_block_sizes(const stp_f2_handle* h, int32_t* block_sizes) {
 }
 
 const double* stp_f2_handle_f2_device(const stp_f2_handle* h) {
-    return (h && h->blocks) ? h->blocks->f2.get() : nullptr;
+    return (h && h->blocks && h->blocks->tier == stp::kTierResident) ? h->blocks->f2.get() : nullptr;
 }
 
 const double* stp_f2_handle_vpair_device(const stp_f2_handle* h) {
-    return (h && h->blocks) ? h->blocks->vpair.get() : nullptr;
+    return (h && h->blocks && h->blocks->tier == stp::kTierResident) ? h->blocks->vpair.get() : nullptr;
 }
 
 int stp_f2_handle_download(const stp_f2_handle* h, double* f2, double* vpair) {
     return guard([&] {
-        const auto& b = blocks_of(h);
-        auto& be = backend(h->ctx);
-        be.activate();
-        const std::size_t tot = static_cast<std::size_t>(b.P) * b.P * b.n_block;
-        if (f2) STP_HIP(hipMemcpyAsync(f2, b.f2.get(), tot * sizeof(double), hipMemcpyDeviceToHost, be.stream()));
-        if (vpair)
-            STP_HIP(hipMemcpyAsync(vpair, b.vpair.get(), tot * sizeof(double), hipMemcpyDeviceToHost, be.stream()));
-        be.sync();
+        backend(h->ctx).download(blocks_of(h), f2, vpair);
     });
 }
 
@@ -568,15 +577,7 @@ int stp_f2bin_write(const char* path, const double* f2, const double* vpair, con
         STP_REQUIRE(path && f2 && vpair && block_sizes, "stp_f2bin_write: null argument");
         STP_REQUIRE(P > 0 && n_block > 0, "stp_f2bin_write: degenerate f2 tensor");
         const std::uint64_t slab = static_cast<std::uint64_t>(P) * P * n_block * sizeof(double);
-        F2BinHeader h{};
-        std::memcpy(h.magic, kMagic, 8);
-        h.version = 1;
-        h.dtype = 1;
-        h.P = P;
-        h.n_block = n_block;
-        h.f2_offset = 64;
-        h.vpair_offset = 64 + slab;
-        h.block_sizes_offset = 64 + 2 * slab;
+        const F2BinHeader h = stp::f2bin_header(P, n_block);
         std::ofstream o(path, std::ios::binary | std::ios::trunc);
         if (!o) throw IoError(std::string("cannot open for write: ") + path);
         o.write(reinterpret_cast<const char*>(&h), sizeof(h));

@@ -401,11 +401,12 @@ void HipBackend::f2_from_qvn(const double* Q, const double* V, const double* N, 
     for (int b = 0; b < n_block; ++b) block_sizes[b] = plan.len[b];
 }
 
-F2Blocks* HipBackend::extract_f2(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision) {
+F2Blocks* HipBackend::extract_f2(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision,
+                                  const TierRequest& tier) {
     activate();
     const std::int64_t M = static_cast<std::int64_t>(t.n_snp);
     STP_REQUIRE(t.n_pop > 0, "extract_f2: at least one population is required");
-    const F2Plan& plan = plan_for(t.n_pop, M, block_id, n_block);
+    (void)make_f2_plan(t.n_pop, M, block_id, n_block);   // validate the partition before decoding
     const std::int64_t ld = round_up(std::max<std::int64_t>(M, 1), 64);
     a2_.ensure(static_cast<std::size_t>(t.n_pop) * ld);
     n_.ensure(static_cast<std::size_t>(t.n_pop) * ld);
@@ -414,22 +415,7 @@ F2Blocks* HipBackend::extract_f2(const stp_tile& t, const std::int32_t* block_id
     launch_decode_counts(d_packed_, static_cast<std::int64_t>(t.bytes_per_record), M, pop_offsets_.get(),
                          ploidy_.get(), t.n_pop, a2_.get(), n_.get(), ld, stream_);
     mark_end(kDecode);
-    auto out = std::make_unique<F2Blocks>();
-    out->P = t.n_pop;
-    out->n_block = n_block;
-    const std::size_t tot = static_cast<std::size_t>(t.n_pop) * t.n_pop * n_block;
-    out->f2.alloc(tot);
-    out->vpair.alloc(tot);
-    out->block_sizes = plan.len;
-    const std::uint16_t* a2 = a2_.get();
-    const std::uint16_t* nn = n_.get();
-    F2Source src;
-    src.a2 = a2;
-    src.n = nn;
-    src.ld = ld;
-    run_f2(plan, precision, src, out->f2.get(), out->vpair.get());
-    sync();
-    return out.release();
+    return f2_counts_tiered(a2_.get(), n_.get(), ld, t.n_pop, M, block_id, n_block, precision, tier);
 }
 
 F2Blocks* HipBackend::upload(const double* f2, const double* vpair, const std::int32_t* block_sizes, int P,

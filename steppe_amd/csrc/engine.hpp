@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -77,12 +78,46 @@ struct F2Plan {
 // core::block_ranges throws on the same conditions, block_partition_rule.hpp:54-96).
 F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_block);
 
+// Where an f2 result lives (the reference's OutputTier, src/device/tier_select.hpp:12-17):
+// in HBM, in host memory, or in an STPF2BK1 f2.bin file written block range by block range.
+enum F2Tier { kTierResident = 0, kTierHost = 1, kTierDisk = 2 };
+
+// Host memory of the host tier: 2 MiB aligned and advised for transparent huge pages (a
+// result of many GB would otherwise spend most of its copy time in 4 KiB page faults);
+// left uninitialised, since the streamed contraction writes every block.
+class HostArray {
+public:
+    void alloc(std::size_t n);
+    double* get() const { return p_.get(); }
+    std::size_t size() const { return n_; }
+
+private:
+    struct Free {
+        void operator()(double* p) const;
+    };
+    std::unique_ptr<double, Free> p_;
+    std::size_t n_ = 0;
+};
+
 class F2Blocks {
 public:
     int P = 0, n_block = 0;
-    DevBuf<double> f2, vpair;
+    int tier = kTierResident;
+    DevBuf<double> f2, vpair;                 // resident tier
+    HostArray host_f2, host_vpair;            // host tier
+    std::string disk_path;                    // disk tier
     std::vector<std::int32_t> block_sizes;
     std::vector<std::string> labels;   // population labels when built from files
+};
+
+// Tier of an f2 computation: -1 chooses from free memory as the reference's
+// select_output_tier does (src/device/tier_select.hpp:57-80: resident when the result
+// fits 70% of free HBM, else host when it fits 60% of free host memory, else disk);
+// chunk_blocks (0: from free HBM) bounds the blocks computed per streamed chunk.
+struct TierRequest {
+    int tier = -1;
+    std::string disk_path;
+    std::int64_t chunk_blocks = 0;
 };
 
 // Genotype files -> f2 (the reference's run_extract_f2 arguments,
@@ -93,6 +128,7 @@ struct ExtractRequest {
     double maf_min = 0.0, maxmiss = 1.0;
     bool autosomes_only = false, drop_monomorphic = false, transversions_only = false;
     int strand_mode = 0;          // 0 drop strand-ambiguous SNPs, 1 keep, 2 flip (= keep)
+    TierRequest tier;
     double blgsize = 0.05;        // jackknife block size in Morgans
     int ploidy = 0;               // 0 detect per sample, 1 pseudo-haploid, 2 diploid
     int precision = 0;
@@ -182,10 +218,13 @@ public:
     void f2_from_qvn(const double* Q, const double* V, const double* N, int P, std::int64_t M,
                      const std::int32_t* block_id, int n_block, int precision, double* f2,
                      double* vpair, std::int32_t* block_sizes);
-    F2Blocks* extract_f2(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision);
+    F2Blocks* extract_f2(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision,
+                         const TierRequest& tier = TierRequest{kTierResident, "", 0});
     // Genotype files -> resident f2 (frontend_host.cpp): read the packed triple, build the
     // individual-major tile in HBM, decode, keep / compact SNPs, block, contract.
     F2Blocks* extract_f2_files(const ExtractRequest& req, ExtractInfo& info);
+    // Host copies of any tier's tensors (f2 / vpair may be null).
+    void download(const F2Blocks& h, double* f2, double* vpair);
     F2Blocks* upload(const double* f2, const double* vpair, const std::int32_t* block_sizes, int P,
                      int n_block);
     // Batched qpAdm over models on a resident tensor; results to host (qpadm_host.cpp).
@@ -264,6 +303,11 @@ private:
         const double* nq = nullptr;
     };
     void run_f2(const F2Plan& plan, int precision, const F2Source& src, double* d_f2, double* d_vpair);
+    // f2 over device counts into the requested tier (tiered.cpp): the whole tensor in HBM,
+    // or block ranges computed into a HBM slab and streamed to host memory or an f2.bin.
+    F2Blocks* f2_counts_tiered(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P,
+                               std::int64_t M, const std::int32_t* block_id, int n_block, int precision,
+                               const TierRequest& tier);
 
     int device_ = 0;
     int n_cu_ = 0;

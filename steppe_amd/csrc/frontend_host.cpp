@@ -221,21 +221,9 @@ F2Blocks* HipBackend::extract_f2_files(const ExtractRequest& req, ExtractInfo& i
         a2 = fe_a2_.get();
         nn = fe_n_.get();
     }
-    const F2Plan& plan = plan_for(P, Mk, block_id.data(), nb);
-    auto out = std::make_unique<F2Blocks>();
-    out->P = P;
-    out->n_block = nb;
-    const std::size_t tot = static_cast<std::size_t>(P) * P * nb;
-    out->f2.alloc(tot);
-    out->vpair.alloc(tot);
-    out->block_sizes = plan.len;
+    auto out = std::unique_ptr<F2Blocks>(f2_counts_tiered(a2, nn, ldk, P, Mk, block_id.data(), nb, req.precision,
+                                                          req.tier));
     for (const auto& g : groups) out->labels.push_back(g.label);
-    F2Source src;
-    src.a2 = a2;
-    src.n = nn;
-    src.ld = ldk;
-    run_f2(plan, req.precision, src, out->f2.get(), out->vpair.get());
-    sync();
     info.n_snp_total = M;
     info.n_snp_kept = Mk;
     return out.release();

@@ -77,6 +77,8 @@ void keep_top(std::vector<SweepHit>& res, std::int64_t K, int filter, double& ta
 
 void HipBackend::survivor_list(const F2Blocks& h, std::vector<std::int32_t>& surv, std::vector<std::int32_t>& bsz,
                                double& n_total) {
+    STP_REQUIRE(h.tier == kTierResident,
+                "f-statistics: the f2 tensor is held in host memory or on disk; fits need it resident in HBM");
     const int P = h.P, nb = h.n_block;
     surv.clear();
     bsz.clear();

@@ -79,6 +79,8 @@ void HipBackend::qpadm_batch(const F2Blocks& h, int n_models, const std::int32_t
                 "qpadm: offsets are not cumulative");
 
     // ---- survivor blocks ----
+    STP_REQUIRE(h.tier == kTierResident,
+                "qpadm: the f2 tensor is held in host memory or on disk; fits need it resident in HBM");
     const std::int64_t slab = static_cast<std::int64_t>(P) * P;
     qp_keep_.ensure(static_cast<std::size_t>(nb));
     launch_survivors(h.vpair.get(), slab, nb, qp_keep_.get(), stream_);

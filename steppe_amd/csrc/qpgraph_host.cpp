@@ -250,6 +250,8 @@ QgSet* HipBackend::qpgraph_set_create(const std::vector<const QgTopology*>& topo
     for (int g = 0; g < n; ++g) {
         STP_REQUIRE(f2[g] != nullptr, "qpgraph: null f2 handle");
         const F2Blocks& h = *f2[g];
+        STP_REQUIRE(h.tier == kTierResident,
+                    "qpgraph: the f2 tensor is held in host memory or on disk; fits need it resident in HBM");
         job_topo[g] = g;
         job_P[g] = h.P;
         job_nb[g] = h.n_block;

@@ -98,3 +98,44 @@ def test_raw_fixture_reader_matches_reference_layout(golden_dir):
     f2, bs = read_raw_fixture(os.path.join(golden_dir, "at2", "f2_fit0_9pop.bin"))
     assert f2.shape == (9, 9, len(bs)) and len(bs) == 710
     assert np.allclose(f2, f2.transpose(1, 0, 2))
+
+
+def test_tier_names_and_request_layout():
+    # tier names map onto STP_TIER_* (include/steppe_hip.h); the disk tier needs a path
+    assert [_core._tier(t, "p") for t in ("auto", "resident", "host", "HostRam", "disk")] == [-1, 0, 1, 1, 2]
+    with pytest.raises(ValueError, match="disk_path"):
+        _core._tier("disk", "")
+    with pytest.raises(ValueError, match="tier"):
+        _core._tier("nvme", "")
+    names = [f[0] for f in _core.ExtractRequest._fields_]
+    assert names[-4:] == ["precision", "tier", "disk_path", "chunk_blocks"]
+
+
+STRUCTS = {"stp_tile": "Tile", "stp_extract_request": "ExtractRequest", "stp_extract_info": "ExtractInfo",
+           "stp_qpadm_options": "QpAdmOptions", "stp_sweep_request": "SweepRequest"}
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror has the C compiler's size and field offsets for its struct."""
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "steppe_hip.h"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        lines.append(f'printf("{pyname} size %zu\\n", sizeof({cname}));')
+        for f in getattr(_core, pyname)._fields_:
+            lines.append(f'printf("{pyname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = str(tmp_path / "layout")
+    subprocess.run([gcc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", exe], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    rows = [ln for ln in out if ln]
+    assert len(rows) == sum(len(getattr(_core, n)._fields_) + 1 for n in STRUCTS.values())
+    for ln in rows:
+        pyname, field, val = ln.split()
+        cls = getattr(_core, pyname)
+        got = C.sizeof(cls) if field == "size" else getattr(cls, field).offset
+        assert got == int(val), (pyname, field, got, val)
