@@ -394,6 +394,92 @@ int stp_qpgraph_fit_fleet(stp_context* ctx, stp_qpgraph_set* s, const stp_qpgrap
                           double* restart_spread, double* theta, double* theta_lo, double* theta_hi,
                           double* edge_length, double* f3_fit, int32_t* iterations, int32_t* status);
 
+/* ---- qpGraph topology enumeration and search ------------------------------------- */
+
+/* A list of candidate graphs (reference: EnumeratedTopology,
+ * src/core/qpadm/qpgraph_enumerate.hpp:19-25): edges (parent, child names; leaves keep
+ * their labels, other nodes are "N<id>"), admixture count, id and shape hash. */
+typedef struct stp_qpgraph_list stp_qpgraph_list;
+/* Every tree on the leaves, then (max_nadmix >= 1) every distinct one-admixture graph
+ * (reference: enumerate_bounded_space, src/core/qpadm/qpgraph_enumerate.cpp:229-236; 105
+ * trees and 1485 admixture graphs on 5 leaves).  Same order, names and hashes. */
+int stp_qpgraph_enumerate(int32_t n_leaves, const char* const* leaves, int32_t max_nadmix,
+                          stp_qpgraph_list** out);
+/* Hill-climb moves from a graph (reference: topology_neighbors,
+ * src/core/qpadm/qpgraph_enumerate.cpp:383-407). */
+int stp_qpgraph_neighbors(int32_t n_edges, const char* const* from, const char* const* to, int32_t nadmix,
+                          uint64_t hash, int32_t n_leaves, const char* const* leaves, int32_t max_nadmix,
+                          stp_qpgraph_list** out);
+/* Shape hash of an edge list, leaves ranked by label (reference: graph_hash,
+ * src/core/qpadm/qpgraph_enumerate.cpp:183-202). */
+int stp_qpgraph_graph_hash(int32_t n_edges, const char* const* from, const char* const* to, uint64_t* hash);
+int stp_qpgraph_list_size(const stp_qpgraph_list* l, int32_t* n);
+int stp_qpgraph_list_get(const stp_qpgraph_list* l, int32_t i, int32_t* nadmix, int32_t* id, uint64_t* hash,
+                         int32_t* n_edges);
+/* Edge k of graph i; names are cut to len - 1 bytes. */
+int stp_qpgraph_list_edge(const stp_qpgraph_list* l, int32_t i, int32_t k, char* from, char* to, int32_t len);
+int stp_qpgraph_list_free(stp_qpgraph_list* l);
+
+/* Options of stp_qpgraph_search_run (reference: QpGraphSearchOptions,
+ * include/steppe/qpgraph_search.hpp:30-36, and QpGraphOptions for the fits). */
+typedef struct stp_qpgraph_search_options {
+    int32_t max_nadmix;          /* 0: trees only; 1: trees and one-admixture graphs (larger acts as 1) */
+    int32_t numstart;            /* restarts per admixture graph (10) */
+    int32_t maxit;               /* L-BFGS iterations per restart (200) */
+    double tol;                  /* 1e-9 */
+    double fudge;                /* 1e-4 */
+    double diag_f3;              /* 1e-5 */
+    int32_t constrained;         /* 1 */
+    int32_t run_heuristic;       /* 1: also run the hill-climb from heuristic_seeds starts (8) */
+    int32_t heuristic_seeds;
+} stp_qpgraph_search_options;
+typedef struct stp_qpgraph_search_summary {
+    int32_t status;              /* 0 ok, 3 non_spd_covariance */
+    int32_t n_trees;
+    int32_t n_admix1;
+    int32_t n_candidates;        /* enumerated */
+    int32_t n_fitted;            /* candidates that parse on this pop-set */
+    int32_t best;                /* index into the fitted candidates */
+    int32_t best_candidate;      /* index into the enumeration */
+    int32_t best_nadmix;
+    uint64_t best_hash;
+    double best_score;
+    double second_best_score;
+    int32_t heuristic_recovered;
+    int32_t n_seeds;
+    double fit_all_wall_ms;      /* the fleet fit of every candidate */
+    double topologies_per_s;
+} stp_qpgraph_search_summary;
+typedef struct stp_qpgraph_search stp_qpgraph_search;
+/* The topology search (reference: run_qpgraph_search, src/core/qpadm/qpgraph_search.cpp:
+ * 110-297): every candidate on pops (labels among names, the f2 handle's population
+ * labels; at least 3, distinct) is parsed with pops[0] as f3 base, every candidate shares
+ * one f3 basis f3(pops[0]; a, b), a <= b over pops[1..], with its covariance and inverse
+ * computed once on the device, and all restarts of all candidates are fitted in one
+ * device fleet; the best is the first lowest score.  A pop outside names returns
+ * STP_ERR_INVALID. */
+int stp_qpgraph_search_run(stp_context* ctx, const stp_f2_handle* f2, int32_t n_names, const char* const* names,
+                           int32_t n_pops, const char* const* pops, const stp_qpgraph_search_options* options,
+                           stp_qpgraph_search** out);
+int stp_qpgraph_search_get_summary(const stp_qpgraph_search* s, stp_qpgraph_search_summary* out);
+/* The enumeration the search scored (borrowed; valid until stp_qpgraph_search_free). */
+const stp_qpgraph_list* stp_qpgraph_search_candidates(const stp_qpgraph_search* s);
+/* Per fitted candidate (n_fitted rows, enumeration order): its index in the enumeration,
+ * best score and restart spread (reference: QpGraphCandidate).  Any output may be NULL. */
+int stp_qpgraph_search_scores(const stp_qpgraph_search* s, int32_t* candidate, double* score,
+                              double* restart_spread);
+/* Final graph hash of each hill-climb seed (n_seeds). */
+int stp_qpgraph_search_seeds(const stp_qpgraph_search* s, uint64_t* hashes);
+/* The best graph's fit (reference: QpGraphSearchResult::best_fit): dims as
+ * stp_qpgraph_topology_dims; weights, their restart range (nadmix), edge lengths (nedge),
+ * fitted and observed f3 and the unridged Q diagonal (npair).  Outputs may be NULL. */
+int stp_qpgraph_search_best_fit(const stp_qpgraph_search* s, int32_t* dims, double* weight, double* weight_lo,
+                                double* weight_hi, double* edge_length, double* f3_fit, double* f_obs,
+                                double* qdiag);
+/* Labels of the best graph, kinds as stp_qpgraph_topology_label. */
+int stp_qpgraph_search_best_label(const stp_qpgraph_search* s, int32_t kind, int32_t index, char* buf, int32_t len);
+int stp_qpgraph_search_free(stp_qpgraph_search* s);
+
 /* ---- STPF2BK1 f2.bin (reference: src/device/f2_disk_format.hpp:17-50,
  *      src/app/f2_dir_writer.cpp:306-440, src/app/f2_dir_io.cpp:52-158) ------------- */
 

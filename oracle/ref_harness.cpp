@@ -17,6 +17,10 @@
 //   ref_f4               steppe::run_f4 (host tensor)       src/core/qpadm/f4.cpp:106
 //   ref_f3               steppe::run_f3 (host tensor)       src/core/qpadm/f3.cpp:105
 //   ref_f4ratio          steppe::run_f4ratio (host tensor)  src/core/qpadm/f4ratio.cpp:94
+//   ref_qg_enumerate     core::qpadm::enumerate_bounded_space  src/core/qpadm/qpgraph_enumerate.cpp:229
+//   ref_qg_neighbors     core::qpadm::topology_neighbors       src/core/qpadm/qpgraph_enumerate.cpp:383
+//   ref_qg_graph_hash    core::qpadm::graph_hash               src/core/qpadm/qpgraph_enumerate.cpp:183
+//   ref_qg_search        steppe::run_qpgraph_search (host tensor)  src/core/qpadm/qpgraph_search.cpp:290
 //   ref_extract_*        the CPU branch of steppe::run_extract_f2 (src/app/extract_f2_core.cpp:
 //                        96-407, on_device == false) over the reference's own GenoReader,
 //                        read_ind, read_snp, ComputeBackend::transpose_to_canonical,
@@ -38,6 +42,7 @@
 #include "io/geno_reader.hpp"
 #include "io/ind_reader.hpp"
 #include "io/snp_reader.hpp"
+#include "core/qpadm/qpgraph_enumerate.hpp"
 #include "core/qpadm/qpgraph_model.hpp"
 #include "core/qpadm/qpgraph_objective.hpp"
 #include "core/internal/views.hpp"
@@ -50,6 +55,7 @@
 #include "steppe/fstats.hpp"
 #include "steppe/qpadm.hpp"
 #include "steppe/qpgraph.hpp"
+#include "steppe/qpgraph_search.hpp"
 
 #include "stp_qpadm_result.h"
 
@@ -510,6 +516,125 @@ int ref_extract_fetch(void* h, double* f2, double* vpair, std::int32_t* bsz) {
     for (int b = 0; b < x->nb; ++b) bsz[b] = x->t.block_sizes[static_cast<std::size_t>(b)];
     return 0;
 }
+
+// ---- qpGraph topology enumeration and search --------------------------------------
+struct RefTopoList {
+    std::vector<int> nadmix, id;
+    std::vector<unsigned long long> hash;
+    std::vector<std::string> edges;          // "from>to;from>to;..."
+    std::vector<double> score, spread;       // search candidates only
+};
+
+static std::string join_edges(const std::vector<steppe::QpGraphEdge>& e) {
+    std::string s;
+    for (std::size_t k = 0; k < e.size(); ++k) {
+        if (k) s += ';';
+        s += e[k].from + ">" + e[k].to;
+    }
+    return s;
+}
+
+static std::vector<steppe::QpGraphEdge> split_edges(const char* str) {
+    std::vector<steppe::QpGraphEdge> out;
+    std::string s(str);
+    std::size_t at = 0;
+    while (at < s.size()) {
+        std::size_t end = s.find(';', at);
+        if (end == std::string::npos) end = s.size();
+        const std::string tok = s.substr(at, end - at);
+        const std::size_t gt = tok.find('>');
+        if (gt != std::string::npos) out.push_back({tok.substr(0, gt), tok.substr(gt + 1)});
+        at = end + 1;
+    }
+    return out;
+}
+
+static RefTopoList* to_list(const std::vector<steppe::core::qpadm::EnumeratedTopology>& v) {
+    auto* L = new RefTopoList;
+    for (const auto& t : v) {
+        L->nadmix.push_back(t.nadmix);
+        L->id.push_back(t.id);
+        L->hash.push_back(t.hash);
+        L->edges.push_back(join_edges(t.edges));
+    }
+    return L;
+}
+
+void* ref_qg_enumerate(int n, const char* const* leaves, int max_nadmix) {
+    return to_list(steppe::core::qpadm::enumerate_bounded_space(qg_names(n, leaves), max_nadmix));
+}
+
+void* ref_qg_neighbors(const char* edges, int nadmix, unsigned long long hash, int n, const char* const* leaves,
+                       int max_nadmix) {
+    steppe::core::qpadm::EnumeratedTopology cur;
+    cur.edges = split_edges(edges);
+    cur.nadmix = nadmix;
+    cur.hash = hash;
+    return to_list(steppe::core::qpadm::topology_neighbors(cur, qg_names(n, leaves), max_nadmix));
+}
+
+unsigned long long ref_qg_graph_hash(const char* edges) {
+    return steppe::core::qpadm::graph_hash(split_edges(edges));
+}
+
+// scal: best_score, second_best_score, status, heuristic_recovered, best_nadmix,
+//       n_trees, n_admix1, n_candidates, fit_all_wall_ms; best_hash[0]
+void* ref_qg_search(const double* f2, const double* vpair, const std::int32_t* bsz, int P, int nb, int n_names,
+                    const char* const* names, int n_pops, const char* const* pops, int max_nadmix, int numstart,
+                    double fudge, double diag_f3, int constrained, int run_heuristic, double* scal,
+                    unsigned long long* best_hash) {
+    try {
+        steppe::F2BlockTensor t = host_tensor(f2, vpair, bsz, P, nb);
+        steppe::QpGraphSearchOptions o;
+        o.pops = qg_names(n_pops, pops);
+        o.max_nadmix = max_nadmix;
+        o.fit.numstart = numstart;
+        o.fit.fudge = fudge;
+        o.fit.diag_f3 = diag_f3;
+        o.fit.constrained = constrained != 0;
+        o.run_heuristic = run_heuristic != 0;
+        auto res = cpu_resources();
+        const steppe::QpGraphSearchResult r = steppe::run_qpgraph_search(t, qg_names(n_names, names), o, res);
+        scal[0] = r.best.score;
+        scal[1] = r.second_best_score;
+        scal[2] = static_cast<double>(static_cast<int>(r.status));
+        scal[3] = r.heuristic_recovered ? 1.0 : 0.0;
+        scal[4] = r.best.nadmix;
+        scal[5] = r.n_trees;
+        scal[6] = r.n_admix1;
+        scal[7] = r.n_candidates;
+        scal[8] = r.fit_all_wall_ms;
+        best_hash[0] = r.best.hash;
+        auto* L = new RefTopoList;
+        for (const auto& c : r.candidates) {
+            L->nadmix.push_back(c.nadmix);
+            L->id.push_back(c.id);
+            L->hash.push_back(c.hash);
+            L->edges.push_back(join_edges(c.edges));
+            L->score.push_back(c.score);
+            L->spread.push_back(c.restart_spread);
+        }
+        return L;
+    } catch (const std::exception&) {
+        return nullptr;
+    }
+}
+
+int ref_qg_list_size(void* h) { return static_cast<int>(static_cast<RefTopoList*>(h)->hash.size()); }
+
+int ref_qg_list_get(void* h, int i, int* nadmix, int* id, unsigned long long* hash, double* score, double* spread) {
+    const auto* L = static_cast<RefTopoList*>(h);
+    *nadmix = L->nadmix[i];
+    *id = L->id[i];
+    *hash = L->hash[i];
+    *score = L->score.empty() ? 0.0 : L->score[i];
+    *spread = L->spread.empty() ? 0.0 : L->spread[i];
+    return 0;
+}
+
+const char* ref_qg_list_edges(void* h, int i) { return static_cast<RefTopoList*>(h)->edges[i].c_str(); }
+
+void ref_qg_list_free(void* h) { delete static_cast<RefTopoList*>(h); }
 
 const char* ref_extract_label(void* h, int i) { return static_cast<RefExtract*>(h)->labels[i].c_str(); }
 

@@ -118,3 +118,58 @@ def f4_sweep(f2, pops=None, *, min_z=3.0, top_k=0, sure=False, all_combinations=
 def f3_sweep(f2, pops=None, *, min_z=3.0, top_k=0, sure=False, all_combinations=False):
     """Every f3 over the 3-subsets of pops, filtered by |z| (run_f3_sweep)."""
     return run_fstat_sweep(f2, 3, pops, min_z=min_z, top_k=top_k, sure=sure, all_combinations=all_combinations)
+
+
+class QpGraphSearchResult:
+    """Topology-search result with the reference facade's attributes
+    (bindings/steppe/__init__.py:721-815): n_trees, n_admix1, n_candidates, best_score,
+    second_best_score, best_nadmix, best_hash, best_edges, heuristic_recovered,
+    fit_all_wall_ms, topologies_per_s, argmin and candidates (a DataFrame of nadmix,
+    hash, score, restart_spread in enumeration order)."""
+
+    def __init__(self, d: dict):
+        self._d = d
+
+    def __getattr__(self, name):
+        d = self.__dict__.get("_d", {})
+        if name in ("n_trees", "n_admix1", "n_candidates", "best_nadmix", "best_hash"):
+            return int(d[name])
+        if name in ("best_score", "second_best_score", "fit_all_wall_ms", "topologies_per_s"):
+            return float(d[name])
+        if name == "heuristic_recovered":
+            return bool(d[name])
+        if name == "best_edges":
+            return [(str(p), str(c)) for p, c in d["best_edges"]]
+        raise AttributeError(name)
+
+    @property
+    def argmin(self) -> int:
+        scores = self._d.get("cand_score") or []
+        if not scores:
+            return -1
+        best, idx = scores[0], 0
+        for i, s in enumerate(scores):
+            if s < best:
+                best, idx = s, i
+        return idx
+
+    @property
+    def candidates(self):
+        import pandas as pd
+        return pd.DataFrame({"nadmix": list(self._d.get("cand_nadmix") or []),
+                             "hash": list(self._d.get("cand_hash") or []),
+                             "score": list(self._d.get("cand_score") or []),
+                             "restart_spread": list(self._d.get("cand_restart_spread") or [])})
+
+    def __repr__(self) -> str:
+        return (f"QpGraphSearchResult(n_candidates={self.n_candidates}, best_score={self.best_score:.6f}, "
+                f"best_nadmix={self.best_nadmix}, heuristic_recovered={self.heuristic_recovered})")
+
+
+def qpgraph_search(f2, *, pops, max_nadmix=1, numstart=10, fudge=1e-4, diag_f3=1e-5, constrained=True,
+                   run_heuristic=True):
+    """qpGraph topology search (bindings/steppe/__init__.py:820-857): every rooted topology
+    on ``pops`` with at most ``max_nadmix`` (<= 1) admixture events, fitted on the device."""
+    from .qpgraph import run_qpgraph_search
+    return QpGraphSearchResult(run_qpgraph_search(f2, pops, max_nadmix, numstart, fudge, diag_f3, constrained,
+                                                  run_heuristic))

@@ -3,6 +3,7 @@
 // Every entry point runs inside guard(): exceptions never cross the ABI; they are
 // mapped to status codes and the message is kept per thread for stp_last_error().
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -30,6 +31,15 @@ struct stp_qpgraph_topology {
 struct stp_qpgraph_set {
     stp_context* ctx = nullptr;
     std::unique_ptr<stp::QgSet> s;
+};
+
+struct stp_qpgraph_list {
+    std::vector<stp::qgenum::Candidate> c;
+};
+
+struct stp_qpgraph_search {
+    stp_qpgraph_list cands;
+    stp::QgSearchResult r;
 };
 
 namespace {
@@ -424,24 +434,229 @@ int stp_qpgraph_topology_dims(const stp_qpgraph_topology* t, int32_t* dims) {
     });
 }
 
+namespace {
+void topology_label(const stp::QgTopology& m, int32_t kind, int32_t index, char* buf, int32_t len) {
+    STP_REQUIRE(buf && len > 0, "qpgraph label: null argument");
+    const std::vector<std::string>* v = nullptr;
+    switch (kind) {
+        case 0: v = &m.leaves; break;
+        case 1: v = &m.admix_from; break;
+        case 2: v = &m.admix_to; break;
+        case 3: v = &m.edge_from; break;
+        case 4: v = &m.edge_to; break;
+        default: throw stp::ArgError("qpgraph label: kind must be 0..4");
+    }
+    STP_REQUIRE(index >= 0 && index < static_cast<int32_t>(v->size()), "qpgraph label: index out of range");
+    std::strncpy(buf, (*v)[index].c_str(), static_cast<std::size_t>(len - 1));
+    buf[len - 1] = '\0';
+}
+
+std::vector<std::pair<std::string, std::string>> edge_list(int32_t n, const char* const* from, const char* const* to) {
+    STP_REQUIRE(n >= 0 && (n == 0 || (from && to)), "qpgraph: null edge list");
+    std::vector<std::pair<std::string, std::string>> e;
+    for (int i = 0; i < n; ++i) {
+        STP_REQUIRE(from[i] && to[i], "qpgraph: null edge endpoint");
+        e.emplace_back(from[i], to[i]);
+    }
+    return e;
+}
+
+std::vector<std::string> label_list(int32_t n, const char* const* names) {
+    STP_REQUIRE(n >= 0 && (n == 0 || names), "qpgraph: null label list");
+    std::vector<std::string> v;
+    for (int i = 0; i < n; ++i) {
+        STP_REQUIRE(names[i], "qpgraph: null label");
+        v.emplace_back(names[i]);
+    }
+    return v;
+}
+}  // namespace
+
 int stp_qpgraph_topology_label(const stp_qpgraph_topology* t, int32_t kind, int32_t index, char* buf,
                                int32_t len) {
     return guard([&] {
-        STP_REQUIRE(t && buf && len > 0, "stp_qpgraph_topology_label: null argument");
-        const auto& m = t->t;
-        const std::vector<std::string>* v = nullptr;
-        switch (kind) {
-            case 0: v = &m.leaves; break;
-            case 1: v = &m.admix_from; break;
-            case 2: v = &m.admix_to; break;
-            case 3: v = &m.edge_from; break;
-            case 4: v = &m.edge_to; break;
-            default: throw stp::ArgError("stp_qpgraph_topology_label: kind must be 0..4");
-        }
-        STP_REQUIRE(index >= 0 && index < static_cast<int32_t>(v->size()), "stp_qpgraph_topology_label: index out of range");
-        std::strncpy(buf, (*v)[index].c_str(), static_cast<std::size_t>(len - 1));
-        buf[len - 1] = '\0';
+        STP_REQUIRE(t, "stp_qpgraph_topology_label: null argument");
+        topology_label(t->t, kind, index, buf, len);
     });
+}
+
+// ---- topology enumeration and search ----------------------------------------------------
+int stp_qpgraph_enumerate(int32_t n_leaves, const char* const* leaves, int32_t max_nadmix, stp_qpgraph_list** out) {
+    return guard([&] {
+        STP_REQUIRE(out, "stp_qpgraph_enumerate: null output");
+        STP_REQUIRE(n_leaves >= 1 && n_leaves <= 9, "stp_qpgraph_enumerate: 1..9 leaves");
+        auto l = std::make_unique<stp_qpgraph_list>();
+        l->c = stp::qgenum::enumerate_space(label_list(n_leaves, leaves), max_nadmix);
+        *out = l.release();
+    });
+}
+
+int stp_qpgraph_neighbors(int32_t n_edges, const char* const* from, const char* const* to, int32_t nadmix,
+                          uint64_t hash, int32_t n_leaves, const char* const* leaves, int32_t max_nadmix,
+                          stp_qpgraph_list** out) {
+    return guard([&] {
+        STP_REQUIRE(out, "stp_qpgraph_neighbors: null output");
+        STP_REQUIRE(nadmix == 0 || nadmix == 1, "stp_qpgraph_neighbors: moves are defined for 0 or 1 admixture events");
+        stp::qgenum::Candidate cur;
+        cur.edges = edge_list(n_edges, from, to);
+        cur.nadmix = nadmix;
+        cur.hash = hash;
+        auto l = std::make_unique<stp_qpgraph_list>();
+        l->c = stp::qgenum::neighbors(cur, label_list(n_leaves, leaves), max_nadmix);
+        *out = l.release();
+    });
+}
+
+int stp_qpgraph_graph_hash(int32_t n_edges, const char* const* from, const char* const* to, uint64_t* hash) {
+    return guard([&] {
+        STP_REQUIRE(hash, "stp_qpgraph_graph_hash: null output");
+        *hash = stp::qgenum::graph_hash(edge_list(n_edges, from, to));
+    });
+}
+
+int stp_qpgraph_list_size(const stp_qpgraph_list* l, int32_t* n) {
+    return guard([&] {
+        STP_REQUIRE(l && n, "stp_qpgraph_list_size: null argument");
+        *n = static_cast<int32_t>(l->c.size());
+    });
+}
+
+int stp_qpgraph_list_get(const stp_qpgraph_list* l, int32_t i, int32_t* nadmix, int32_t* id, uint64_t* hash,
+                         int32_t* n_edges) {
+    return guard([&] {
+        STP_REQUIRE(l, "stp_qpgraph_list_get: null list");
+        STP_REQUIRE(i >= 0 && i < static_cast<int32_t>(l->c.size()), "stp_qpgraph_list_get: index out of range");
+        const auto& c = l->c[i];
+        if (nadmix) *nadmix = c.nadmix;
+        if (id) *id = c.id;
+        if (hash) *hash = c.hash;
+        if (n_edges) *n_edges = static_cast<int32_t>(c.edges.size());
+    });
+}
+
+int stp_qpgraph_list_edge(const stp_qpgraph_list* l, int32_t i, int32_t k, char* from, char* to, int32_t len) {
+    return guard([&] {
+        STP_REQUIRE(l && from && to && len > 0, "stp_qpgraph_list_edge: null argument");
+        STP_REQUIRE(i >= 0 && i < static_cast<int32_t>(l->c.size()), "stp_qpgraph_list_edge: index out of range");
+        const auto& e = l->c[i].edges;
+        STP_REQUIRE(k >= 0 && k < static_cast<int32_t>(e.size()), "stp_qpgraph_list_edge: edge out of range");
+        std::strncpy(from, e[k].first.c_str(), static_cast<std::size_t>(len - 1));
+        from[len - 1] = '\0';
+        std::strncpy(to, e[k].second.c_str(), static_cast<std::size_t>(len - 1));
+        to[len - 1] = '\0';
+    });
+}
+
+int stp_qpgraph_list_free(stp_qpgraph_list* l) {
+    return guard([&] { delete l; });
+}
+
+int stp_qpgraph_search_run(stp_context* ctx, const stp_f2_handle* f2, int32_t n_names, const char* const* names,
+                           int32_t n_pops, const char* const* pops, const stp_qpgraph_search_options* options,
+                           stp_qpgraph_search** out) {
+    return guard([&] {
+        STP_REQUIRE(out && f2 && f2->blocks && options, "stp_qpgraph_search_run: null argument");
+        STP_REQUIRE(f2->ctx == ctx, "stp_qpgraph_search_run: the f2 handle belongs to another context");
+        STP_REQUIRE(n_pops <= 8, "qpgraph-search: at most 8 populations (the bounded space grows factorially)");
+        STP_REQUIRE(options->numstart >= 1 && options->maxit >= 0 && options->tol >= 0.0,
+                    "qpgraph-search: numstart must be >= 1, maxit and tol >= 0");
+        stp::QgSearchOptions o;
+        o.max_nadmix = options->max_nadmix;
+        o.numstart = options->numstart;
+        o.diag_f3 = options->diag_f3;
+        o.fit.maxit = options->maxit;
+        o.fit.gtol = options->tol * 1e-2;
+        o.fit.xtol = options->tol * 1e-3;
+        o.fit.fudge = options->fudge;
+        o.fit.constrained = options->constrained != 0;
+        o.run_heuristic = options->run_heuristic != 0;
+        o.heuristic_seeds = options->heuristic_seeds;
+        auto s = std::make_unique<stp_qpgraph_search>();
+        s->r = backend(ctx).qpgraph_search(*f2->blocks, label_list(n_names, names), label_list(n_pops, pops), o);
+        s->cands.c = s->r.cands;
+        *out = s.release();
+    });
+}
+
+int stp_qpgraph_search_get_summary(const stp_qpgraph_search* s, stp_qpgraph_search_summary* out) {
+    return guard([&] {
+        STP_REQUIRE(s && out, "stp_qpgraph_search_get_summary: null argument");
+        const auto& r = s->r;
+        stp_qpgraph_search_summary m{};
+        m.status = r.status;
+        m.n_trees = r.n_trees;
+        m.n_admix1 = r.n_admix1;
+        m.n_candidates = r.n_candidates;
+        m.n_fitted = static_cast<int32_t>(r.fitted.size());
+        m.best = r.best;
+        m.best_candidate = r.best >= 0 ? r.fitted[r.best] : -1;
+        m.best_nadmix = r.best >= 0 ? r.cands[r.fitted[r.best]].nadmix : 0;
+        m.best_hash = r.best >= 0 ? r.cands[r.fitted[r.best]].hash : 0;
+        m.best_score = r.best_score;
+        m.second_best_score = r.second_best_score;
+        m.heuristic_recovered = r.heuristic_recovered ? 1 : 0;
+        m.n_seeds = static_cast<int32_t>(r.seed_hashes.size());
+        m.fit_all_wall_ms = r.fit_all_wall_ms;
+        m.topologies_per_s = r.topologies_per_s;
+        *out = m;
+    });
+}
+
+const stp_qpgraph_list* stp_qpgraph_search_candidates(const stp_qpgraph_search* s) { return s ? &s->cands : nullptr; }
+
+int stp_qpgraph_search_scores(const stp_qpgraph_search* s, int32_t* candidate, double* score, double* restart_spread) {
+    return guard([&] {
+        STP_REQUIRE(s, "stp_qpgraph_search_scores: null search");
+        const auto& r = s->r;
+        for (std::size_t k = 0; k < r.fitted.size(); ++k) {
+            if (candidate) candidate[k] = r.fitted[k];
+            if (score) score[k] = k < r.score.size() ? r.score[k] : std::nan("");
+            if (restart_spread) restart_spread[k] = k < r.restart_spread.size() ? r.restart_spread[k] : 0.0;
+        }
+    });
+}
+
+int stp_qpgraph_search_seeds(const stp_qpgraph_search* s, uint64_t* hashes) {
+    return guard([&] {
+        STP_REQUIRE(s && hashes, "stp_qpgraph_search_seeds: null argument");
+        std::copy(s->r.seed_hashes.begin(), s->r.seed_hashes.end(), hashes);
+    });
+}
+
+int stp_qpgraph_search_best_fit(const stp_qpgraph_search* s, int32_t* dims, double* weight, double* weight_lo,
+                                double* weight_hi, double* edge_length, double* f3_fit, double* f_obs,
+                                double* qdiag) {
+    return guard([&] {
+        STP_REQUIRE(s, "stp_qpgraph_search_best_fit: null search");
+        const auto& r = s->r;
+        STP_REQUIRE(r.best >= 0, "qpgraph-search: no graph was fitted");
+        const auto& m = r.best_topo;
+        if (dims) {
+            const int32_t d[6] = {m.npop, m.ne, m.nadmix, m.npair, m.npath, m.base};
+            std::copy(d, d + 6, dims);
+        }
+        auto put = [](double* dst, const std::vector<double>& v) {
+            if (dst) std::copy(v.begin(), v.end(), dst);
+        };
+        put(weight, r.theta);
+        put(weight_lo, r.theta_lo);
+        put(weight_hi, r.theta_hi);
+        put(edge_length, r.edge_length);
+        put(f3_fit, r.f3_fit);
+        put(f_obs, r.f_obs);
+        put(qdiag, r.qdiag);
+    });
+}
+
+int stp_qpgraph_search_best_label(const stp_qpgraph_search* s, int32_t kind, int32_t index, char* buf, int32_t len) {
+    return guard([&] {
+        STP_REQUIRE(s && s->r.best >= 0, "stp_qpgraph_search_best_label: no best graph");
+        topology_label(s->r.best_topo, kind, index, buf, len);
+    });
+}
+
+int stp_qpgraph_search_free(stp_qpgraph_search* s) {
+    return guard([&] { delete s; });
 }
 
 int stp_qpgraph_set_create(stp_context* ctx, int32_t n_graphs, const stp_qpgraph_topology* const* topologies,

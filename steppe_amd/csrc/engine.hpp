@@ -20,6 +20,7 @@
 #include "fstats.hpp"
 #include "geno_io.hpp"
 #include "qpgraph.hpp"
+#include "qpgraph_enum.hpp"
 #include "steppe_hip.h"
 
 namespace stp {
@@ -143,6 +144,7 @@ struct ExtractInfo {
 struct QgSet {
     int n = 0, np_max = 0, ne_max = 0, npop_max = 0, npath_max = 0, dmax = 0;
     std::vector<QgTopology> topo;
+    std::vector<std::int32_t> slot;   // graph -> its basis slot in fobs / qdiag / qinv
     std::vector<std::int32_t> host_status;
     DevBuf<QgTopoDev> d_topo;
     DevBuf<std::int32_t> d_iarena, status;
@@ -190,6 +192,38 @@ struct QgFitOptions {
 struct QgFleetOut {
     std::vector<double> score, restart_spread, theta, theta_lo, theta_hi, edge_length, f3_fit;
     std::vector<std::int32_t> status, iterations;
+};
+
+// Options of the topology search (the reference's QpGraphSearchOptions,
+// include/steppe/qpgraph_search.hpp:30-36, with the fit options of QpGraphOptions).
+struct QgSearchOptions {
+    int max_nadmix = 1;          // 0: trees only; larger values act as 1 (the bounded space)
+    int numstart = 10;
+    double diag_f3 = 1e-5;
+    QgFitOptions fit;
+    bool run_heuristic = true;
+    int heuristic_seeds = 8;
+};
+
+// Result of the topology search (the reference's QpGraphSearchResult,
+// include/steppe/qpgraph_search.hpp:48-72).  cands is the whole enumeration; fitted[k]
+// is the candidate index of fitted graph k, whose best score and restart spread are
+// score[k] and restart_spread[k]; best indexes fitted graphs.  status: 0 ok, 3
+// non_spd_covariance (the shared basis covariance or every fit failed).
+struct QgSearchResult {
+    int n_trees = 0, n_admix1 = 0, n_candidates = 0;
+    std::vector<qgenum::Candidate> cands;
+    std::vector<int> fitted;
+    std::vector<double> score, restart_spread;
+    int best = -1;
+    double best_score = 0.0, second_best_score = 0.0;
+    QgTopology best_topo;
+    int best_fit_status = 3;
+    std::vector<double> theta, theta_lo, theta_hi, edge_length, f3_fit, f_obs, qdiag;
+    bool heuristic_recovered = false;
+    std::vector<std::uint64_t> seed_hashes;
+    double fit_all_wall_ms = 0.0, topologies_per_s = 0.0;
+    int status = 0;
 };
 
 class HipBackend {
@@ -247,6 +281,12 @@ public:
     // npair values, then npair x npair row-major), as the reference's fleet fit receives.
     QgSet* qpgraph_set_from_basis(const std::vector<const QgTopology*>& topos, const double* f_obs,
                                   const double* qinv);
+    // Graphs that all use one f3 basis (one npair for all): one f_obs / Qinv / Q diagonal.
+    QgSet* qpgraph_set_shared_basis(const std::vector<const QgTopology*>& topos, const double* f_obs,
+                                    const double* qinv, const double* qdiag);
+    // The bounded topology search over pops (qpgraph_search.cpp); names are the f2 labels.
+    QgSearchResult qpgraph_search(const F2Blocks& f2, const std::vector<std::string>& names,
+                                  const std::vector<std::string>& pops, const QgSearchOptions& o);
     void qpgraph_set_eval(QgSet& s, int n_inst, const std::int32_t* inst_graph, const double* theta,
                           const double* bl_in, int mode, double fudge, int constrained, double* score,
                           double* grad_theta, double* grad_bl, double* bl_out, double* fit_out);

@@ -29,9 +29,10 @@ struct QgTopology {
 };
 
 // Parses a parent->child edge list against population labels; throws ArgError with
-// the reference's messages for invalid graphs (qpgraph_model.cpp:41-243).
+// the reference's messages for invalid graphs (qpgraph_model.cpp:41-243).  base_leaf
+// names the f3 base population (the reference's f3basepop); empty: the first leaf.
 QgTopology parse_qpgraph(const std::vector<std::pair<std::string, std::string>>& edges,
-                         const std::vector<std::string>& names);
+                         const std::vector<std::string>& names, const std::string& base_leaf = "");
 
 // Device header of one topology inside the int32 / double arenas.
 struct QgTopoDev {

@@ -27,7 +27,7 @@ namespace stp {
 int QgTopology::col_to_leaf(int c) const { return c < base ? c : c + 1; }
 
 QgTopology parse_qpgraph(const std::vector<std::pair<std::string, std::string>>& edges,
-                         const std::vector<std::string>& names) {
+                         const std::vector<std::string>& names, const std::string& base_leaf) {
     STP_REQUIRE(!edges.empty(), "qpgraph: empty edge list");
     QgTopology t;
     std::vector<std::string> node;
@@ -77,6 +77,11 @@ QgTopology parse_qpgraph(const std::vector<std::pair<std::string, std::string>>&
         t.leaf_to_f2.push_back(it->second);
     }
     t.base = 0;
+    if (!base_leaf.empty()) {
+        const auto it = std::find(t.leaves.begin(), t.leaves.end(), base_leaf);
+        STP_REQUIRE(it != t.leaves.end(), "qpgraph: f3basepop '" + base_leaf + "' is not a leaf");
+        t.base = static_cast<int>(it - t.leaves.begin());
+    }
     // admixture edges and normal edges
     std::vector<int> admix_id(E, 0), normal_id(E, -1);
     for (int v = 0; v < V; ++v)
@@ -221,6 +226,7 @@ void HipBackend::qpgraph_upload_topologies(QgSet& s, const std::vector<const QgT
         set->npath_max = std::max(set->npath_max, t.npath);
         set->dmax = std::max(set->dmax, t.nadmix);
         set->topo.push_back(t);
+        set->slot.push_back(g);
     }
     set->d_topo.alloc(n);
     set->d_iarena.alloc(std::max<std::size_t>(ia.size(), 1));
@@ -355,6 +361,31 @@ QgSet* HipBackend::qpgraph_set_from_basis(const std::vector<const QgTopology*>& 
     return set.release();
 }
 
+QgSet* HipBackend::qpgraph_set_shared_basis(const std::vector<const QgTopology*>& topos, const double* f_obs,
+                                            const double* qinv, const double* qdiag) {
+    activate();
+    const int m = topos.empty() ? 0 : topos[0]->npair;
+    for (const QgTopology* t : topos)
+        STP_REQUIRE(t->npair == m, "qpgraph: graphs sharing one f3 basis need the same number of basis pairs");
+    auto set = std::make_unique<QgSet>();
+    qpgraph_upload_topologies(*set, topos);
+    const int n = set->n;
+    set->slot.assign(n, 0);
+    const std::size_t mm = static_cast<std::size_t>(m) * m;
+    set->fobs.alloc(m);
+    set->qdiag.alloc(m);
+    set->qinv.alloc(mm);
+    set->status.alloc(n);
+    STP_HIP(hipMemcpyAsync(set->fobs.get(), f_obs, m * sizeof(double), hipMemcpyHostToDevice, stream_));
+    STP_HIP(hipMemcpyAsync(set->qinv.get(), qinv, mm * sizeof(double), hipMemcpyHostToDevice, stream_));
+    std::vector<double> zeros(m, 0.0);
+    STP_HIP(hipMemcpyAsync(set->qdiag.get(), qdiag ? qdiag : zeros.data(), m * sizeof(double),
+                           hipMemcpyHostToDevice, stream_));
+    sync();
+    set->host_status.assign(n, 0);
+    return set.release();
+}
+
 void HipBackend::qpgraph_set_eval(QgSet& s, int n_inst, const std::int32_t* inst_graph, const double* theta,
                                   const double* bl_in, int mode, double fudge, int constrained, double* score,
                                   double* grad_theta, double* grad_bl, double* bl_out, double* fit_out) {
@@ -368,8 +399,8 @@ void HipBackend::qpgraph_set_eval(QgSet& s, int n_inst, const std::int32_t* inst
         const int g = inst_graph[i];
         STP_REQUIRE(g >= 0 && g < s.n, "qpgraph: instance graph index out of range");
         STP_REQUIRE(s.host_status[g] == 0, "qpgraph: graph " + std::to_string(g) + " has no valid covariance");
-        offs[i] = static_cast<std::int64_t>(g) * NP;
-        offs[n_inst + i] = static_cast<std::int64_t>(g) * NP * NP;
+        offs[i] = static_cast<std::int64_t>(s.slot[g]) * NP;
+        offs[n_inst + i] = static_cast<std::int64_t>(s.slot[g]) * NP * NP;
         T = std::max(T, ne_bucket(s.topo[g].ne));
     }
     if (mode == 2) STP_REQUIRE(bl_in != nullptr, "qpgraph: the joint mode needs edge lengths");
@@ -471,8 +502,8 @@ void HipBackend::qpgraph_set_fit(QgSet& s, int n_inst, const std::int32_t* inst_
         const int g = inst_graph[i];
         STP_REQUIRE(g >= 0 && g < s.n, "qpgraph: instance graph index out of range");
         STP_REQUIRE(s.host_status[g] == 0, "qpgraph: graph " + std::to_string(g) + " has no valid covariance");
-        offs[i] = static_cast<std::int64_t>(g) * NP;
-        offs[n_inst + i] = static_cast<std::int64_t>(g) * NP * NP;
+        offs[i] = static_cast<std::int64_t>(s.slot[g]) * NP;
+        offs[n_inst + i] = static_cast<std::int64_t>(s.slot[g]) * NP * NP;
         T = std::max(T, ne_bucket(s.topo[g].ne));
     }
     s.i32.ensure(2 * static_cast<std::size_t>(n_inst));
@@ -540,17 +571,85 @@ void HipBackend::qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOption
     out.theta_hi.assign(static_cast<std::size_t>(n) * D, 0.0);
     out.edge_length.assign(static_cast<std::size_t>(n) * NE, 0.0);
     out.f3_fit.assign(static_cast<std::size_t>(n) * NP, 0.0);
-    // restarts of every fittable graph, the reference's start points
-    std::vector<std::int32_t> ig, first(n + 1, 0);
+    // Grid seeds: graphs with one or two admixture weights are also scored on a coarse
+    // grid over [0, 1] that includes both bounds (kGrid1 points, or kGrid2 x kGrid2) with
+    // their gradients.  Seeds are the grid's local minima (no higher than their grid
+    // neighbours) and the lower end of each grid interval over which a partial
+    // derivative changes sign from - to + (a minimum between two grid points); the
+    // kGridSeeds lowest become more L-BFGS starts after the reference's.  A narrow basin
+    // that none of the reference's start points falls in is then still found, and so is a
+    // minimum at a bound behind a barrier (the score can drop at exactly theta = 0 or 1,
+    // where a path's weight vanishes).
+    constexpr int kGrid1 = 33, kGrid2 = 9, kGridSeeds = 4;
+    std::vector<std::int32_t> gi, g0(n, -1);
+    std::vector<double> gx;
+    for (int g = 0; g < n; ++g) {
+        const int d = s.topo[g].nadmix;
+        if (s.host_status[g] != 0 || d < 1 || d > 2) continue;
+        const int k = d == 1 ? kGrid1 : kGrid2;
+        g0[g] = static_cast<std::int32_t>(gi.size());
+        for (int a = 0; a < k; ++a)
+            for (int b = 0; b < (d == 2 ? k : 1); ++b) {
+                gi.push_back(g);
+                for (int j = 0; j < D; ++j)
+                    gx.push_back(j == 0 ? a / (k - 1.0) : (j == 1 && d == 2 ? b / (k - 1.0) : 0.0));
+            }
+    }
+    std::vector<std::vector<std::pair<double, int>>> grid_best(n);
+    if (!gi.empty()) {
+        const int ng = static_cast<int>(gi.size());
+        std::vector<double> gsc(ng), ggr(static_cast<std::size_t>(ng) * D);
+        qpgraph_set_eval(s, ng, gi.data(), gx.data(), nullptr, 1, o.fudge, o.constrained, gsc.data(), ggr.data(),
+                         nullptr, nullptr, nullptr);
+        for (int g = 0; g < n; ++g) {
+            if (g0[g] < 0) continue;
+            const int d = s.topo[g].nadmix, k = d == 1 ? kGrid1 : kGrid2, rows = d == 2 ? k : 1;
+            auto id = [&](int a, int b) { return g0[g] + a * rows + b; };
+            auto at = [&](int a, int b) { return gsc[id(a, b)]; };
+            auto grad = [&](int a, int b, int j) { return ggr[static_cast<std::size_t>(id(a, b)) * D + j]; };
+            std::vector<char> seed(static_cast<std::size_t>(k) * rows, 0);
+            for (int a = 0; a < k; ++a)
+                for (int b = 0; b < rows; ++b) {
+                    const double v = at(a, b);
+                    if (!std::isfinite(v)) continue;
+                    bool low = true;
+                    if (a > 0 && std::isfinite(at(a - 1, b)) && at(a - 1, b) < v) low = false;
+                    if (a + 1 < k && std::isfinite(at(a + 1, b)) && at(a + 1, b) < v) low = false;
+                    if (b > 0 && std::isfinite(at(a, b - 1)) && at(a, b - 1) < v) low = false;
+                    if (b + 1 < rows && std::isfinite(at(a, b + 1)) && at(a, b + 1) < v) low = false;
+                    if (low) seed[a * rows + b] = 1;
+                    // a sign change of d/dtheta_j between this point and the next along j
+                    for (int j = 0; j < d; ++j) {
+                        const int a2 = a + (j == 0), b2 = b + (j == 1);
+                        if (a2 >= k || b2 >= rows || !std::isfinite(at(a2, b2))) continue;
+                        if (grad(a, b, j) < 0.0 && grad(a2, b2, j) > 0.0)
+                            seed[at(a2, b2) < v ? a2 * rows + b2 : a * rows + b] = 1;
+                    }
+                }
+            auto& v = grid_best[g];
+            for (int x = 0; x < k * rows; ++x)
+                if (seed[x]) v.emplace_back(gsc[g0[g] + x], g0[g] + x);
+            std::stable_sort(v.begin(), v.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+            if (v.size() > kGridSeeds) v.resize(kGridSeeds);
+        }
+    }
+    // restarts of every fittable graph: the reference's start points, then the grid seeds
+    std::vector<std::int32_t> ig, first(n + 1, 0), n_ref(n, 0);
     std::vector<double> x0;
     for (int g = 0; g < n; ++g) {
         first[g] = static_cast<std::int32_t>(ig.size());
         if (s.host_status[g] != 0) continue;
         const int d = s.topo[g].nadmix;
         const int rs = d > 0 ? numstart : 1;
+        n_ref[g] = rs;
         for (int r = 0; r < rs; ++r) {
             ig.push_back(g);
             for (int j = 0; j < D; ++j) x0.push_back(j < d ? multistart_theta(static_cast<unsigned>(r), j) : 0.0);
+        }
+        for (const auto& [sc0, i] : grid_best[g]) {
+            ig.push_back(g);
+            x0.insert(x0.end(), gx.begin() + static_cast<std::ptrdiff_t>(i) * D,
+                      gx.begin() + static_cast<std::ptrdiff_t>(i + 1) * D);
         }
     }
     first[n] = static_cast<std::int32_t>(ig.size());
@@ -559,18 +658,22 @@ void HipBackend::qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOption
     std::vector<double> th(static_cast<std::size_t>(m) * D), sc(m);
     std::vector<std::int32_t> iters(m);
     qpgraph_set_fit(s, m, ig.data(), x0.data(), o, th.data(), sc.data(), iters.data());
-    // best restart per graph (first of equal scores), spread and per-coordinate range
+    // best restart per graph (first of equal scores, over every start); the spread and
+    // per-coordinate range are over the reference's restarts, as it reports them
     std::vector<std::int32_t> best_g;
     std::vector<double> best_th;
     for (int g = 0; g < n; ++g) {
         if (first[g] == first[g + 1]) continue;
         const int d = s.topo[g].nadmix;
+        const int ref_end = first[g] + n_ref[g];
         int b = -1;
         double smin = std::numeric_limits<double>::infinity(), smax = -std::numeric_limits<double>::infinity();
         int it = 0;
         for (int i = first[g]; i < first[g + 1]; ++i) {
-            smin = std::min(smin, sc[i]);
-            smax = std::max(smax, sc[i]);
+            if (i < ref_end) {
+                smin = std::min(smin, sc[i]);
+                smax = std::max(smax, sc[i]);
+            }
             it = std::max(it, iters[i]);
             if (b < 0 || sc[i] < sc[b]) b = i;
         }
@@ -578,7 +681,7 @@ void HipBackend::qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOption
         out.restart_spread[g] = (std::isfinite(smin) && std::isfinite(smax)) ? smax - smin : 0.0;
         for (int j = 0; j < d; ++j) {
             double lo = 1.0, hi = 0.0;
-            for (int i = first[g]; i < first[g + 1]; ++i) {
+            for (int i = first[g]; i < ref_end; ++i) {
                 lo = std::min(lo, th[static_cast<std::size_t>(i) * D + j]);
                 hi = std::max(hi, th[static_cast<std::size_t>(i) * D + j]);
             }
@@ -611,14 +714,13 @@ void HipBackend::qpgraph_set_basis(const QgSet& s, int g, double* f_obs, double*
     activate();
     STP_REQUIRE(g >= 0 && g < s.n, "qpgraph: graph index out of range");
     const int np = s.topo[g].npair, NP = s.np_max;
+    const std::size_t k = static_cast<std::size_t>(s.slot[g]);
     if (f_obs)
-        STP_HIP(hipMemcpyAsync(f_obs, s.fobs.get() + static_cast<std::size_t>(g) * NP, np * sizeof(double),
-                               hipMemcpyDeviceToHost, stream_));
+        STP_HIP(hipMemcpyAsync(f_obs, s.fobs.get() + k * NP, np * sizeof(double), hipMemcpyDeviceToHost, stream_));
     if (qdiag)
-        STP_HIP(hipMemcpyAsync(qdiag, s.qdiag.get() + static_cast<std::size_t>(g) * NP, np * sizeof(double),
-                               hipMemcpyDeviceToHost, stream_));
+        STP_HIP(hipMemcpyAsync(qdiag, s.qdiag.get() + k * NP, np * sizeof(double), hipMemcpyDeviceToHost, stream_));
     if (qinv)
-        STP_HIP(hipMemcpyAsync(qinv, s.qinv.get() + static_cast<std::size_t>(g) * NP * NP,
+        STP_HIP(hipMemcpyAsync(qinv, s.qinv.get() + k * NP * NP,
                                static_cast<std::size_t>(np) * np * sizeof(double), hipMemcpyDeviceToHost, stream_));
     sync();
 }

@@ -13,6 +13,10 @@
   the reference's splitmix multistart points.
 * ``run_qpgraph``: the reference's single-graph fit (bindings/bind_qpgraph.cpp:14-36) with
   the finite-difference coordinate search replaced by that device L-BFGS.
+* ``enumerate_graphs`` / ``graph_hash`` / ``topology_neighbors``: the bounded topology
+  space and its moves (src/core/qpadm/qpgraph_enumerate.cpp), and ``run_qpgraph_search``:
+  every candidate fitted in one device fleet against one shared f3 basis
+  (bindings/bind_qpgraph.cpp:41-95, src/core/qpadm/qpgraph_search.cpp).
 """
 from __future__ import annotations
 
@@ -23,6 +27,22 @@ import numpy as np
 from . import _core
 
 _SIG_DONE = False
+
+
+class SearchOptions(C.Structure):
+    """stp_qpgraph_search_options (include/steppe_hip.h)."""
+    _fields_ = [("max_nadmix", C.c_int32), ("numstart", C.c_int32), ("maxit", C.c_int32), ("tol", C.c_double),
+                ("fudge", C.c_double), ("diag_f3", C.c_double), ("constrained", C.c_int32),
+                ("run_heuristic", C.c_int32), ("heuristic_seeds", C.c_int32)]
+
+
+class SearchSummary(C.Structure):
+    """stp_qpgraph_search_summary (include/steppe_hip.h)."""
+    _fields_ = [("status", C.c_int32), ("n_trees", C.c_int32), ("n_admix1", C.c_int32),
+                ("n_candidates", C.c_int32), ("n_fitted", C.c_int32), ("best", C.c_int32),
+                ("best_candidate", C.c_int32), ("best_nadmix", C.c_int32), ("best_hash", C.c_uint64),
+                ("best_score", C.c_double), ("second_best_score", C.c_double), ("heuristic_recovered", C.c_int32),
+                ("n_seeds", C.c_int32), ("fit_all_wall_ms", C.c_double), ("topologies_per_s", C.c_double)]
 
 
 class FitOptions(C.Structure):
@@ -50,6 +70,22 @@ def _lib():
             "stp_qpgraph_set_create_basis": (C.c_int, [vp, i32, vp, vp, vp, C.POINTER(vp)]),
             "stp_qpgraph_fit": (C.c_int, [vp, vp, i32, vp, vp, C.POINTER(FitOptions), vp, vp, vp]),
             "stp_qpgraph_fit_fleet": (C.c_int, [vp, vp, C.POINTER(FitOptions)] + [vp] * 9),
+            "stp_qpgraph_enumerate": (C.c_int, [i32, vp, i32, C.POINTER(vp)]),
+            "stp_qpgraph_neighbors": (C.c_int, [i32, vp, vp, i32, C.c_uint64, i32, vp, i32, C.POINTER(vp)]),
+            "stp_qpgraph_graph_hash": (C.c_int, [i32, vp, vp, C.POINTER(C.c_uint64)]),
+            "stp_qpgraph_list_size": (C.c_int, [vp, C.POINTER(i32)]),
+            "stp_qpgraph_list_get": (C.c_int, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_uint64),
+                                              C.POINTER(i32)]),
+            "stp_qpgraph_list_edge": (C.c_int, [vp, i32, i32, C.c_char_p, C.c_char_p, i32]),
+            "stp_qpgraph_list_free": (C.c_int, [vp]),
+            "stp_qpgraph_search_run": (C.c_int, [vp, vp, i32, vp, i32, vp, C.POINTER(SearchOptions), C.POINTER(vp)]),
+            "stp_qpgraph_search_get_summary": (C.c_int, [vp, C.POINTER(SearchSummary)]),
+            "stp_qpgraph_search_candidates": (vp, [vp]),
+            "stp_qpgraph_search_scores": (C.c_int, [vp, vp, vp, vp]),
+            "stp_qpgraph_search_seeds": (C.c_int, [vp, vp]),
+            "stp_qpgraph_search_best_fit": (C.c_int, [vp] + [vp] * 8),
+            "stp_qpgraph_search_best_label": (C.c_int, [vp, i32, i32, C.c_char_p, i32]),
+            "stp_qpgraph_search_free": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -353,3 +389,134 @@ def run_qpgraph(f2, edges, numstart=10, fudge=1e-4, diag_f3=1e-5, constrained=Tr
     return dict(base, status="ok", score=score, restart_spread=fit["restart_spread"],
                 weight=fit["theta"].tolist(), weight_lo=fit["theta_lo"].tolist(), weight_hi=fit["theta_hi"].tolist(),
                 edge_length=fit["edge_length"].tolist(), worst_residual_z=worst, worst_pop2=p2, worst_pop3=p3)
+
+
+# ---------------------------------------------------------------------------------------
+# topology enumeration and search
+# ---------------------------------------------------------------------------------------
+def _read_list(h):
+    """stp_qpgraph_list -> [{"nadmix", "id", "hash", "edges"}], in list order."""
+    L = _lib()
+    n = C.c_int32()
+    _core.check(L.stp_qpgraph_list_size(h, C.byref(n)))
+    out = []
+    na, gid, ne = C.c_int32(), C.c_int32(), C.c_int32()
+    hv = C.c_uint64()
+    fb, tb = C.create_string_buffer(256), C.create_string_buffer(256)
+    for i in range(n.value):
+        _core.check(L.stp_qpgraph_list_get(h, i, C.byref(na), C.byref(gid), C.byref(hv), C.byref(ne)))
+        edges = []
+        for k in range(ne.value):
+            _core.check(L.stp_qpgraph_list_edge(h, i, k, fb, tb, 256))
+            edges.append((fb.value.decode(), tb.value.decode()))
+        out.append({"nadmix": int(na.value), "id": int(gid.value), "hash": int(hv.value), "edges": edges})
+    return out
+
+
+def _taking_list(fn, *args):
+    L = _lib()
+    h = C.c_void_p()
+    _core.check(fn(*args, C.byref(h)), "qpgraph")
+    try:
+        return _read_list(h)
+    finally:
+        L.stp_qpgraph_list_free(h)
+
+
+def enumerate_graphs(leaves, max_nadmix=1):
+    """Every rooted tree on ``leaves`` then every distinct one-admixture graph
+    (enumerate_bounded_space, src/core/qpadm/qpgraph_enumerate.cpp:229-236): dicts with
+    nadmix, id, hash and edges, in the reference's order."""
+    leaves = [str(x) for x in leaves]
+    return _taking_list(_lib().stp_qpgraph_enumerate, len(leaves), _cstrs(leaves), int(max_nadmix))
+
+
+def graph_hash(edges):
+    """Shape hash of an edge list (graph_hash, src/core/qpadm/qpgraph_enumerate.cpp:183-202)."""
+    edges = [(str(a), str(b)) for a, b in edges]
+    h = C.c_uint64()
+    _core.check(_lib().stp_qpgraph_graph_hash(len(edges), _cstrs([a for a, _ in edges]), _cstrs([b for _, b in edges]),
+                                              C.byref(h)), "graph_hash")
+    return int(h.value)
+
+
+def topology_neighbors(graph, leaves, max_nadmix=1):
+    """Hill-climb moves from ``graph`` (a dict from enumerate_graphs, or one with edges,
+    nadmix and hash) (topology_neighbors, src/core/qpadm/qpgraph_enumerate.cpp:383-407)."""
+    edges = [(str(a), str(b)) for a, b in graph["edges"]]
+    leaves = [str(x) for x in leaves]
+    hv = graph.get("hash")
+    if hv is None:
+        hv = graph_hash(edges)
+    return _taking_list(_lib().stp_qpgraph_neighbors, len(edges), _cstrs([a for a, _ in edges]),
+                        _cstrs([b for _, b in edges]), int(graph["nadmix"]), C.c_uint64(int(hv)), len(leaves),
+                        _cstrs(leaves), int(max_nadmix))
+
+
+def run_qpgraph_search(f2, pops, max_nadmix=1, numstart=10, fudge=1e-4, diag_f3=1e-5, constrained=True,
+                       run_heuristic=True, *, heuristic_seeds=8, maxit=200, tol=1e-9, details=False) -> dict:
+    """The topology search (bindings/bind_qpgraph.cpp:41-95): every candidate on ``pops``
+    fitted in one device fleet against one f3 basis f3(pops[0]; a, b) shared by all
+    candidates.  Returns the reference's dict (n_trees, n_admix1, n_candidates,
+    best_score, second_best_score, best_nadmix, best_hash, best_edges,
+    heuristic_recovered, fit_all_wall_ms, topologies_per_s, cand_nadmix, cand_hash,
+    cand_score, cand_restart_spread) plus cand_id and seed_hashes; ``details=True`` adds
+    cand_edges and best_fit (the best graph's run_qpgraph-style result)."""
+    pops = [str(p) for p in pops]
+    if len(pops) < 3:
+        raise ValueError("qpgraph-search: need >= 3 population labels")
+    L = _lib()
+    names = list(f2.pops)
+    opt = SearchOptions(int(max_nadmix), int(numstart), int(maxit), float(tol), float(fudge), float(diag_f3),
+                        1 if constrained else 0, 1 if run_heuristic else 0, int(heuristic_seeds))
+    ctx = _core.Context.get(f2.device)
+    h = C.c_void_p()
+    _core.check(L.stp_qpgraph_search_run(ctx.h, f2.raw, len(names), _cstrs(names), len(pops), _cstrs(pops),
+                                         C.byref(opt), C.byref(h)), "qpgraph-search")
+    try:
+        sm = SearchSummary()
+        _core.check(L.stp_qpgraph_search_get_summary(h, C.byref(sm)))
+        cands = _read_list(L.stp_qpgraph_search_candidates(h))
+        nf = int(sm.n_fitted)
+        idx = np.zeros(nf, np.int32)
+        score = np.zeros(nf)
+        spread = np.zeros(nf)
+        _core.check(L.stp_qpgraph_search_scores(h, _p(idx), _p(score), _p(spread)))
+        seeds = np.zeros(max(int(sm.n_seeds), 1), np.uint64)
+        _core.check(L.stp_qpgraph_search_seeds(h, _p(seeds)))
+        d = {"status": "ok" if sm.status == 0 else "non_spd_covariance",
+             "n_trees": int(sm.n_trees), "n_admix1": int(sm.n_admix1), "n_candidates": int(sm.n_candidates),
+             "best_score": float(sm.best_score), "second_best_score": float(sm.second_best_score),
+             "best_nadmix": int(sm.best_nadmix), "best_hash": int(sm.best_hash),
+             "best_edges": list(cands[sm.best_candidate]["edges"]) if sm.best_candidate >= 0 else [],
+             "heuristic_recovered": bool(sm.heuristic_recovered), "fit_all_wall_ms": float(sm.fit_all_wall_ms),
+             "topologies_per_s": float(sm.topologies_per_s),
+             "cand_nadmix": [cands[i]["nadmix"] for i in idx], "cand_hash": [cands[i]["hash"] for i in idx],
+             "cand_id": [cands[i]["id"] for i in idx], "cand_score": score.tolist(),
+             "cand_restart_spread": spread.tolist(), "seed_hashes": [int(x) for x in seeds[:int(sm.n_seeds)]]}
+        if details:
+            d["cand_edges"] = [cands[i]["edges"] for i in idx]
+            if sm.best >= 0:
+                dims = np.zeros(6, np.int32)
+                _core.check(L.stp_qpgraph_search_best_fit(h, _p(dims), None, None, None, None, None, None, None))
+                npop, ne, na, npair = (int(x) for x in dims[:4])
+                w, lo, hi = np.zeros(max(na, 1)), np.zeros(max(na, 1)), np.zeros(max(na, 1))
+                bl, fit, fo, qd = np.zeros(ne), np.zeros(npair), np.zeros(npair), np.zeros(npair)
+                _core.check(L.stp_qpgraph_search_best_fit(h, _p(dims), _p(w), _p(lo), _p(hi), _p(bl), _p(fit),
+                                                          _p(fo), _p(qd)))
+                buf = C.create_string_buffer(256)
+
+                def labels(kind, n):
+                    out = []
+                    for i in range(n):
+                        _core.check(L.stp_qpgraph_search_best_label(h, kind, i, buf, 256))
+                        out.append(buf.value.decode())
+                    return out
+                d["best_fit"] = {"score": float(sm.best_score), "weight": w[:na].tolist(),
+                                 "weight_lo": lo[:na].tolist(), "weight_hi": hi[:na].tolist(),
+                                 "edge_length": bl.tolist(), "f3_fit": fit.tolist(), "f3_obs": fo.tolist(),
+                                 "leaves": labels(0, npop), "admix_from": labels(1, na), "admix_to": labels(2, na),
+                                 "edge_from": labels(3, ne), "edge_to": labels(4, ne)}
+        return d
+    finally:
+        L.stp_qpgraph_search_free(h)

@@ -362,6 +362,70 @@ class _Lib:
         return fo, Q.reshape((npair, npair), order="F"), Qi.reshape((npair, npair), order="F")
 
 
+    # ---- qpGraph topology enumeration and search (oracle/_ref only) ------------
+    def _qg_list(self, h):
+        L = self.lib
+        L.ref_qg_list_size.argtypes = [C.c_void_p]
+        L.ref_qg_list_get.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 5
+        L.ref_qg_list_edges.argtypes = [C.c_void_p, C.c_int]
+        L.ref_qg_list_edges.restype = C.c_char_p
+        L.ref_qg_list_free.argtypes = [C.c_void_p]
+        out = []
+        na, gid = C.c_int(), C.c_int()
+        hv = C.c_ulonglong()
+        sc, sp = C.c_double(), C.c_double()
+        for i in range(L.ref_qg_list_size(h)):
+            L.ref_qg_list_get(h, i, C.byref(na), C.byref(gid), C.byref(hv), C.byref(sc), C.byref(sp))
+            es = L.ref_qg_list_edges(h, i).decode()
+            edges = [tuple(t.split(">", 1)) for t in es.split(";") if t]
+            out.append({"nadmix": na.value, "id": gid.value, "hash": hv.value, "edges": edges,
+                        "score": sc.value, "restart_spread": sp.value})
+        L.ref_qg_list_free(h)
+        return out
+
+    def qg_enumerate(self, leaves, max_nadmix=1):
+        f = self.fn("qg_enumerate")
+        f.restype = C.c_void_p
+        return self._qg_list(f(C.c_int(len(leaves)), self._strs(leaves), C.c_int(max_nadmix)))
+
+    def qg_neighbors(self, graph, leaves, max_nadmix=1):
+        f = self.fn("qg_neighbors")
+        f.restype = C.c_void_p
+        f.argtypes = [C.c_char_p, C.c_int, C.c_ulonglong, C.c_int, C.c_void_p, C.c_int]
+        es = ";".join(f"{a}>{b}" for a, b in graph["edges"]).encode()
+        return self._qg_list(f(es, graph["nadmix"], graph["hash"], len(leaves), self._strs(leaves), max_nadmix))
+
+    def qg_graph_hash(self, edges):
+        f = self.fn("qg_graph_hash")
+        f.restype = C.c_ulonglong
+        f.argtypes = [C.c_char_p]
+        return int(f(";".join(f"{a}>{b}" for a, b in edges).encode()))
+
+    def qg_search(self, f2, vpair, block_sizes, names, pops, max_nadmix=1, numstart=10, fudge=1e-4,
+                  diag_f3=1e-5, constrained=True, run_heuristic=False):
+        """run_qpgraph_search on a host tensor: (summary dict, candidate list with scores)."""
+        f2 = np.asarray(f2, np.float64)
+        P, _, nb = f2.shape
+        fl = np.asfortranarray(f2).ravel(order="F")
+        vp = np.zeros_like(fl) if vpair is None else np.asfortranarray(np.asarray(vpair, np.float64)).ravel(order="F")
+        bs = np.ascontiguousarray(block_sizes, np.int32)
+        scal = np.zeros(9)
+        bh = (C.c_ulonglong * 1)()
+        f = self.fn("qg_search")
+        f.restype = C.c_void_p
+        nm, pp = self._strs(list(names)), self._strs(list(pops))
+        h = f(fl.ctypes.data_as(C.c_void_p), vp.ctypes.data_as(C.c_void_p), bs.ctypes.data_as(C.c_void_p),
+              C.c_int(P), C.c_int(nb), C.c_int(len(names)), nm, C.c_int(len(pops)), pp, C.c_int(max_nadmix),
+              C.c_int(numstart), C.c_double(fudge), C.c_double(diag_f3), C.c_int(1 if constrained else 0),
+              C.c_int(1 if run_heuristic else 0), scal.ctypes.data_as(C.c_void_p), bh)
+        assert h, "reference search failed"
+        summary = {"best_score": scal[0], "second_best_score": scal[1], "status": int(scal[2]),
+                   "heuristic_recovered": bool(scal[3]), "best_nadmix": int(scal[4]), "n_trees": int(scal[5]),
+                   "n_admix1": int(scal[6]), "n_candidates": int(scal[7]), "fit_all_wall_ms": scal[8],
+                   "best_hash": int(bh[0])}
+        return summary, self._qg_list(h)
+
+
 _cache = {}
 
 
