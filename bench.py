@@ -530,6 +530,70 @@ def qpgraph_bench(ctx, n_graphs: int, cpu: bool, numstart: int = 10):
     return out
 
 
+SEARCH_NAMES = ["England_BellBeaker", "Czechia_EBA_CordedWare", "Turkey_N", "Mbuti", "Israel_Natufian",
+                "Iran_GanjDareh_N", "Han", "Papuan", "Karitiana"]
+SEARCH_POPS = ["Mbuti", "Han", "Iran_GanjDareh_N", "Israel_Natufian", "Czechia_EBA_CordedWare"]
+
+
+def qpgraph_search_bench(ctx, cpu: bool, reps: int = 3):
+    """The topology search on the AT2 qpGraph fixture (9 populations, 710 blocks): every
+    tree and one-admixture graph on 5 of them (1590 candidates), 10 restarts per admixture
+    graph, fitted in one device fleet against one shared f3 basis; then the 8-seed
+    hill-climb.  Reports the fleet's wall time (the reference's fit_all_wall_ms) and the
+    whole search's."""
+    from steppe_amd import _core, qpgraph
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from fixtures import read_raw_fixture
+    f2, bs = read_raw_fixture(os.path.join(ROOT, "tests", "golden", "at2", "f2_qpgraph_9pop.bin"))
+    h = _core.upload_f2(f2, None, bs, SEARCH_NAMES)
+    qpgraph.run_qpgraph_search(h, SEARCH_POPS[:3], run_heuristic=False)   # warm-up: code load
+    ctx.timing(True)
+    ctx.read_timing(reset=True)
+    fleet, wall = [], []
+    d = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        d = qpgraph.run_qpgraph_search(h, SEARCH_POPS)
+        wall.append(time.perf_counter() - t0)
+        fleet.append(d["fit_all_wall_ms"])
+    tm = ctx.read_timing(reset=True)
+    ctx.timing(False)
+    n = d["n_candidates"]
+    fl = float(np.median(fleet))
+    out = {"metric": "qpGraph topology search: candidate topologies fitted/sec", "value": n / (fl / 1e3),
+           "unit": "topologies/s", "candidates": n, "fleet_ms": fl, "search_wall_s": float(np.median(wall)),
+           "kernel_ms_per_search": tm["qpgraph"][0] / reps, "best_score": d["best_score"],
+           "best_hash": d["best_hash"], "heuristic_recovered": d["heuristic_recovered"],
+           "workload": "AT2 qpGraph fixture (9 populations, 710 blocks); all 105 trees and 1485 one-admixture "
+                       "graphs on Mbuti, Han, Iran_GanjDareh_N, Israel_Natufian, Czechia_EBA_CordedWare; "
+                       "10 restarts per admixture graph (plus grid seeds), device L-BFGS, FP64; then the "
+                       "8-seed hill-climb (in search_wall_s)",
+           "baseline": {"value": 1450.0, "unit": "topologies/s", "source": "reference README: 1,590 graphs in "
+                        "1.1 s on an RTX 5090 (AADR)"},
+           "cpu_baseline": None}
+    log(f"[bench] qpgraph search: {n} candidates, fleet {fl:.1f} ms ({out['value']:.0f} topologies/s), "
+        f"whole search {out['search_wall_s']:.3f} s, kernels {out['kernel_ms_per_search']:.1f} ms")
+    if cpu:
+        import oracle_lib
+        lib = oracle_lib.ref()
+        kind = "reference"
+        if lib is None:
+            lib, kind = oracle_lib.cref(), "port"
+        cands = qpgraph.enumerate_graphs(SEARCH_POPS, 1)
+        pick = cands[::8]
+        t0 = time.perf_counter()
+        k = 0
+        while k < len(pick) and time.perf_counter() - t0 < 10.0:
+            lib.qpgraph_fit(f2, None, bs, pick[k]["edges"], SEARCH_NAMES)
+            k += 1
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k / cdt, "unit": "topologies/s", "cores": 1, "kind": kind,
+                               "sample": f"run_qpgraph on {k} of the candidates (every 8th, numstart 10), "
+                                         f"{cdt:.1f} s single-threaded ({os.path.basename(lib.path)})"}
+        log(f"[bench] qpgraph search cpu baseline {out['cpu_baseline']}")
+    return out
+
+
 def calibrate_copy():
     """A plain 1 GiB device-to-device copy, three times (counter calibration)."""
     import torch
@@ -559,6 +623,7 @@ def main():
     ap.add_argument("--frontend-snps", type=int, default=1_200_000)
     ap.add_argument("--no-frontend", action="store_true")
     ap.add_argument("--no-tiers", action="store_true")
+    ap.add_argument("--no-qpgraph-search", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -666,6 +731,9 @@ def main():
     qg = None
     if not args.no_qpgraph:
         qg = qpgraph_bench(ctx, args.qpgraph_graphs, not args.no_cpu_baseline)
+    qs = None
+    if not args.no_qpgraph_search:
+        qs = qpgraph_search_bench(ctx, not args.no_cpu_baseline)
 
     out = {
         "metric": METRIC, "value": value, "unit": "pop-pair·SNPs/s", "n_gpus": 1,
@@ -678,7 +746,7 @@ def main():
                    "P": P, "M": M, "n_block": nb, "n_individuals": n_ind,
                    "precision": args.precision, "parallelism": "single GPU"},
         "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg, "fstats": fs, "frontend": fe,
-        "tiers": tiers,
+        "tiers": tiers, "qpgraph_search": qs,
     }
     print(json.dumps(out), flush=True)
 

@@ -771,6 +771,19 @@ std::size_t qg_eval_lds_bytes(const QgEvalArgs& a) {
     return n * sizeof(double);
 }
 
+int qg_blocks_per_cu(int T, bool fit, std::size_t lds) {
+    int n = 0;
+    auto q = [&](auto kernel) {
+        if (lds > 64 * 1024)
+            STP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        STP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kThreads, lds));
+    };
+    if (T == 32) fit ? q(qg_fit_kernel<32>) : q(qg_eval_kernel<32>);
+    else fit ? q(qg_fit_kernel<64>) : q(qg_eval_kernel<64>);
+    return n > 0 ? n : 1;
+}
+
 void launch_qg_eval(int T, const QgEvalArgs& a, int n_workgroups, hipStream_t st) {
     if (a.n_inst <= 0) return;
     const std::size_t lds = qg_eval_lds_bytes(a);

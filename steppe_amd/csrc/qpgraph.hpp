@@ -102,6 +102,9 @@ struct QgBasisArgs {
 void launch_qg_eval(int T, const QgEvalArgs& a, int n_workgroups, hipStream_t st);
 void launch_qg_fit(int T, const QgEvalArgs& a, const QgFitArgs& fa, int n_workgroups, hipStream_t st);
 std::size_t qg_eval_lds_bytes(const QgEvalArgs& a);
+// Workgroups of the persistent score (fit = false) or fit kernel of bucket T that fit on
+// one CU at once with lds bytes of dynamic LDS (the launches fill every CU to this).
+int qg_blocks_per_cu(int T, bool fit, std::size_t lds);
 void launch_qg_basis(const QgBasisArgs& a, int n_workgroups, hipStream_t st);
 std::size_t qg_basis_lds_bytes(const QgBasisArgs& a);
 

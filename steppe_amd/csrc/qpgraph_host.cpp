@@ -454,7 +454,7 @@ void HipBackend::qpgraph_set_eval(QgSet& s, int n_inst, const std::int32_t* inst
     STP_REQUIRE(qg_eval_lds_bytes(a) <= 150 * 1024, "qpgraph: graphs too large for the device score");
     launch_zero_u32(a.counter, 1, stream_);
     mark_begin(kQpgraph);
-    launch_qg_eval(T, a, std::min(n_inst, 2 * n_cu_), stream_);
+    launch_qg_eval(T, a, std::min(n_inst, qg_blocks_per_cu(T, false, qg_eval_lds_bytes(a)) * n_cu_), stream_);
     mark_end(kQpgraph);
     STP_HIP(hipMemcpyAsync(score, d_score, n_inst * sizeof(double), hipMemcpyDeviceToHost, stream_));
     if (grad_theta && mode != 0)
@@ -546,7 +546,7 @@ void HipBackend::qpgraph_set_fit(QgSet& s, int n_inst, const std::int32_t* inst_
     fa.ftol = o.ftol;
     launch_zero_u32(a.counter, 1, stream_);
     mark_begin(kQpgraph);
-    launch_qg_fit(T, a, fa, std::min(n_inst, 2 * n_cu_), stream_);
+    launch_qg_fit(T, a, fa, std::min(n_inst, qg_blocks_per_cu(T, true, qg_eval_lds_bytes(a)) * n_cu_), stream_);
     mark_end(kQpgraph);
     if (theta_out)
         STP_HIP(hipMemcpyAsync(theta_out, fa.theta_out, static_cast<std::size_t>(n_inst) * D * sizeof(double),
