@@ -170,7 +170,7 @@ def test_joint_adjoint(cref, cases, gset):
             bm[e] -= h
             fd = (float(qr.joint_score(t, tt.detach(), torch.tensor(bp), fo, Qi)) -
                   float(qr.joint_score(t, tt.detach(), torch.tensor(bm), fo, Qi))) / (2 * h)
-            assert abs(r["grad_bl"][k, e] - fd) <= 1e-4 * abs(fd) + FD_NOISE * abs(float(val)) / h
+            assert abs(r["grad_bl"][k, e] - fd) <= 1e-4 * abs(fd) + FD_NOISE * abs(v) / h
 
 
 def test_autograd_function(cases, gset):
