@@ -3,7 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2] [--precision split_bf16|fp64]
                     [--qpadm-models N] [--no-qpadm] [--qpgraph-graphs N] [--no-qpgraph]
                     [--fstat-quartets N] [--no-fstats] [--frontend-snps N] [--no-frontend]
-                    [--no-cpu-baseline]
+                    [--no-tiers] [--no-qpgraph-search] [--no-cpu-baseline]
     python bench.py --calibrate-copy      # a plain 1 GiB device copy, for counter calibration
 
 One step = one pass of the hot path over the synthetic panel already resident in
@@ -41,6 +41,12 @@ all-combination f3 sweep (1.3M triples) and f4 sweep (64.7M quartets, Gram form 
 FP64 MFMA) at |z| >= 3, with the reference CPU run_f4 rate on a sample.  ``frontend``:
 genotype files -> resident f2 for GENO and TGENO triples of the c2 shape (random codes,
 1.2M SNPs), with the reference's own reader / filter / f2 chain on a 40k-SNP sample.
+``tiers``: the same decode + f2 of the default workload with the result kept in HBM,
+streamed into host memory, and streamed into an f2.bin file (wall time and GB/s of
+result).  ``qpgraph_search``: the topology search on the AT2 qpGraph fixture (all 1590
+trees and one-admixture graphs on 5 of its populations, one device fleet), candidate
+topologies fitted/s with the reference's published rate beside it and the reference CPU
+fit rate on a sample of the candidates.
 """
 from __future__ import annotations
 
