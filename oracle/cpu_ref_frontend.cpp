@@ -1,5 +1,5 @@
 // oracle/cpu_ref_frontend.cpp — CPU restatement of the reference's genotype-file extract:
-// packed .geno (TGENO / GENO) + .snp + .ind -> f2 block tensor.
+// packed .geno (TGENO / GENO) + .snp + .ind, or PLINK .bed + .bim + .fam -> f2 block tensor.
 //
 // TEST INFRASTRUCTURE ONLY: the parity tests call it as the checker; the product
 // library never links or calls it.
@@ -12,6 +12,12 @@
 //   TGENO gather            src/io/geno_reader.cpp:274-372
 //   GENO transpose          src/io/geno_reader.cpp:375-505 + src/device/backend.hpp:790-830
 //                           (ComputeBackend::transpose_to_canonical)
+//   PLINK .bed              src/io/geno_reader.cpp:48-72, 208-238 (magic 6c 1b 01, geometry from
+//                           the .bim / .fam beside the .bed), 585-634 (LSB-first codes,
+//                           kBedToCanon = {2, 3, 1, 0}, src/io/eigenstrat_format.hpp:66-77)
+//   .bim / .fam             src/io/plink_reader.cpp:53-151 (A1 is the counted allele, '0' -> 'N';
+//                           the population is the .fam phenotype column, 1 Control, 2 Case,
+//                           9 / -9 / 0 skipped), src/io/genotype_source.cpp:33-44
 //   front end               src/core/stats/genotype_front_end.cpp:9-37 (M0 = min(n_snp, rows))
 //   decode / ploidy         cpu_ref.cpp (cref_decode_af, cref_detect_ploidy)
 //   keep mask               src/io/filter/snp_filter.cpp:11-117, snp_summary_reduce.hpp:36-77,
@@ -103,10 +109,35 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
     std::string text;
     for (int i = 0; i < 48 && head[i]; ++i) text.push_back(head[i]);
     const auto tok = split(text);
-    if (tok.empty() || (tok[0] != "TGENO" && tok[0] != "GENO")) throw std::runtime_error("not TGENO / GENO");
-    const bool tgeno = tok[0] == "TGENO";
+    const bool packed_hdr = !tok.empty() && (tok[0] == "TGENO" || tok[0] == "GENO");
+    const bool plink = !packed_hdr && g.gcount() >= 3 && static_cast<unsigned char>(head[0]) == 0x6c &&
+                       static_cast<unsigned char>(head[1]) == 0x1b;
+    if (!packed_hdr && !plink) throw std::runtime_error("not TGENO / GENO / PLINK .bed");
+    if (plink && static_cast<unsigned char>(head[2]) != 0x01) throw std::runtime_error("individual-major .bed");
+    const bool tgeno = packed_hdr && tok[0] == "TGENO";
     std::size_t counts[2] = {0, 0};
-    {
+    if (plink) {
+        // geometry from the non-empty lines of the .bim and .fam next to the .bed
+        std::string base(geno);
+        if (base.size() >= 4 && base.compare(base.size() - 4, 4, ".bed") == 0) base.resize(base.size() - 4);
+        auto lines = [](const std::string& p) {
+            std::ifstream in(p, std::ios::binary);
+            std::size_t n = 0;
+            std::string l;
+            while (std::getline(in, l)) {
+                if (!l.empty() && l.back() == '\r') l.pop_back();
+                if (l.empty()) {
+                    if (in.peek() == std::char_traits<char>::eof()) break;
+                    continue;
+                }
+                ++n;
+            }
+            return n;
+        };
+        counts[1] = lines(base + ".bim");
+        counts[0] = lines(base + ".fam");
+        if (counts[0] == 0 || counts[1] == 0) throw std::runtime_error(".bim / .fam missing or empty");
+    } else {
         int got = 0;
         std::size_t i = text.find(tok[0]) + tok[0].size();
         while (i < text.size() && got < 2) {
@@ -119,8 +150,8 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
         if (got < 2) throw std::runtime_error("header counts");
     }
     const std::size_t n_ind = counts[0], n_snp = counts[1];
-    const std::size_t bpr = tgeno ? (n_snp + 3) / 4 : std::max<std::size_t>(48, (n_ind + 3) / 4);
-    const std::size_t hdr = tgeno ? 48 : bpr;
+    const std::size_t bpr = plink ? (n_ind + 3) / 4 : tgeno ? (n_snp + 3) / 4 : std::max<std::size_t>(48, (n_ind + 3) / 4);
+    const std::size_t hdr = plink ? 3 : tgeno ? 48 : bpr;
     const std::size_t n_records = tgeno ? n_ind : n_snp;
     g.seekg(0, std::ios::end);
     const std::size_t fsize = static_cast<std::size_t>(g.tellg());
@@ -140,11 +171,29 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
     std::string line;
     while (std::getline(fi, line)) {
         const auto t = split(line);
-        if (t.size() < 3 || row >= present) continue;
-        auto it = at.find(t[2]);
+        std::string label;
+        if (plink) {
+            // <fid> <iid> <father> <mother> <sex> <phenotype>: the phenotype names the group
+            if (t.size() < 6) continue;
+            if (row >= present) {
+                ++row;
+                continue;
+            }
+            label = t[5];
+            if (label == "9" || label == "-9" || label == "0") {
+                ++row;
+                continue;
+            }
+            if (label == "1") label = "Control";
+            else if (label == "2") label = "Case";
+        } else {
+            if (t.size() < 3 || row >= present) continue;
+            label = t[2];
+        }
+        auto it = at.find(label);
         if (it == at.end()) {
-            at[t[2]] = groups.size();
-            groups.push_back({t[2], groups.size(), {row}});
+            at[label] = groups.size();
+            groups.push_back({label, groups.size(), {row}});
         } else {
             groups[it->second].rows.push_back(row);
         }
@@ -180,9 +229,17 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
     std::map<std::string, int> other;
     int next_other = -1;
     while (std::getline(fs, line)) {
-        const auto f = split(line);
+        auto f = split(line);
         if (f.empty()) break;
-        if (f.size() < 3) throw std::runtime_error("bad .snp line");
+        if (plink) {
+            // <chrom> <id> <genpos> <physpos> <A1> <A2>; A1 (the counted allele) is ref
+            if (f.size() < 6) throw std::runtime_error("bad .bim line");
+            for (int k : {4, 5})
+                if (f[k][0] == '0') f[k] = "N";
+            f = {f[1], f[0], f[2], f[3], f[4], f[5]};
+        } else if (f.size() < 3) {
+            throw std::runtime_error("bad .snp line");
+        }
         int c = 0;
         const bool digits = std::all_of(f[1].begin(), f[1].end(), [](char ch) { return std::isdigit((unsigned char)ch); });
         if (!(digits && full(f[1], c))) {
@@ -228,7 +285,13 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
             g.seekg(static_cast<std::streamoff>(hdr + s * bpr));
             g.read(reinterpret_cast<char*>(rec.data()), static_cast<std::streamsize>(bpr));
             for (std::size_t k = 0; k < sel.size(); ++k) {
-                const std::uint8_t code = (rec[sel[k] / 4] >> (6 - 2 * (sel[k] % 4))) & 3;
+                std::uint8_t code;
+                if (plink) {
+                    static const std::uint8_t bed_to_canon[4] = {2, 3, 1, 0};
+                    code = bed_to_canon[(rec[sel[k] / 4] >> (2 * (sel[k] % 4))) & 3];
+                } else {
+                    code = (rec[sel[k] / 4] >> (6 - 2 * (sel[k] % 4))) & 3;
+                }
                 tile[k * cbpr + s / 4] |= static_cast<std::uint8_t>(code << (6 - 2 * (s % 4)));
             }
         }

@@ -40,6 +40,7 @@
 #include "io/filter/include_exclude.hpp"
 #include "io/filter/snp_filter.hpp"
 #include "io/geno_reader.hpp"
+#include "io/genotype_source.hpp"
 #include "io/ind_reader.hpp"
 #include "io/snp_reader.hpp"
 #include "core/qpadm/qpgraph_enumerate.hpp"
@@ -379,21 +380,23 @@ void* ref_extract_open(const char* geno, const char* snp, const char* ind, int p
         for (int i = 0; i < n_pops; ++i) sel.labels.emplace_back(pops[i]);
         sel.k = static_cast<std::size_t>(top_k);
         sel.min_n = static_cast<std::size_t>(min_n);
-        const io::IndPartition part = io::read_ind(ind, sel, reader.records_present());
+        const io::IndPartition part = io::read_ind_partition(reader.header().format, ind, sel, reader.records_present());
         if (pop_mode == 0)
             for (const auto& want : sel.labels) {
                 bool found = false;
                 for (const auto& g : part.groups) found = found || g.label == want;
                 if (!found) throw std::invalid_argument("unknown population " + want);
             }
-        const io::SnpTable snptab = io::read_snp(snp, SIZE_MAX);
+        const io::SnpTable snptab = io::read_snp_table(reader.header().format, snp, SIZE_MAX);
         const std::size_t M0 = std::min(reader.header().n_snp, snptab.count);
         auto be = steppe::device::make_cpu_backend();
         io::GenotypeTile tile;
         if (reader.header().format == io::GenoFormat::Tgeno) {
             tile = reader.read_tile(part, 0, M0);
         } else {
-            const io::SnpMajorTile src = reader.read_snp_major_tile(part, 0, M0);
+            const io::SnpMajorTile src = reader.header().format == io::GenoFormat::Plink
+                                             ? reader.read_plink_snp_major_tile(part, 0, M0)
+                                             : reader.read_snp_major_tile(part, 0, M0);
             steppe::SnpMajorTileView v;
             v.snp_major = src.snp_major.data();
             v.src_bytes_per_record = src.src_bytes_per_record;

@@ -477,7 +477,8 @@ def extract_f2_files(geno: str, snp: str, ind: str, pops=None, *, top_k: int = 0
                      maxmiss: float = 0.0, autosomes_only: bool = True, drop_monomorphic: bool = False,
                      transversions_only: bool = False, ploidy="auto", strand_mode: str = "drop",
                      precision=None, tier="auto", disk_path: str = "", chunk_blocks: int = 0):
-    """Packed genotype triple -> f2 (src/app/extract_f2_core.cpp:96-407): pops
+    """Packed genotype triple (TGENO / GENO .geno + .snp + .ind, or PLINK .bed + .bim +
+    .fam, told apart by the first bytes of ``geno``) -> f2 (src/app/extract_f2_core.cpp:96-407): pops
     (explicit labels), or the top_k largest populations, or every population with at least
     min_n individuals.  tier: "auto" (HBM if the tensor fits 0.70 of free device memory,
     else host memory, else the f2.bin file disk_path), "resident", "host" or "disk";
@@ -535,12 +536,21 @@ def extract_f2_files(geno: str, snp: str, ind: str, pops=None, *, top_k: int = 0
     return h, inf
 
 
+def genotype_triple(prefix: str):
+    """The files of a genotype prefix (src/io/genotype_source.cpp:15-30): <prefix>.geno /
+    .snp / .ind when the .geno exists, else the PLINK <prefix>.bed / .bim / .fam."""
+    if not os.path.exists(prefix + ".geno") and os.path.exists(prefix + ".bed"):
+        return prefix + ".bed", prefix + ".bim", prefix + ".fam"
+    return prefix + ".geno", prefix + ".snp", prefix + ".ind"
+
+
 def run_extract_f2(packed, *args, **kw):
     """Genotypes -> f2.  With a path prefix as the first argument this is the reference's
     binding (bindings/bind_fstats.cpp:131-215): run_extract_f2(prefix, pops, out="",
     device=0, blgsize=0.05, maf=0.0, maxmiss=0.0, autosomes_only=True,
     drop_monomorphic=False, transversions_only=False, ploidy="auto", strand_mode="drop",
-    precision=None) reads <prefix>.{geno,snp,ind} and returns an F2Handle (or the out path).
+    precision=None) reads <prefix>.{geno,snp,ind} (or the PLINK <prefix>.{bed,bim,fam}) and
+    returns an F2Handle (or the out path).
     Otherwise it takes an in-memory tile (run_extract_f2_tile)."""
     if isinstance(packed, (str, os.PathLike)):
         names = ("pops", "out", "device", "blgsize", "maf", "maxmiss", "autosomes_only", "drop_monomorphic",
@@ -550,7 +560,7 @@ def run_extract_f2(packed, *args, **kw):
         pops = kw.pop("pops", None)
         if not pops:
             raise ValueError("extract_f2: pops needs at least one population name")
-        h, _ = extract_f2_files(prefix + ".geno", prefix + ".snp", prefix + ".ind", pops,
+        h, _ = extract_f2_files(*genotype_triple(prefix), pops,
                                 **{k: v for k, v in kw.items() if v is not None or k == "precision"})
         return h
     return run_extract_f2_tile(packed, *args, **kw)

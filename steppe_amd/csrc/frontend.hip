@@ -35,19 +35,24 @@ __global__ __launch_bounds__(256) void transpose_gather_kernel(const std::uint8_
                                                                std::int64_t src_bpr, std::int64_t n_snp,
                                                                const std::int32_t* __restrict__ sel, std::int64_t n_sel,
                                                                std::uint8_t* __restrict__ out, std::int64_t out_bpr,
-                                                               std::int64_t col_off, std::int64_t n_groups) {
+                                                               std::int64_t col_off, std::int64_t n_groups, int bed) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const std::int64_t g = t % n_sel;
     const std::int64_t w = t / n_sel;
     if (w >= n_groups) return;
     const std::int32_t row = sel[g];
     const std::uint8_t* col = src + (row >> 2);
-    const int shift = 6 - 2 * (row & 3);
+    // GENO: individual r in bits 6-2(r%4) (first in the top bits).  PLINK .bed: bits
+    // 2(r%4), codes 00 / 01 / 10 / 11 = 2 / missing / 1 / 0 copies, mapped through the
+    // 2-bit table 0x1E (entry c at bits 2c).
+    const int shift = bed ? 2 * (row & 3) : 6 - 2 * (row & 3);
+    const std::uint32_t table = bed ? 0x1Eu : 0xE4u;   // 0xE4: the identity
     std::uint32_t word[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
         const std::int64_t s = 64 * w + j;
-        const std::uint32_t code = s < n_snp ? (col[s * src_bpr] >> shift) & 3u : 0u;
+        const std::uint32_t raw = s < n_snp ? (col[s * src_bpr] >> shift) & 3u : 3u;
+        const std::uint32_t code = s < n_snp ? (table >> (2 * raw)) & 3u : 0u;
         // output byte j/4 (within the 16), position j%4 from the top; bytes little-endian in a word
         word[j >> 4] |= code << (8 * ((j >> 2) & 3) + 6 - 2 * (j & 3));
     }
@@ -149,7 +154,7 @@ __global__ __launch_bounds__(256) void compact_counts_kernel(const std::uint16_t
 
 void launch_transpose_gather(const std::uint8_t* src, std::int64_t src_bpr, std::int64_t n_snp, const std::int32_t* sel,
                              std::int64_t n_sel, std::uint8_t* out, std::int64_t out_bpr, std::int64_t col_off,
-                             hipStream_t st) {
+                             bool bed, hipStream_t st) {
     if (n_snp <= 0 || n_sel <= 0) return;
     const std::int64_t groups = ceil_div(n_snp, 64);
     STP_REQUIRE(out_bpr % 16 == 0 && col_off % 16 == 0, "transpose: output stride and offset must be 16-byte aligned");
@@ -157,7 +162,7 @@ void launch_transpose_gather(const std::uint8_t* src, std::int64_t src_bpr, std:
     const std::int64_t threads = groups * n_sel;
     STP_REQUIRE(ceil_div(threads, 256) <= 0x7fffffffLL, "transpose: block too large for one launch");
     transpose_gather_kernel<<<static_cast<unsigned>(ceil_div(threads, 256)), 256, 0, st>>>(
-        src, src_bpr, n_snp, sel, n_sel, out, out_bpr, col_off, groups);
+        src, src_bpr, n_snp, sel, n_sel, out, out_bpr, col_off, groups, bed ? 1 : 0);
     STP_LAUNCH_CHECK();
 }
 

@@ -21,9 +21,10 @@ struct KeepRule {
 // SNP-major records (src, n_snp records of src_bpr bytes) -> individual-major canonical
 // records: individual g of the selection (source column sel[g]) gets ceil(n_snp/64)*16
 // bytes at out + g*out_bpr + col_off.  out_bpr and col_off are multiples of 16.
+// bed: the source is a PLINK .bed (LSB-first codes, recoded to copies of A1).
 void launch_transpose_gather(const std::uint8_t* src, std::int64_t src_bpr, std::int64_t n_snp, const std::int32_t* sel,
                              std::int64_t n_sel, std::uint8_t* out, std::int64_t out_bpr, std::int64_t col_off,
-                             hipStream_t st);
+                             bool bed, hipStream_t st);
 
 // flags[s] = 1 when SNP s (counts a2 / n at p*ld + s) passes the rule.
 void launch_snp_keep(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P, std::int64_t M,

@@ -5,8 +5,9 @@
 // device path: the genotype front end (src/core/stats/genotype_front_end.cpp:9-37: open
 // the .geno, read the .ind selection against the records present, read the .snp,
 // M0 = min(header SNPs, .snp rows), build the canonical individual-major tile — TGENO
-// records gathered on the host, GENO records streamed in blocks of about 1 GiB and
-// transposed on the device, read_canonical_tile.cpp:60-180), per-sample ploidy, the
+// records gathered on the host, GENO and PLINK .bed records streamed in blocks of about
+// 1 GiB and transposed (and, for .bed, recoded) on the device,
+// read_canonical_tile.cpp:60-180), per-sample ploidy, the
 // decode, the regime-B keep mask and compaction (decode_compact_kernel.cu,
 // cuda_backend_decode.cu:490-620), assign_blocks on the kept SNPs' genetic positions
 // (block_partition_rule.cpp:14-87, with the 2 Mb physical fallback when the map is all
@@ -72,13 +73,15 @@ F2Blocks* HipBackend::extract_f2_files(const ExtractRequest& req, ExtractInfo& i
 
     // ---- front end: files ----
     const GenoFile f = open_geno(req.geno);
-    const std::vector<PopGroup> groups = read_ind(req.ind, req.pops, f.records_present);
+    const bool plink = f.header.format == GenoFormat::Plink;
+    const std::vector<PopGroup> groups =
+        plink ? read_fam(req.ind, req.pops, f.records_present) : read_ind(req.ind, req.pops, f.records_present);
     if (req.pops.mode == PopSelect::kExplicit)
         for (const auto& want : req.pops.labels) {
             const bool found = std::any_of(groups.begin(), groups.end(), [&](const PopGroup& g) { return g.label == want; });
             STP_REQUIRE(found, "extract_f2: pops contains an unknown population '" + want + "' (not present in the .ind)");
         }
-    const SnpTable snp = read_snp(req.snp);
+    const SnpTable snp = plink ? read_bim(req.snp) : read_snp(req.snp);
     const std::size_t M0 = std::min(f.header.n_snp, snp.size());
     const int P = static_cast<int>(groups.size());
     STP_REQUIRE(P > 0 && M0 > 0, "extract_f2: empty selection or no SNPs");
@@ -129,7 +132,7 @@ F2Blocks* HipBackend::extract_f2_files(const ExtractRequest& req, ExtractInfo& i
             mark_begin(kFront);
             launch_transpose_gather(fe_stage_.get(), static_cast<std::int64_t>(bpr),
                                     static_cast<std::int64_t>(s1 - s0), fe_i32_.get(), n_sel, fe_tile_.get(), out_bpr,
-                                    static_cast<std::int64_t>(s0 / 4), stream_);
+                                    static_cast<std::int64_t>(s0 / 4), plink, stream_);
             mark_end(kFront);
             sync();   // host is reused by the next block
         }

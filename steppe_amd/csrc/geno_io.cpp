@@ -40,6 +40,88 @@ std::vector<std::string> words(const std::string& line) {
     return t;
 }
 
+constexpr unsigned char kBedMagic0 = 0x6c, kBedMagic1 = 0x1b, kBedSnpMajor = 0x01;
+
+// The .bim / .fam beside a .bed: "x.bed" -> "x.bim", any other name -> name + ext.
+std::string plink_sibling(const std::string& bed, const char* ext) {
+    if (bed.size() >= 4 && bed.compare(bed.size() - 4, 4, ".bed") == 0) return bed.substr(0, bed.size() - 4) + ext;
+    return bed + ext;
+}
+
+// Non-empty lines (a trailing CR removed); 0 when the file cannot be opened.
+std::size_t count_lines(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return 0;
+    std::size_t n = 0;
+    std::string line;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (!line.empty()) ++n;
+    }
+    return n;
+}
+
+struct RawGroup {
+    std::string label;
+    std::size_t first;
+    std::vector<std::size_t> rows;
+};
+
+// The population selection over groups in order of first appearance, sorted by label.
+std::vector<PopGroup> select_groups(const std::vector<RawGroup>& groups, const PopSelect& sel, const std::string& path) {
+    std::vector<const RawGroup*> chosen;
+    if (sel.mode == PopSelect::kExplicit) {
+        const std::unordered_set<std::string> want(sel.labels.begin(), sel.labels.end());
+        for (const auto& g : groups)
+            if (want.count(g.label)) chosen.push_back(&g);
+    } else if (sel.mode == PopSelect::kTopK) {
+        for (const auto& g : groups) chosen.push_back(&g);
+        std::stable_sort(chosen.begin(), chosen.end(), [](const RawGroup* a, const RawGroup* b) {
+            if (a->rows.size() != b->rows.size()) return a->rows.size() > b->rows.size();
+            return a->first < b->first;
+        });
+        chosen.resize(std::min(sel.k, chosen.size()));
+    } else {
+        for (const auto& g : groups)
+            if (g.rows.size() >= sel.min_n) chosen.push_back(&g);
+    }
+    if (chosen.empty()) io_fail("ind: population selection is empty for " + path);
+    std::sort(chosen.begin(), chosen.end(), [](const RawGroup* a, const RawGroup* b) { return a->label < b->label; });
+    std::vector<PopGroup> out;
+    out.reserve(chosen.size());
+    for (const RawGroup* g : chosen) out.push_back(PopGroup{g->label, g->rows});
+    return out;
+}
+
+void add_row(std::vector<RawGroup>& groups, std::unordered_map<std::string, std::size_t>& at, const std::string& label,
+             std::size_t row) {
+    const auto it = at.find(label);
+    if (it == at.end()) {
+        at.emplace(label, groups.size());
+        groups.push_back(RawGroup{label, groups.size(), {row}});
+    } else {
+        groups[it->second].rows.push_back(row);
+    }
+}
+
+// Chromosome codes: 1..22 (any digit string) as numbers, X 23, Y 24, MT/M 90, other
+// names -1, -2, ... in order of first appearance.
+int chrom_code(const std::string& tok, std::map<std::string, int>& other, int& next_other) {
+    int chrom = 0;
+    const bool numeric = !tok.empty() && std::all_of(tok.begin(), tok.end(), [](char c) {
+        return std::isdigit(static_cast<unsigned char>(c));
+    });
+    if (numeric && parse_all(tok, chrom)) return chrom;
+    if (tok == "X" || tok == "x") return 23;
+    if (tok == "Y" || tok == "y") return 24;
+    if (tok == "MT" || tok == "mt" || tok == "M") return 90;
+    const auto it = other.find(tok);
+    if (it != other.end()) return it->second;
+    chrom = next_other--;
+    other.emplace(tok, chrom);
+    return chrom;
+}
+
 }  // namespace
 
 GenoHeader parse_geno_header(const char* head, std::size_t len) {
@@ -90,11 +172,28 @@ GenoFile open_geno(const std::string& path) {
     if (!in) io_fail("geno: cannot open " + path);
     char head[kHeaderBytes] = {};
     in.read(head, kHeaderBytes);
+    const std::streamsize got = in.gcount();
     GenoFile f;
     f.path = path;
-    if (in.gcount() == static_cast<std::streamsize>(kHeaderBytes)) f.header = parse_geno_header(head, kHeaderBytes);
+    if (got == static_cast<std::streamsize>(kHeaderBytes)) f.header = parse_geno_header(head, kHeaderBytes);
+    const auto* m = reinterpret_cast<const unsigned char*>(head);
+    if (f.header.format == GenoFormat::Unknown && got >= 3 && m[0] == kBedMagic0 && m[1] == kBedMagic1) {
+        if (m[2] != kBedSnpMajor)
+            io_fail("geno: PLINK .bed is individual-major (mode byte 0x00); only SNP-major (0x01) is read: " + path);
+        const std::string bim = plink_sibling(path, ".bim"), fam = plink_sibling(path, ".fam");
+        const std::size_t n_snp = count_lines(bim), n_ind = count_lines(fam);
+        if (n_snp == 0 || n_ind == 0)
+            io_fail("geno: PLINK .bed found but its .bim / .fam (" + bim + ", " + fam + ") are missing or empty");
+        f.header.format = GenoFormat::Plink;
+        f.header.n_ind = n_ind;
+        f.header.n_snp = n_snp;
+        f.header.n_records = n_snp;
+        f.header.bytes_per_record = packed_bytes(n_ind);
+        f.header.header_bytes = 3;
+    }
     if (f.header.format == GenoFormat::Unknown)
-        io_fail("geno: not a packed TGENO / GENO file (48-byte header with magic TGENO or GENO): " + path);
+        io_fail("geno: not a packed TGENO / GENO file (48-byte header with magic TGENO or GENO) or a PLINK .bed: " +
+                path);
     if (f.header.bytes_per_record == 0 || f.header.n_records == 0)
         io_fail("geno: degenerate header (zero records or stride): " + path);
     in.seekg(0, std::ios::end);
@@ -110,12 +209,7 @@ std::vector<PopGroup> read_ind(const std::string& path, const PopSelect& sel, st
                                std::size_t* n_individuals_total) {
     std::ifstream in(path);
     if (!in) io_fail("ind: cannot open " + path);
-    struct Raw {
-        std::string label;
-        std::size_t first;
-        std::vector<std::size_t> rows;
-    };
-    std::vector<Raw> groups;
+    std::vector<RawGroup> groups;
     std::unordered_map<std::string, std::size_t> at;
     std::size_t row = 0;
     std::string line;
@@ -123,39 +217,31 @@ std::vector<PopGroup> read_ind(const std::string& path, const PopSelect& sel, st
         const auto t = words(line);
         if (t.size() < 3) continue;
         if (row >= n_records_present) continue;
-        const auto it = at.find(t[2]);
-        if (it == at.end()) {
-            at.emplace(t[2], groups.size());
-            groups.push_back(Raw{t[2], groups.size(), {row}});
-        } else {
-            groups[it->second].rows.push_back(row);
-        }
+        add_row(groups, at, t[2], row);
         ++row;
     }
     if (n_individuals_total) *n_individuals_total = row;
     if (groups.empty()) io_fail("ind: no individuals parsed from " + path);
-    std::vector<const Raw*> chosen;
-    if (sel.mode == PopSelect::kExplicit) {
-        const std::unordered_set<std::string> want(sel.labels.begin(), sel.labels.end());
-        for (const auto& g : groups)
-            if (want.count(g.label)) chosen.push_back(&g);
-    } else if (sel.mode == PopSelect::kTopK) {
-        for (const auto& g : groups) chosen.push_back(&g);
-        std::stable_sort(chosen.begin(), chosen.end(), [](const Raw* a, const Raw* b) {
-            if (a->rows.size() != b->rows.size()) return a->rows.size() > b->rows.size();
-            return a->first < b->first;
-        });
-        chosen.resize(std::min(sel.k, chosen.size()));
-    } else {
-        for (const auto& g : groups)
-            if (g.rows.size() >= sel.min_n) chosen.push_back(&g);
+    return select_groups(groups, sel, path);
+}
+
+std::vector<PopGroup> read_fam(const std::string& path, const PopSelect& sel, std::size_t n_records_present) {
+    std::ifstream in(path);
+    if (!in) io_fail("fam: cannot open " + path);
+    std::vector<RawGroup> groups;
+    std::unordered_map<std::string, std::size_t> at;
+    std::size_t row = 0;
+    std::string line;
+    while (std::getline(in, line)) {
+        const auto t = words(line);
+        if (t.size() < 6) continue;
+        const std::string& pheno = t[5];
+        if (row < n_records_present && pheno != "9" && pheno != "-9" && pheno != "0")
+            add_row(groups, at, pheno == "1" ? "Control" : pheno == "2" ? "Case" : pheno, row);
+        ++row;
     }
-    if (chosen.empty()) io_fail("ind: population selection is empty for " + path);
-    std::sort(chosen.begin(), chosen.end(), [](const Raw* a, const Raw* b) { return a->label < b->label; });
-    std::vector<PopGroup> out;
-    out.reserve(chosen.size());
-    for (const Raw* g : chosen) out.push_back(PopGroup{g->label, g->rows});
-    return out;
+    if (groups.empty()) io_fail("fam: no individuals parsed from " + path);
+    return select_groups(groups, sel, path);
 }
 
 SnpTable read_snp(const std::string& path) {
@@ -175,22 +261,7 @@ SnpTable read_snp(const std::string& path) {
         }
         if (f.size() < 3)
             io_fail("snp: expected <id> <chrom> <genpos> at line " + std::to_string(line_no) + " of " + path);
-        int chrom = 0;
-        bool numeric = !f[1].empty() && std::all_of(f[1].begin(), f[1].end(),
-                                                     [](char c) { return std::isdigit(static_cast<unsigned char>(c)); });
-        if (!(numeric && parse_all(f[1], chrom))) {
-            if (f[1] == "X" || f[1] == "x") chrom = 23;
-            else if (f[1] == "Y" || f[1] == "y") chrom = 24;
-            else if (f[1] == "MT" || f[1] == "mt" || f[1] == "M") chrom = 90;
-            else {
-                const auto it = other.find(f[1]);
-                if (it != other.end()) chrom = it->second;
-                else {
-                    chrom = next_other--;
-                    other.emplace(f[1], chrom);
-                }
-            }
-        }
+        const int chrom = chrom_code(f[1], other, next_other);
         double g = 0.0;
         if (!parse_all(f[2], g) || !std::isfinite(g))
             io_fail("snp: malformed genetic position \"" + f[2] + "\" at line " + std::to_string(line_no));
@@ -203,6 +274,41 @@ SnpTable read_snp(const std::string& path) {
         t.physpos.push_back(ph);
         t.ref.push_back(alleles && !f[4].empty() ? f[4][0] : 'N');
         t.alt.push_back(alleles && !f[5].empty() ? f[5][0] : 'N');
+    }
+    return t;
+}
+
+SnpTable read_bim(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) io_fail("bim: cannot open " + path);
+    SnpTable t;
+    std::map<std::string, int> other;
+    int next_other = -1;
+    std::string line;
+    std::size_t line_no = 0;
+    auto allele = [](const std::string& a) { return a.empty() || a[0] == '0' ? 'N' : a[0]; };
+    while (std::getline(in, line)) {
+        ++line_no;
+        const auto f = words(line);
+        if (f.empty()) {
+            if (in.peek() == std::char_traits<char>::eof()) break;
+            io_fail("bim: blank line at line " + std::to_string(line_no) + " of " + path);
+        }
+        if (f.size() < 6)
+            io_fail("bim: expected <chrom> <id> <genpos> <physpos> <A1> <A2> at line " + std::to_string(line_no) +
+                    " of " + path);
+        const int chrom = chrom_code(f[0], other, next_other);
+        double g = 0.0;
+        if (!parse_all(f[2], g) || !std::isfinite(g))
+            io_fail("bim: malformed genetic position \"" + f[2] + "\" at line " + std::to_string(line_no));
+        double ph = 0.0;
+        if (!(parse_all(f[3], ph) && std::isfinite(ph))) ph = 0.0;
+        t.id.push_back(f[1]);
+        t.chrom.push_back(chrom);
+        t.genpos.push_back(g);
+        t.physpos.push_back(ph);
+        t.ref.push_back(allele(f[4]));
+        t.alt.push_back(allele(f[5]));
     }
     return t;
 }
@@ -233,7 +339,8 @@ void read_tgeno_selected(const GenoFile& f, const std::vector<PopGroup>& groups,
 }
 
 void read_geno_records(const GenoFile& f, std::size_t s0, std::size_t s1, std::vector<std::uint8_t>& out) {
-    STP_REQUIRE(f.header.format == GenoFormat::Geno, "geno: not a GENO (SNP-major) file");
+    STP_REQUIRE(f.header.format == GenoFormat::Geno || f.header.format == GenoFormat::Plink,
+                "geno: not a SNP-major (GENO or PLINK .bed) file");
     STP_REQUIRE(s0 < s1 && s1 <= f.records_present, "geno: SNP records beyond the file");
     const std::size_t bpr = f.header.bytes_per_record;
     out.resize((s1 - s0) * bpr);

@@ -9,6 +9,13 @@
 //   2-bit codes, first position in the two most significant bits, 3 = missing.
 //   .ind: "<id> <sex> <population>" per individual, in record order.
 //   .snp: "<id> <chrom> <genpos (Morgans)> [<physpos> [<ref> <alt>]]" per SNP.
+// PLINK (src/io/geno_reader.cpp:208-238, 585-634; src/io/plink_reader.cpp): a SNP-major
+//   .bed (magic 6c 1b 01, one record of ceil(n_ind/4) bytes per SNP, individual i in bits
+//   2(i%4) of byte i/4, codes 00 two copies of A1, 01 missing, 10 heterozygous, 11 none),
+//   its geometry counted from the non-empty lines of the .bim and .fam beside it; the
+//   .bim "<chrom> <id> <genpos> <physpos> <A1> <A2>" (A1, the counted allele, is ref;
+//   allele '0' is 'N'); the .fam phenotype column names the population (1 Control,
+//   2 Case, 9 / -9 / 0 not assigned).
 #pragma once
 
 #include <cstddef>
@@ -18,7 +25,7 @@
 
 namespace stp {
 
-enum class GenoFormat { Unknown = 0, Tgeno = 1, Geno = 2 };
+enum class GenoFormat { Unknown = 0, Tgeno = 1, Geno = 2, Plink = 3 };
 
 struct GenoHeader {
     GenoFormat format = GenoFormat::Unknown;
@@ -55,6 +62,9 @@ struct PopGroup {
 std::vector<PopGroup> read_ind(const std::string& path, const PopSelect& sel, std::size_t n_records_present,
                                std::size_t* n_individuals_total = nullptr);
 
+// .fam groups (src/io/plink_reader.cpp:104-151), selected as read_ind selects.
+std::vector<PopGroup> read_fam(const std::string& path, const PopSelect& sel, std::size_t n_records_present);
+
 // .snp table (src/io/snp_reader.cpp:20-63, detail/snp_text_parse.hpp): chromosome codes
 // 1..22 as numbers, X 23, Y 24, MT/M 90, other names -1, -2, ... in order of appearance;
 // genetic position strict, physical position 0 when unparseable, alleles 'N' when absent.
@@ -66,13 +76,15 @@ struct SnpTable {
     std::size_t size() const { return chrom.size(); }
 };
 SnpTable read_snp(const std::string& path);
+// .bim table (src/io/plink_reader.cpp:53-100), in the .snp table's terms.
+SnpTable read_bim(const std::string& path);
 
 // TGENO: the first ceil(n_snp/4) bytes of each selected individual's record, in the
 // groups' order, into out (n_sel x that many bytes) (src/io/geno_reader.cpp:274-372).
 void read_tgeno_selected(const GenoFile& f, const std::vector<PopGroup>& groups, std::size_t n_snp,
                          std::vector<std::uint8_t>& out);
-// GENO: SNP records [s0, s1) verbatim (s1 - s0 records of bytes_per_record bytes)
-// (src/io/geno_reader.cpp:465-505).
+// GENO or PLINK .bed: SNP records [s0, s1) verbatim (s1 - s0 records of
+// bytes_per_record bytes) (src/io/geno_reader.cpp:465-505, 585-634).
 void read_geno_records(const GenoFile& f, std::size_t s0, std::size_t s1, std::vector<std::uint8_t>& out);
 
 }  // namespace stp

@@ -99,5 +99,37 @@ def synthetic_triple(prefix: str, seed: int, n_snp: int = 3000, fmt: str = "tgen
     ref[low] = np.char.lower(ref[low])
     snp = {"id": [f"rs{k}" for k in range(n_snp)], "chrom": chrom, "genpos": genpos, "physpos": physpos,
            "ref": list(ref), "alt": list(alt)}
-    write_triple(prefix, codes, ind_pops, snp, fmt, records_missing)
+    if fmt == "plink":
+        # group labels as .fam phenotypes: two of them become the reserved codes 1 / 2
+        # (Control / Case) and one is the skipped missing phenotype -9
+        pheno = {"Pop_A": "1", "Pop_B": "2", "Pop_I": "-9"}
+        write_plink(prefix, codes, [pheno.get(p, p) for p in ind_pops], snp, records_missing)
+    else:
+        write_triple(prefix, codes, ind_pops, snp, fmt, records_missing)
     return list(ind_pops)
+
+
+def write_plink(prefix: str, codes: np.ndarray, ind_pops, snp, records_missing: int = 0):
+    """PLINK SNP-major .bed (magic 6c 1b 01; per SNP ceil(n_ind/4) bytes, individual i in
+    bits 2(i%4) of byte i/4: 00 two copies of A1, 01 missing, 10 heterozygous, 11 no A1),
+    .bim (<chrom> <id> <genpos> <physpos> <A1> <A2>, A1 = the counted allele) and .fam
+    (<fid> <iid> <father> <mother> <sex> <phenotype>, the phenotype naming the group).
+    codes: (n_ind, n_snp) copies of A1 0/1/2, or 3 missing."""
+    n_ind, n_snp = codes.shape
+    bed_of = np.array([3, 2, 0, 1], np.uint8)          # A1 copies 0, 1, 2, missing -> .bed code
+    c = bed_of[codes.T]                                 # (n_snp, n_ind)
+    w = (n_ind + 3) // 4
+    pad = np.zeros((n_snp, 4 * w), np.uint8)
+    pad[:, :n_ind] = c
+    q = pad.reshape(n_snp, w, 4)
+    rec = q[:, :, 0] | (q[:, :, 1] << 2) | (q[:, :, 2] << 4) | (q[:, :, 3] << 6)
+    with open(prefix + ".bed", "wb") as f:
+        f.write(bytes([0x6C, 0x1B, 0x01]) + rec[: n_snp - records_missing].tobytes())
+    with open(prefix + ".fam", "w") as f:
+        for i, p in enumerate(ind_pops):
+            f.write(f"F{i} I{i:05d} 0 0 {1 + i % 2} {p}\n")
+    with open(prefix + ".bim", "w") as f:
+        for k in range(len(snp["id"])):
+            a1 = "0" if snp["ref"][k] == "N" else snp["ref"][k]
+            f.write(f"{snp['chrom'][k]}\t{snp['id'][k]}\t{snp['genpos'][k]!r}\t{snp['physpos'][k]:.0f}\t"
+                    f"{a1}\t{snp['alt'][k]}\n")

@@ -99,3 +99,42 @@ def test_errors(cref, ref, tmp_path):
             lib.extract(*_paths(pre), pops=["Pop_A", "Nowhere"])
         with pytest.raises(ValueError):
             lib.extract(*_paths(pre), pops=["Pop_A"], maf=0.6)        # folded MAF <= 0.5: nothing kept
+
+
+# PLINK: the .fam phenotype names the group (Pop_A is written as 1 -> "Control", Pop_B as
+# 2 -> "Case", Pop_I as the skipped -9)
+PLINK_CASES = [
+    dict(pops=["Control", "Pop_C", "Pop_F", "Pop_H"]),
+    dict(top_k=5, maxmiss=1.0, autosomes_only=False),
+    dict(min_n=6, maf=0.05, drop_monomorphic=True, transversions_only=True, strand_mode=1),
+    dict(pops=["Case", "Pop_E", "Control"], maxmiss=0.34, strand_mode=2, ploidy=2, blgsize=0.02),
+]
+
+
+def _plink(pre):
+    return pre + ".bed", pre + ".bim", pre + ".fam"
+
+
+@pytest.mark.parametrize("case", range(len(PLINK_CASES)))
+def test_plink_oracle_bit_identical_to_reference(cref, ref, tmp_path, case):
+    pre = str(tmp_path / "b")
+    synthetic_triple(pre, 14, n_snp=2400, fmt="plink")
+    a = cref.extract(*_plink(pre), **PLINK_CASES[case])
+    b = ref.extract(*_plink(pre), **PLINK_CASES[case])
+    _same(a, b)
+    assert a["info"]["n_snp_kept"] > 100
+    assert "Pop_I" not in a["labels"] and "Pop_A" not in a["labels"]
+
+
+def test_plink_equals_geno_and_short_bed(cref, ref, tmp_path):
+    # the same panel as PLINK and as GENO: identical tensors over populations named alike
+    b, g = str(tmp_path / "b"), str(tmp_path / "g")
+    synthetic_triple(b, 15, n_snp=1800, fmt="plink")
+    synthetic_triple(g, 15, n_snp=1800, fmt="geno")
+    for kw in (dict(pops=["Pop_C", "Pop_D", "Pop_H"]), dict(pops=["Pop_E", "Pop_G"], ploidy=1, maxmiss=1.0)):
+        _same(cref.extract(*_plink(b), **kw), cref.extract(*_paths(g), **kw))
+    # fewer SNP records in the .bed than the .bim lists: an error in both
+    synthetic_triple(b, 15, n_snp=1800, fmt="plink", records_missing=9)
+    for lib in (cref, ref):
+        with pytest.raises(ValueError):
+            lib.extract(*_plink(b), pops=["Pop_C", "Pop_D"])
