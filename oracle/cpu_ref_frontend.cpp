@@ -1,5 +1,6 @@
 // oracle/cpu_ref_frontend.cpp — CPU restatement of the reference's genotype-file extract:
-// packed .geno (TGENO / GENO) + .snp + .ind, or PLINK .bed + .bim + .fam -> f2 block tensor.
+// packed .geno (TGENO / GENO) or EIGENSTRAT text .geno + .snp + .ind, or PLINK .bed + .bim +
+// .fam -> f2 block tensor.
 //
 // TEST INFRASTRUCTURE ONLY: the parity tests call it as the checker; the product
 // library never links or calls it.
@@ -12,6 +13,7 @@
 //   TGENO gather            src/io/geno_reader.cpp:274-372
 //   GENO transpose          src/io/geno_reader.cpp:375-505 + src/device/backend.hpp:790-830
 //                           (ComputeBackend::transpose_to_canonical)
+//   EIGENSTRAT text .geno   src/io/geno_reader.cpp:83-120 (geometry), 516-583 (0/1/2/9 per char)
 //   PLINK .bed              src/io/geno_reader.cpp:48-72, 208-238 (magic 6c 1b 01, geometry from
 //                           the .bim / .fam beside the .bed), 585-634 (LSB-first codes,
 //                           kBedToCanon = {2, 3, 1, 0}, src/io/eigenstrat_format.hpp:66-77)
@@ -112,7 +114,28 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
     const bool packed_hdr = !tok.empty() && (tok[0] == "TGENO" || tok[0] == "GENO");
     const bool plink = !packed_hdr && g.gcount() >= 3 && static_cast<unsigned char>(head[0]) == 0x6c &&
                        static_cast<unsigned char>(head[1]) == 0x1b;
-    if (!packed_hdr && !plink) throw std::runtime_error("not TGENO / GENO / PLINK .bed");
+    // EIGENSTRAT text: equal-length lines of 0 / 1 / 2 / 9, an empty line only at the end
+    std::vector<std::string> text_lines;
+    bool ascii = false;
+    if (!packed_hdr && !plink) {
+        std::ifstream t(geno, std::ios::binary);
+        std::string l;
+        bool ok = true;
+        while (ok && std::getline(t, l)) {
+            if (!l.empty() && l.back() == '\r') l.pop_back();
+            if (l.empty()) {
+                if (t.peek() == std::char_traits<char>::eof()) break;
+                ok = false;
+                break;
+            }
+            if (!text_lines.empty() && l.size() != text_lines[0].size()) ok = false;
+            for (char c : l)
+                if (!((c >= '0' && c <= '2') || c == '9')) ok = false;
+            text_lines.push_back(l);
+        }
+        ascii = ok && !text_lines.empty() && !text_lines[0].empty();
+    }
+    if (!packed_hdr && !plink && !ascii) throw std::runtime_error("not TGENO / GENO / EIGENSTRAT / PLINK .bed");
     if (plink && static_cast<unsigned char>(head[2]) != 0x01) throw std::runtime_error("individual-major .bed");
     const bool tgeno = packed_hdr && tok[0] == "TGENO";
     std::size_t counts[2] = {0, 0};
@@ -137,6 +160,9 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
         counts[1] = lines(base + ".bim");
         counts[0] = lines(base + ".fam");
         if (counts[0] == 0 || counts[1] == 0) throw std::runtime_error(".bim / .fam missing or empty");
+    } else if (ascii) {
+        counts[0] = text_lines[0].size();
+        counts[1] = text_lines.size();
     } else {
         int got = 0;
         std::size_t i = text.find(tok[0]) + tok[0].size();
@@ -155,7 +181,7 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
     const std::size_t n_records = tgeno ? n_ind : n_snp;
     g.seekg(0, std::ios::end);
     const std::size_t fsize = static_cast<std::size_t>(g.tellg());
-    const std::size_t present = std::min((fsize - hdr) / bpr, n_records);
+    const std::size_t present = ascii ? n_records : std::min((fsize - hdr) / bpr, n_records);
 
     // ---- .ind ----
     std::ifstream fi(ind);
@@ -279,6 +305,13 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
             g.seekg(static_cast<std::streamoff>(hdr + sel[k] * bpr));
             g.read(reinterpret_cast<char*>(tile.data() + k * cbpr), static_cast<std::streamsize>(cbpr));
         }
+    } else if (ascii) {
+        for (std::size_t s = 0; s < M0; ++s)
+            for (std::size_t k = 0; k < sel.size(); ++k) {
+                const char ch = text_lines[s][sel[k]];
+                const std::uint8_t code = ch == '9' ? 3 : static_cast<std::uint8_t>(ch - '0');
+                tile[k * cbpr + s / 4] |= static_cast<std::uint8_t>(code << (6 - 2 * (s % 4)));
+            }
     } else {
         std::vector<std::uint8_t> rec(bpr);
         for (std::size_t s = 0; s < M0; ++s) {

@@ -394,9 +394,11 @@ void* ref_extract_open(const char* geno, const char* snp, const char* ind, int p
         if (reader.header().format == io::GenoFormat::Tgeno) {
             tile = reader.read_tile(part, 0, M0);
         } else {
-            const io::SnpMajorTile src = reader.header().format == io::GenoFormat::Plink
-                                             ? reader.read_plink_snp_major_tile(part, 0, M0)
-                                             : reader.read_snp_major_tile(part, 0, M0);
+            const io::GenoFormat fmt = reader.header().format;
+            const io::SnpMajorTile src = fmt == io::GenoFormat::Plink ? reader.read_plink_snp_major_tile(part, 0, M0)
+                                       : fmt == io::GenoFormat::Eigenstrat
+                                           ? reader.read_eigenstrat_snp_major_tile(part, 0, M0)
+                                           : reader.read_snp_major_tile(part, 0, M0);
             steppe::SnpMajorTileView v;
             v.snp_major = src.snp_major.data();
             v.src_bytes_per_record = src.src_bytes_per_record;

@@ -125,9 +125,12 @@ F2Blocks* HipBackend::extract_f2_files(const ExtractRequest& req, ExtractInfo& i
         blk = std::min(blk, (M0 + 63) / 64 * 64);
         fe_stage_.ensure(blk * bpr);
         std::vector<std::uint8_t> host;
+        std::unique_ptr<AsciiGenoReader> text;
+        if (f.header.format == GenoFormat::Eigenstrat) text = std::make_unique<AsciiGenoReader>(f);
         for (std::size_t s0 = 0; s0 < M0; s0 += blk) {
             const std::size_t s1 = std::min(M0, s0 + blk);
-            read_geno_records(f, s0, s1, host);
+            if (text) text->next(s1 - s0, host);
+            else read_geno_records(f, s0, s1, host);
             STP_HIP(hipMemcpyAsync(fe_stage_.get(), host.data(), host.size(), hipMemcpyHostToDevice, stream_));
             mark_begin(kFront);
             launch_transpose_gather(fe_stage_.get(), static_cast<std::int64_t>(bpr),

@@ -61,6 +61,49 @@ std::size_t count_lines(const std::string& path) {
     return n;
 }
 
+bool genotype_char(char c, std::uint8_t& code) {
+    if (c >= '0' && c <= '2') {
+        code = static_cast<std::uint8_t>(c - '0');
+        return true;
+    }
+    if (c == '9') {
+        code = 3;
+        return true;
+    }
+    return false;
+}
+
+// EIGENSTRAT text geometry; Unknown unless every line has the first line's length and
+// only genotype characters (an empty line ends the file only at its end).
+GenoHeader eigenstrat_geometry(const std::string& path) {
+    GenoHeader h;
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return h;
+    std::string line;
+    std::size_t n_ind = 0, n_snp = 0;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.empty()) {
+            if (in.peek() == std::char_traits<char>::eof()) break;
+            return GenoHeader{};
+        }
+        if (n_snp == 0) n_ind = line.size();
+        else if (line.size() != n_ind) return GenoHeader{};
+        std::uint8_t code;
+        for (char c : line)
+            if (!genotype_char(c, code)) return GenoHeader{};
+        ++n_snp;
+    }
+    if (n_snp == 0 || n_ind == 0) return GenoHeader{};
+    h.format = GenoFormat::Eigenstrat;
+    h.n_ind = n_ind;
+    h.n_snp = n_snp;
+    h.n_records = n_snp;
+    h.bytes_per_record = packed_bytes(n_ind);
+    h.header_bytes = 0;
+    return h;
+}
+
 struct RawGroup {
     std::string label;
     std::size_t first;
@@ -177,7 +220,14 @@ GenoFile open_geno(const std::string& path) {
     f.path = path;
     if (got == static_cast<std::streamsize>(kHeaderBytes)) f.header = parse_geno_header(head, kHeaderBytes);
     const auto* m = reinterpret_cast<const unsigned char*>(head);
-    if (f.header.format == GenoFormat::Unknown && got >= 3 && m[0] == kBedMagic0 && m[1] == kBedMagic1) {
+    const bool bed_magic = got >= 3 && m[0] == kBedMagic0 && m[1] == kBedMagic1;
+    // (a file opening with the .bed magic cannot be EIGENSTRAT text: 'l' is not a genotype)
+    if (f.header.format == GenoFormat::Unknown && !bed_magic) f.header = eigenstrat_geometry(path);
+    if (f.header.format == GenoFormat::Eigenstrat) {
+        f.records_present = f.header.n_records;
+        return f;
+    }
+    if (f.header.format == GenoFormat::Unknown && bed_magic) {
         if (m[2] != kBedSnpMajor)
             io_fail("geno: PLINK .bed is individual-major (mode byte 0x00); only SNP-major (0x01) is read: " + path);
         const std::string bim = plink_sibling(path, ".bim"), fam = plink_sibling(path, ".fam");
@@ -336,6 +386,30 @@ void read_tgeno_selected(const GenoFile& f, const std::vector<PopGroup>& groups,
             if (in.gcount() != static_cast<std::streamsize>(bpr)) io_fail("geno: short read in " + f.path);
             ++k;
         }
+}
+
+AsciiGenoReader::AsciiGenoReader(const GenoFile& f) : f_(f), in_(f.path, std::ios::binary) {
+    STP_REQUIRE(f.header.format == GenoFormat::Eigenstrat, "geno: not an EIGENSTRAT text file");
+    if (!in_) io_fail("geno: cannot reopen " + f.path);
+}
+
+void AsciiGenoReader::next(std::size_t n, std::vector<std::uint8_t>& out) {
+    const std::size_t bpr = f_.header.bytes_per_record, n_ind = f_.header.n_ind;
+    out.assign(n * bpr, 0);
+    std::string line;
+    for (std::size_t s = 0; s < n; ++s) {
+        if (!std::getline(in_, line)) io_fail("geno: unexpected end of " + f_.path + " at SNP line " + std::to_string(line_ + 1));
+        ++line_;
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.size() != n_ind) io_fail("geno: ragged SNP line " + std::to_string(line_) + " in " + f_.path);
+        std::uint8_t* rec = out.data() + s * bpr;
+        for (std::size_t i = 0; i < n_ind; ++i) {
+            std::uint8_t code = 0;
+            if (!genotype_char(line[i], code))
+                io_fail("geno: illegal genotype character at SNP line " + std::to_string(line_) + " of " + f_.path);
+            rec[i >> 2] |= static_cast<std::uint8_t>(code << (6 - 2 * (i & 3)));
+        }
+    }
 }
 
 void read_geno_records(const GenoFile& f, std::size_t s0, std::size_t s1, std::vector<std::uint8_t>& out) {

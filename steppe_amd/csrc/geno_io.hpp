@@ -9,6 +9,8 @@
 //   2-bit codes, first position in the two most significant bits, 3 = missing.
 //   .ind: "<id> <sex> <population>" per individual, in record order.
 //   .snp: "<id> <chrom> <genpos (Morgans)> [<physpos> [<ref> <alt>]]" per SNP.
+// EIGENSTRAT ASCII (src/io/geno_reader.cpp:83-120, 516-583): one line per SNP of n_ind
+//   characters 0 / 1 / 2 (copies of ref) or 9 (missing), every line the same length.
 // PLINK (src/io/geno_reader.cpp:208-238, 585-634; src/io/plink_reader.cpp): a SNP-major
 //   .bed (magic 6c 1b 01, one record of ceil(n_ind/4) bytes per SNP, individual i in bits
 //   2(i%4) of byte i/4, codes 00 two copies of A1, 01 missing, 10 heterozygous, 11 none),
@@ -20,12 +22,13 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <fstream>
 #include <string>
 #include <vector>
 
 namespace stp {
 
-enum class GenoFormat { Unknown = 0, Tgeno = 1, Geno = 2, Plink = 3 };
+enum class GenoFormat { Unknown = 0, Tgeno = 1, Geno = 2, Plink = 3, Eigenstrat = 4 };
 
 struct GenoHeader {
     GenoFormat format = GenoFormat::Unknown;
@@ -83,6 +86,19 @@ SnpTable read_bim(const std::string& path);
 // groups' order, into out (n_sel x that many bytes) (src/io/geno_reader.cpp:274-372).
 void read_tgeno_selected(const GenoFile& f, const std::vector<PopGroup>& groups, std::size_t n_snp,
                          std::vector<std::uint8_t>& out);
+// EIGENSTRAT ASCII, read line by line in order: the next n SNP lines packed into
+// canonical SNP-major records of ceil(n_ind/4) bytes (first individual in the top bits).
+class AsciiGenoReader {
+public:
+    explicit AsciiGenoReader(const GenoFile& f);
+    void next(std::size_t n, std::vector<std::uint8_t>& out);
+
+private:
+    const GenoFile& f_;
+    std::ifstream in_;
+    std::size_t line_ = 0;
+};
+
 // GENO or PLINK .bed: SNP records [s0, s1) verbatim (s1 - s0 records of
 // bytes_per_record bytes) (src/io/geno_reader.cpp:465-505, 585-634).
 void read_geno_records(const GenoFile& f, std::size_t s0, std::size_t s1, std::vector<std::uint8_t>& out);

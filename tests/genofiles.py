@@ -99,7 +99,9 @@ def synthetic_triple(prefix: str, seed: int, n_snp: int = 3000, fmt: str = "tgen
     ref[low] = np.char.lower(ref[low])
     snp = {"id": [f"rs{k}" for k in range(n_snp)], "chrom": chrom, "genpos": genpos, "physpos": physpos,
            "ref": list(ref), "alt": list(alt)}
-    if fmt == "plink":
+    if fmt == "ascii":
+        write_eigenstrat_text(prefix, codes, ind_pops, snp)
+    elif fmt == "plink":
         # group labels as .fam phenotypes: two of them become the reserved codes 1 / 2
         # (Control / Case) and one is the skipped missing phenotype -9
         pheno = {"Pop_A": "1", "Pop_B": "2", "Pop_I": "-9"}
@@ -133,3 +135,20 @@ def write_plink(prefix: str, codes: np.ndarray, ind_pops, snp, records_missing: 
             a1 = "0" if snp["ref"][k] == "N" else snp["ref"][k]
             f.write(f"{snp['chrom'][k]}\t{snp['id'][k]}\t{snp['genpos'][k]!r}\t{snp['physpos'][k]:.0f}\t"
                     f"{a1}\t{snp['alt'][k]}\n")
+
+
+def write_eigenstrat_text(prefix: str, codes: np.ndarray, ind_pops, snp):
+    """EIGENSTRAT text .geno (one line per SNP, one character 0 / 1 / 2 / 9 per
+    individual) with the same .ind and .snp as the packed triple."""
+    n_ind, n_snp = codes.shape
+    chars = np.array([ord("0"), ord("1"), ord("2"), ord("9")], np.uint8)[codes.T]   # (n_snp, n_ind)
+    body = np.concatenate([chars, np.full((n_snp, 1), ord("\n"), np.uint8)], axis=1)
+    with open(prefix + ".geno", "wb") as f:
+        f.write(body.tobytes())
+    with open(prefix + ".ind", "w") as f:
+        for i, p in enumerate(ind_pops):
+            f.write(f"I{i:05d} {'MF'[i % 2]} {p}\n")
+    with open(prefix + ".snp", "w") as f:
+        for k in range(len(snp["id"])):
+            f.write(f"{snp['id'][k]} {snp['chrom'][k]} {snp['genpos'][k]!r} {snp['physpos'][k]:.0f} "
+                    f"{snp['ref'][k]} {snp['alt'][k]}\n")

@@ -140,3 +140,22 @@ def test_plink_matches_oracle(cref, tmp_path, monkeypatch):
     synthetic_triple(g, 14, n_snp=2400, fmt="geno")
     hg = _core.run_extract_f2(g, ["Pop_C", "Pop_D", "Pop_H"], precision="fp64")
     assert np.array_equal(hp._f2_numpy(), hg._f2_numpy()) and np.array_equal(hp._vpair_numpy(), hg._vpair_numpy())
+
+
+def test_eigenstrat_text_matches_oracle(cref, tmp_path, monkeypatch):
+    """EIGENSTRAT text .geno (one character per genotype) through the device, against the
+    oracle, in one and in several transpose blocks, and equal to the packed GENO of the
+    same panel."""
+    pre = str(tmp_path / "t")
+    synthetic_triple(pre, 16, n_snp=1500, fmt="ascii")
+    for i, kw in enumerate(CASES[:2]):
+        h, info = _core.extract_f2_files(*_paths(pre), precision="fp64", **kw)
+        check(h, info, cref.extract(*_paths(pre), **_oracle_kw(kw)), 1e-12, f"text case {i}")
+    monkeypatch.setenv("STEPPE_GENO_BLOCK_BYTES", str(64 * 30))
+    many, _ = _core.extract_f2_files(*_paths(pre), precision="fp64", **CASES[0])
+    monkeypatch.delenv("STEPPE_GENO_BLOCK_BYTES")
+    g = str(tmp_path / "g")
+    synthetic_triple(g, 16, n_snp=1500, fmt="geno")
+    packed, _ = _core.extract_f2_files(*_paths(g), precision="fp64", **CASES[0])
+    assert np.array_equal(many._f2_numpy(), packed._f2_numpy())
+    assert np.array_equal(many._vpair_numpy(), packed._vpair_numpy())

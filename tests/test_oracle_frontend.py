@@ -138,3 +138,27 @@ def test_plink_equals_geno_and_short_bed(cref, ref, tmp_path):
     for lib in (cref, ref):
         with pytest.raises(ValueError):
             lib.extract(*_plink(b), pops=["Pop_C", "Pop_D"])
+
+
+@pytest.mark.parametrize("case", [0, 1, 3])
+def test_eigenstrat_text_bit_identical_to_reference(cref, ref, tmp_path, case):
+    pre = str(tmp_path / "e")
+    synthetic_triple(pre, 16, n_snp=1500, fmt="ascii")
+    a = cref.extract(*_paths(pre), **CASES[case])
+    b = ref.extract(*_paths(pre), **CASES[case])
+    _same(a, b)
+    # the same panel packed as GENO gives the same tensor
+    g = str(tmp_path / "g")
+    synthetic_triple(g, 16, n_snp=1500, fmt="geno")
+    _same(a, cref.extract(*_paths(g), **CASES[case]))
+
+
+def test_eigenstrat_text_errors(cref, ref, tmp_path):
+    pre = str(tmp_path / "e")
+    synthetic_triple(pre, 17, n_snp=400, fmt="ascii")
+    raw = open(pre + ".geno", "rb").read().split(b"\n")
+    raw[7] = raw[7][:-1]                                  # one ragged line: not a genotype file
+    open(pre + ".geno", "wb").write(b"\n".join(raw))
+    for lib in (cref, ref):
+        with pytest.raises(ValueError):
+            lib.extract(*_paths(pre), pops=["Pop_C", "Pop_D"])
