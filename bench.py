@@ -340,6 +340,22 @@ def write_random_triple(prefix: str, fmt: str, pop_of, n_snp: int, seed: int):
     columns and 22 autosomes of 0.05-Morgan-spaced SNPs: input for timing the file path."""
     rng = np.random.default_rng(seed)
     n_ind = len(pop_of)
+    chrom = np.minimum(np.arange(n_snp) * 22 // n_snp + 1, 22)
+    gpos = np.arange(n_snp) * (0.05 * 22 * 30 / n_snp)
+    if fmt == "plink":
+        bpr = (n_ind + 3) // 4
+        with open(prefix + ".bed", "wb") as f:
+            f.write(bytes([0x6C, 0x1B, 0x01]))
+            step = max(1, (64 << 20) // bpr)
+            for r0 in range(0, n_snp, step):
+                f.write(rng.integers(0, 256, (min(step, n_snp - r0), bpr), dtype=np.uint8).tobytes())
+        with open(prefix + ".fam", "w") as f:
+            f.write("".join(f"F{g} I{g} 0 0 1 p{pop_of[g]:03d}\n" for g in range(n_ind)))
+        alle = np.array(list("ACGT"))
+        ra, aa = alle[rng.integers(0, 4, n_snp)], alle[rng.integers(0, 4, n_snp)]
+        with open(prefix + ".bim", "w") as f:
+            f.write("".join(f"{chrom[k]} s{k} {gpos[k]:.6f} {k * 100} {ra[k]} {aa[k]}\n" for k in range(n_snp)))
+        return
     if fmt == "tgeno":
         bpr = (n_snp + 3) // 4
         head = f"TGENO {n_ind} {n_snp} 0 0".encode().ljust(48, b"\0")
@@ -355,8 +371,6 @@ def write_random_triple(prefix: str, fmt: str, pop_of, n_snp: int, seed: int):
             f.write(rng.integers(0, 256, (min(step, n_rec - r0), bpr), dtype=np.uint8).tobytes())
     with open(prefix + ".ind", "w") as f:
         f.write("".join(f"I{g} U p{pop_of[g]:03d}\n" for g in range(n_ind)))
-    chrom = np.minimum(np.arange(n_snp) * 22 // n_snp + 1, 22)
-    gpos = np.arange(n_snp) * (0.05 * 22 * 30 / n_snp)
     alle = np.array(list("ACGT"))
     ra, aa = alle[rng.integers(0, 4, n_snp)], alle[rng.integers(0, 4, n_snp)]
     with open(prefix + ".snp", "w") as f:
@@ -364,8 +378,8 @@ def write_random_triple(prefix: str, fmt: str, pop_of, n_snp: int, seed: int):
 
 
 def frontend_bench(ctx, n_snp: int, cpu: bool):
-    """Genotype files -> resident f2: GENO (SNP-major, transposed on the device) and TGENO
-    triples of the c2 shape (200 populations, its individuals, n_snp SNPs), explicit
+    """Genotype files -> resident f2: GENO and PLINK .bed (SNP-major, transposed on the
+    device) and TGENO triples of the c2 shape (200 populations, its individuals, n_snp SNPs), explicit
     selection of every population, autosomes only, strand-ambiguous and non-ACGT pairs
     dropped, no population-missingness cap (random codes leave some single-sample
     population without data at almost every SNP), FP64 f2."""
@@ -377,22 +391,21 @@ def frontend_bench(ctx, n_snp: int, cpu: bool):
     pops = [f"p{i:03d}" for i in range(len(sizes))]
     out = {"metric": "genotype-file extract SNPs/sec", "unit": "SNPs/s", "cpu_baseline": None}
     with tempfile.TemporaryDirectory() as d:
-        for fmt in ("geno", "tgeno"):
+        for fmt in ("geno", "tgeno", "plink"):
             pre = os.path.join(d, fmt)
             t0 = time.perf_counter()
             write_random_triple(pre, fmt, pop_of, n_snp, SEED + 7)
             t_write = time.perf_counter() - t0
-            _core.extract_f2_files(pre + ".geno", pre + ".snp", pre + ".ind", pops[:3], maxmiss=1.0,
-                                   precision="fp64")                      # warm-up (code load)
+            files = _core.genotype_triple(pre)
+            _core.extract_f2_files(*files, pops[:3], maxmiss=1.0, precision="fp64")   # warm-up (code load)
             ctx.timing(True)
             ctx.read_timing(reset=True)
             t0 = time.perf_counter()
-            h, info = _core.extract_f2_files(pre + ".geno", pre + ".snp", pre + ".ind", pops, maxmiss=1.0,
-                                             precision="fp64")
+            h, info = _core.extract_f2_files(*files, pops, maxmiss=1.0, precision="fp64")
             dt = time.perf_counter() - t0
             tm = ctx.read_timing(reset=True)
             ctx.timing(False)
-            geno_bytes = os.path.getsize(pre + ".geno")
+            geno_bytes = os.path.getsize(files[0])
             log(f"[bench] frontend {fmt}: {n_snp} SNPs x {len(pop_of)} individuals ({geno_bytes / 1e9:.2f} GB) in "
                 f"{dt:.3f} s (files written in {t_write:.1f} s); kept {info['n_snp_kept']}; stages {tm}")
             out[fmt] = {"seconds": dt, "value": n_snp / dt, "geno_bytes": geno_bytes, "n_snp_kept": info["n_snp_kept"],
@@ -417,7 +430,7 @@ def frontend_bench(ctx, n_snp: int, cpu: bool):
                                              f"threaded ({os.path.basename(lib.path)})"}
             log(f"[bench] frontend cpu baseline {out['cpu_baseline']}")
     out["value"] = out["geno"]["value"]
-    out["workload"] = (f"GENO and TGENO triples of random codes, {len(sizes)} populations x {len(pop_of)} "
+    out["workload"] = (f"GENO, TGENO and PLINK .bed triples of random codes, {len(sizes)} populations x {len(pop_of)} "
                        f"individuals x {n_snp} SNPs, all populations selected, autosomes, strand-ambiguous / non-ACGT "
                        f"pairs dropped, maxmiss 1, FP64 f2; "
                        f"value = SNPs/s of the GENO file (read, transpose, ploidy, decode, keep, compact, f2)")
