@@ -735,4 +735,22 @@ void launch_qpadm(int T, const QpadmArgs& args, int n_workgroups, hipStream_t st
 
 int qpadm_threads() { return kQThreads; }
 
+int qpadm_blocks_per_cu(int T, std::size_t lds) {
+    int n = 0;
+    auto q = [&](auto kernel) {
+        if (lds > 64 * 1024)
+            STP_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        STP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kQThreads, lds));
+    };
+    switch (T) {
+        case 16: q(qpadm_kernel<16>); break;
+        case 32: q(qpadm_kernel<32>); break;
+        case 48: q(qpadm_kernel<48>); break;
+        case 64: q(qpadm_kernel<64>); break;
+        default: throw ArgError("qpadm: unsupported system size bucket " + std::to_string(T));
+    }
+    return n > 0 ? n : 1;
+}
+
 }  // namespace stp

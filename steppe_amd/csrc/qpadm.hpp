@@ -50,5 +50,7 @@ std::size_t qpadm_lds_bytes(int m_max, int ldq, int wave_slice);
 int qpadm_wave_slice(int nr_max);
 void launch_qpadm(int T, const QpadmArgs& args, int n_workgroups, hipStream_t st);
 int qpadm_threads();
+// Workgroups of qpadm_kernel<T> that fit on one CU at once with lds bytes of dynamic LDS.
+int qpadm_blocks_per_cu(int T, std::size_t lds);
 
 }  // namespace stp

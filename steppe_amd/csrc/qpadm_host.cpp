@@ -131,11 +131,22 @@ void HipBackend::qpadm_batch(const F2Blocks& h, int n_models, const std::int32_t
     // ---- launches per bucket ----
     qp_out_.ensure(static_cast<std::size_t>(n_models));
     STP_HIP(hipMemsetAsync(qp_out_.get(), 0, static_cast<std::size_t>(n_models) * sizeof(stp_qpadm_result), stream_));
-    const int n_wg_max = 2 * n_cu_;
-    int m_all = 0;
-    for (const auto& s : shape) m_all = std::max(m_all, s.m);
-    const std::int64_t stride = static_cast<std::int64_t>(2 * m_all + STP_QPADM_MAX_NL) * nbs;
-    qp_scratch_.ensure(static_cast<std::size_t>(stride) * n_wg_max);
+    // Each launch group fills every CU to the kernel's occupancy; every workgroup has its
+    // own scratch slice (X, LOO and the LOO weights), sized for the largest group.
+    std::size_t scratch_elems = 0;
+    for (int slot = 0; slot < n_models;) {
+        const int T = shape[order[slot]].T;
+        int end = slot, m_max = 0, nr_max = 0;
+        for (; end < n_models && shape[order[end]].T == T; ++end) {
+            m_max = std::max(m_max, shape[order[end]].m);
+            nr_max = std::max(nr_max, shape[order[end]].nr);
+        }
+        const std::size_t lds = qpadm_lds_bytes(m_max, m_max + 1, qpadm_wave_slice(nr_max));
+        const std::int64_t n_wg = std::min<std::int64_t>(end - slot, qpadm_blocks_per_cu(T, lds) * n_cu_);
+        scratch_elems = std::max(scratch_elems, static_cast<std::size_t>((2 * m_max + STP_QPADM_MAX_NL) * nbs * n_wg));
+        slot = end;
+    }
+    qp_scratch_.ensure(scratch_elems);
     counter_.ensure(64);
     mark_begin(kQpadm);
     int slot = 0;
@@ -180,7 +191,7 @@ void HipBackend::qpadm_batch(const F2Blocks& h, int n_models, const std::int32_t
         const std::size_t lds = qpadm_lds_bytes(a.m_max, a.ldq, a.wave_slice);
         STP_REQUIRE(lds <= 96 * 1024, "qpadm: model sizes need more LDS than a workgroup has");
         launch_zero_u32(a.counter, 1, stream_);
-        launch_qpadm(T, a, std::min(n_wg_max, a.n_order), stream_);
+        launch_qpadm(T, a, std::min(qpadm_blocks_per_cu(T, lds) * n_cu_, a.n_order), stream_);
         slot = end;
         ++bucket_id;
         STP_REQUIRE(bucket_id < 32, "qpadm: too many launch groups");
