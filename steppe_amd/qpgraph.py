@@ -118,11 +118,19 @@ class Topology:
         d = np.zeros(6, np.int32)
         _core.check(_lib().stp_qpgraph_topology_dims(h, _p(d)))
         self.npop, self.nedge, self.nadmix, self.npair, self.npath, self.base = (int(x) for x in d)
-        self.leaves = self._labels(0, self.npop)
-        self.admix_from = self._labels(1, self.nadmix)
-        self.admix_to = self._labels(2, self.nadmix)
-        self.edge_from = self._labels(3, self.nedge)
-        self.edge_to = self._labels(4, self.nedge)
+        self._names = {}
+
+    # label lists are read from the engine on first use (a fleet of graphs needs none)
+    leaves = property(lambda self: self._label_list(0, self.npop))
+    admix_from = property(lambda self: self._label_list(1, self.nadmix))
+    admix_to = property(lambda self: self._label_list(2, self.nadmix))
+    edge_from = property(lambda self: self._label_list(3, self.nedge))
+    edge_to = property(lambda self: self._label_list(4, self.nedge))
+
+    def _label_list(self, kind, n):
+        if kind not in self._names:
+            self._names[kind] = self._labels(kind, n)
+        return list(self._names[kind])
 
     def _labels(self, kind, n):
         out = []
