@@ -153,7 +153,7 @@ int stp_f2_handle_tier(const stp_f2_handle* h, int32_t* tier);
  * blgsize: jackknife block size in Morgans (0.05); the 2 Mb physical window is used when
  * every genetic position is 0.  ploidy: 0 detect per sample, 1 pseudo-haploid, 2 diploid. */
 typedef struct stp_extract_request {
-    const char* geno;            /* TGENO, GENO (PACKEDANCESTRYMAP) or EIGENSTRAT text .geno, or a PLINK .bed */
+    const char* geno;            /* TGENO, GENO, EIGENSTRAT text or ANCESTRYMAP .geno, or a PLINK .bed */
     const char* snp;             /* .snp, or the PLINK .bim */
     const char* ind;             /* .ind, or the PLINK .fam (phenotype column = population) */
     int32_t pop_mode;

@@ -14,6 +14,8 @@
 //   GENO transpose          src/io/geno_reader.cpp:375-505 + src/device/backend.hpp:790-830
 //                           (ComputeBackend::transpose_to_canonical)
 //   EIGENSTRAT text .geno   src/io/geno_reader.cpp:83-120 (geometry), 516-583 (0/1/2/9 per char)
+//   ANCESTRYMAP .geno       src/io/geno_reader.cpp:122-177, 648-744 (<snp> <ind> <0/1/2/-1> lines,
+//                           by position; geometry from the .snp / .ind beside the .geno)
 //   PLINK .bed              src/io/geno_reader.cpp:48-72, 208-238 (magic 6c 1b 01, geometry from
 //                           the .bim / .fam beside the .bed), 585-634 (LSB-first codes,
 //                           kBedToCanon = {2, 3, 1, 0}, src/io/eigenstrat_format.hpp:66-77)
@@ -135,7 +137,25 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
         }
         ascii = ok && !text_lines.empty() && !text_lines[0].empty();
     }
-    if (!packed_hdr && !plink && !ascii) throw std::runtime_error("not TGENO / GENO / EIGENSTRAT / PLINK .bed");
+    // ANCESTRYMAP: the first non-blank line is "<snp> <ind> <genotype>"
+    bool amap = false;
+    std::string amap_base;
+    if (!packed_hdr && !plink && !ascii) {
+        std::ifstream t(geno, std::ios::binary);
+        std::string l;
+        std::vector<std::string> w;
+        while (std::getline(t, l)) {
+            w = split(l);
+            if (!w.empty()) break;
+        }
+        const bool tok_ok = w.size() == 3 && ((w[2].size() == 1 && w[2][0] >= '0' && w[2][0] <= '2') || w[2] == "-1");
+        amap_base = geno;
+        if (amap_base.size() >= 5 && amap_base.compare(amap_base.size() - 5, 5, ".geno") == 0)
+            amap_base.resize(amap_base.size() - 5);
+        amap = tok_ok;
+    }
+    if (!packed_hdr && !plink && !ascii && !amap)
+        throw std::runtime_error("not TGENO / GENO / EIGENSTRAT / PLINK .bed / ANCESTRYMAP");
     if (plink && static_cast<unsigned char>(head[2]) != 0x01) throw std::runtime_error("individual-major .bed");
     const bool tgeno = packed_hdr && tok[0] == "TGENO";
     std::size_t counts[2] = {0, 0};
@@ -163,6 +183,20 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
     } else if (ascii) {
         counts[0] = text_lines[0].size();
         counts[1] = text_lines.size();
+    } else if (amap) {
+        auto lines = [](const std::string& p) {
+            std::ifstream in(p, std::ios::binary);
+            std::size_t n = 0;
+            std::string l;
+            while (std::getline(in, l)) {
+                if (!l.empty() && l.back() == '\r') l.pop_back();
+                if (!l.empty()) ++n;
+            }
+            return n;
+        };
+        counts[1] = lines(amap_base + ".snp");
+        counts[0] = lines(amap_base + ".ind");
+        if (counts[0] == 0 || counts[1] == 0) throw std::runtime_error("ANCESTRYMAP .snp / .ind missing or empty");
     } else {
         int got = 0;
         std::size_t i = text.find(tok[0]) + tok[0].size();
@@ -181,7 +215,7 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
     const std::size_t n_records = tgeno ? n_ind : n_snp;
     g.seekg(0, std::ios::end);
     const std::size_t fsize = static_cast<std::size_t>(g.tellg());
-    const std::size_t present = ascii ? n_records : std::min((fsize - hdr) / bpr, n_records);
+    const std::size_t present = (ascii || amap) ? n_records : std::min((fsize - hdr) / bpr, n_records);
 
     // ---- .ind ----
     std::ifstream fi(ind);
@@ -304,6 +338,22 @@ Extract* run(const char* geno, const char* snp, const char* ind, int pop_mode, c
         for (std::size_t k = 0; k < sel.size(); ++k) {
             g.seekg(static_cast<std::streamoff>(hdr + sel[k] * bpr));
             g.read(reinterpret_cast<char*>(tile.data() + k * cbpr), static_cast<std::streamsize>(cbpr));
+        }
+    } else if (amap) {
+        std::ifstream t(geno, std::ios::binary);
+        std::string l;
+        std::vector<std::uint8_t> rec(n_ind);
+        for (std::size_t s = 0; s < M0; ++s) {
+            for (std::size_t i = 0; i < n_ind; ++i) {
+                if (!std::getline(t, l)) throw std::runtime_error("ANCESTRYMAP: unexpected end of file");
+                const auto w = split(l);
+                if (w.size() != 3) throw std::runtime_error("ANCESTRYMAP: malformed line");
+                if (w[2] == "-1") rec[i] = 3;
+                else if (w[2].size() == 1 && w[2][0] >= '0' && w[2][0] <= '2') rec[i] = static_cast<std::uint8_t>(w[2][0] - '0');
+                else throw std::runtime_error("ANCESTRYMAP: illegal genotype");
+            }
+            for (std::size_t k = 0; k < sel.size(); ++k)
+                tile[k * cbpr + s / 4] |= static_cast<std::uint8_t>(rec[sel[k]] << (6 - 2 * (s % 4)));
         }
     } else if (ascii) {
         for (std::size_t s = 0; s < M0; ++s)

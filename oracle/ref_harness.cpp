@@ -398,6 +398,8 @@ void* ref_extract_open(const char* geno, const char* snp, const char* ind, int p
             const io::SnpMajorTile src = fmt == io::GenoFormat::Plink ? reader.read_plink_snp_major_tile(part, 0, M0)
                                        : fmt == io::GenoFormat::Eigenstrat
                                            ? reader.read_eigenstrat_snp_major_tile(part, 0, M0)
+                                       : fmt == io::GenoFormat::Ancestrymap
+                                           ? reader.read_ancestrymap_snp_major_tile(part, 0, M0)
                                            : reader.read_snp_major_tile(part, 0, M0);
             steppe::SnpMajorTileView v;
             v.snp_major = src.snp_major.data();

@@ -126,7 +126,8 @@ F2Blocks* HipBackend::extract_f2_files(const ExtractRequest& req, ExtractInfo& i
         fe_stage_.ensure(blk * bpr);
         std::vector<std::uint8_t> host;
         std::unique_ptr<AsciiGenoReader> text;
-        if (f.header.format == GenoFormat::Eigenstrat) text = std::make_unique<AsciiGenoReader>(f);
+        if (f.header.format == GenoFormat::Eigenstrat || f.header.format == GenoFormat::Ancestrymap)
+            text = std::make_unique<AsciiGenoReader>(f);
         for (std::size_t s0 = 0; s0 < M0; s0 += blk) {
             const std::size_t s1 = std::min(M0, s0 + blk);
             if (text) text->next(s1 - s0, host);

@@ -104,6 +104,45 @@ GenoHeader eigenstrat_geometry(const std::string& path) {
     return h;
 }
 
+bool ancestrymap_code(const std::string& tok, std::uint8_t& code) {
+    if (tok.size() == 1 && tok[0] >= '0' && tok[0] <= '2') {
+        code = static_cast<std::uint8_t>(tok[0] - '0');
+        return true;
+    }
+    if (tok == "-1") {
+        code = 3;
+        return true;
+    }
+    return false;
+}
+
+// ANCESTRYMAP: the first non-blank line has three tokens ending in a genotype, and the
+// .snp / .ind beside the .geno give the geometry.
+GenoHeader ancestrymap_geometry(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return GenoHeader{};
+    std::string line;
+    std::vector<std::string> t;
+    while (std::getline(in, line)) {
+        t = words(line);
+        if (!t.empty()) break;
+    }
+    std::uint8_t code;
+    if (t.size() != 3 || !ancestrymap_code(t[2], code)) return GenoHeader{};
+    std::string base = path;
+    if (base.size() >= 5 && base.compare(base.size() - 5, 5, ".geno") == 0) base.resize(base.size() - 5);
+    const std::size_t n_snp = count_lines(base + ".snp"), n_ind = count_lines(base + ".ind");
+    if (n_snp == 0 || n_ind == 0) return GenoHeader{};
+    GenoHeader h;
+    h.format = GenoFormat::Ancestrymap;
+    h.n_ind = n_ind;
+    h.n_snp = n_snp;
+    h.n_records = n_snp;
+    h.bytes_per_record = packed_bytes(n_ind);
+    h.header_bytes = 0;
+    return h;
+}
+
 struct RawGroup {
     std::string label;
     std::size_t first;
@@ -223,7 +262,8 @@ GenoFile open_geno(const std::string& path) {
     const bool bed_magic = got >= 3 && m[0] == kBedMagic0 && m[1] == kBedMagic1;
     // (a file opening with the .bed magic cannot be EIGENSTRAT text: 'l' is not a genotype)
     if (f.header.format == GenoFormat::Unknown && !bed_magic) f.header = eigenstrat_geometry(path);
-    if (f.header.format == GenoFormat::Eigenstrat) {
+    if (f.header.format == GenoFormat::Unknown && !bed_magic) f.header = ancestrymap_geometry(path);
+    if (f.header.format == GenoFormat::Eigenstrat || f.header.format == GenoFormat::Ancestrymap) {
         f.records_present = f.header.n_records;
         return f;
     }
@@ -389,7 +429,8 @@ void read_tgeno_selected(const GenoFile& f, const std::vector<PopGroup>& groups,
 }
 
 AsciiGenoReader::AsciiGenoReader(const GenoFile& f) : f_(f), in_(f.path, std::ios::binary) {
-    STP_REQUIRE(f.header.format == GenoFormat::Eigenstrat, "geno: not an EIGENSTRAT text file");
+    STP_REQUIRE(f.header.format == GenoFormat::Eigenstrat || f.header.format == GenoFormat::Ancestrymap,
+                "geno: not an EIGENSTRAT or ANCESTRYMAP text file");
     if (!in_) io_fail("geno: cannot reopen " + f.path);
 }
 
@@ -397,6 +438,21 @@ void AsciiGenoReader::next(std::size_t n, std::vector<std::uint8_t>& out) {
     const std::size_t bpr = f_.header.bytes_per_record, n_ind = f_.header.n_ind;
     out.assign(n * bpr, 0);
     std::string line;
+    if (f_.header.format == GenoFormat::Ancestrymap) {
+        for (std::size_t s = 0; s < n; ++s) {
+            std::uint8_t* rec = out.data() + s * bpr;
+            for (std::size_t i = 0; i < n_ind; ++i) {
+                if (!std::getline(in_, line)) io_fail("geno: unexpected end of " + f_.path);
+                ++line_;
+                const auto t = words(line);
+                std::uint8_t code = 0;
+                if (t.size() != 3 || !ancestrymap_code(t[2], code))
+                    io_fail("geno: malformed genotype line " + std::to_string(line_) + " of " + f_.path);
+                rec[i >> 2] |= static_cast<std::uint8_t>(code << (6 - 2 * (i & 3)));
+            }
+        }
+        return;
+    }
     for (std::size_t s = 0; s < n; ++s) {
         if (!std::getline(in_, line)) io_fail("geno: unexpected end of " + f_.path + " at SNP line " + std::to_string(line_ + 1));
         ++line_;

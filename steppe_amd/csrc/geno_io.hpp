@@ -11,6 +11,10 @@
 //   .snp: "<id> <chrom> <genpos (Morgans)> [<physpos> [<ref> <alt>]]" per SNP.
 // EIGENSTRAT ASCII (src/io/geno_reader.cpp:83-120, 516-583): one line per SNP of n_ind
 //   characters 0 / 1 / 2 (copies of ref) or 9 (missing), every line the same length.
+// ANCESTRYMAP (src/io/geno_reader.cpp:122-177, 648-744): "<snp_id> <sample_id> <genotype>"
+//   lines, genotype 0 / 1 / 2 or -1 (missing), SNP-major by position (line L is SNP
+//   L / n_ind, individual L % n_ind; the ids are not looked up), n_snp and n_ind counted
+//   from the .snp and .ind beside the .geno.
 // PLINK (src/io/geno_reader.cpp:208-238, 585-634; src/io/plink_reader.cpp): a SNP-major
 //   .bed (magic 6c 1b 01, one record of ceil(n_ind/4) bytes per SNP, individual i in bits
 //   2(i%4) of byte i/4, codes 00 two copies of A1, 01 missing, 10 heterozygous, 11 none),
@@ -28,7 +32,7 @@
 
 namespace stp {
 
-enum class GenoFormat { Unknown = 0, Tgeno = 1, Geno = 2, Plink = 3, Eigenstrat = 4 };
+enum class GenoFormat { Unknown = 0, Tgeno = 1, Geno = 2, Plink = 3, Eigenstrat = 4, Ancestrymap = 5 };
 
 struct GenoHeader {
     GenoFormat format = GenoFormat::Unknown;
@@ -86,8 +90,8 @@ SnpTable read_bim(const std::string& path);
 // groups' order, into out (n_sel x that many bytes) (src/io/geno_reader.cpp:274-372).
 void read_tgeno_selected(const GenoFile& f, const std::vector<PopGroup>& groups, std::size_t n_snp,
                          std::vector<std::uint8_t>& out);
-// EIGENSTRAT ASCII, read line by line in order: the next n SNP lines packed into
-// canonical SNP-major records of ceil(n_ind/4) bytes (first individual in the top bits).
+// EIGENSTRAT or ANCESTRYMAP text, read in order: the next n SNPs packed into canonical
+// SNP-major records of ceil(n_ind/4) bytes (first individual in the top bits).
 class AsciiGenoReader {
 public:
     explicit AsciiGenoReader(const GenoFile& f);

@@ -101,6 +101,14 @@ def synthetic_triple(prefix: str, seed: int, n_snp: int = 3000, fmt: str = "tgen
            "ref": list(ref), "alt": list(alt)}
     if fmt == "ascii":
         write_eigenstrat_text(prefix, codes, ind_pops, snp)
+    elif fmt == "ancestrymap":
+        write_eigenstrat_text(prefix, codes, ind_pops, snp)
+        tok = np.array(["0", "1", "2", "-1"])
+        n_ind, n_snp = codes.shape
+        with open(prefix + ".geno", "w") as f:
+            for s in range(n_snp):
+                t = tok[codes[:, s]]
+                f.write("".join(f"rs{s} I{i:05d} {t[i]}\n" for i in range(n_ind)))
     elif fmt == "plink":
         # group labels as .fam phenotypes: two of them become the reserved codes 1 / 2
         # (Control / Case) and one is the skipped missing phenotype -9

@@ -159,3 +159,10 @@ def test_eigenstrat_text_matches_oracle(cref, tmp_path, monkeypatch):
     packed, _ = _core.extract_f2_files(*_paths(g), precision="fp64", **CASES[0])
     assert np.array_equal(many._f2_numpy(), packed._f2_numpy())
     assert np.array_equal(many._vpair_numpy(), packed._vpair_numpy())
+
+
+def test_ancestrymap_matches_oracle(cref, tmp_path):
+    pre = str(tmp_path / "a")
+    synthetic_triple(pre, 11, n_snp=2400, fmt="ancestrymap")
+    h, info = _core.extract_f2_files(*_paths(pre), precision="fp64", **CASES[0])
+    check(h, info, cref.extract(*_paths(pre), **_oracle_kw(CASES[0])), 1e-12, "ancestrymap")

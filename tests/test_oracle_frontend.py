@@ -162,3 +162,17 @@ def test_eigenstrat_text_errors(cref, ref, tmp_path):
     for lib in (cref, ref):
         with pytest.raises(ValueError):
             lib.extract(*_paths(pre), pops=["Pop_C", "Pop_D"])
+
+
+@pytest.mark.parametrize("case", [0, 2])
+def test_ancestrymap_bit_identical_to_reference(cref, ref, tmp_path, case):
+    """ANCESTRYMAP "<snp> <ind> <genotype>" triples: the oracle equals the reference's
+    reader chain, and the packed GENO of the same panel."""
+    pre = str(tmp_path / "a")
+    synthetic_triple(pre, 18, n_snp=600, fmt="ancestrymap")
+    a = cref.extract(*_paths(pre), **CASES[case])
+    b = ref.extract(*_paths(pre), **CASES[case])
+    _same(a, b)
+    g = str(tmp_path / "g")
+    synthetic_triple(g, 18, n_snp=600, fmt="geno")
+    _same(a, cref.extract(*_paths(g), **CASES[case]))
