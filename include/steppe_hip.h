@@ -206,7 +206,8 @@ int stp_f2_handle_upload(stp_context* ctx, const double* f2, const double* vpair
                          stp_f2_handle** out);
 int stp_f2_handle_shape(const stp_f2_handle* h, int32_t* P, int32_t* n_block);
 int stp_f2_handle_block_sizes(const stp_f2_handle* h, int32_t* block_sizes);
-/* Device-resident tensors (reference: DeviceF2Blocks::f2_device / vpair_device). */
+/* Device-resident tensors (reference: DeviceF2Blocks::f2_device / vpair_device); NULL for a
+ * handle whose tensor is in the host or disk tier. */
 const double* stp_f2_handle_f2_device(const stp_f2_handle* h);
 const double* stp_f2_handle_vpair_device(const stp_f2_handle* h);
 /* Copy back to host (reference: DeviceF2Blocks::to_host). */
