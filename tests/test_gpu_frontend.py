@@ -110,6 +110,13 @@ def test_errors(tmp_path):
     synthetic_triple(pre, 31, n_snp=800, fmt="geno", records_missing=5)
     with pytest.raises(ValueError):
         _core.extract_f2_files(*_paths(pre), ["Pop_A"])
+    # an individual-major PLINK .bed (mode byte 0) is refused, as by the reference
+    synthetic_triple(pre, 31, n_snp=400, fmt="plink")
+    raw = bytearray(open(pre + ".bed", "rb").read())
+    raw[2] = 0x00
+    open(pre + ".bed", "wb").write(bytes(raw))
+    with pytest.raises(ValueError, match="individual-major"):
+        _core.extract_f2_files(pre + ".bed", pre + ".bim", pre + ".fam", ["Pop_C"])
 
 
 PLINK_CASES = [

@@ -176,3 +176,14 @@ def test_ancestrymap_bit_identical_to_reference(cref, ref, tmp_path, case):
     g = str(tmp_path / "g")
     synthetic_triple(g, 18, n_snp=600, fmt="geno")
     _same(a, cref.extract(*_paths(g), **CASES[case]))
+
+
+def test_individual_major_bed_is_rejected(cref, ref, tmp_path):
+    pre = str(tmp_path / "b")
+    synthetic_triple(pre, 19, n_snp=400, fmt="plink")
+    raw = bytearray(open(pre + ".bed", "rb").read())
+    raw[2] = 0x00                                          # mode byte: individual-major
+    open(pre + ".bed", "wb").write(bytes(raw))
+    for lib in (cref, ref):
+        with pytest.raises(ValueError):
+            lib.extract(pre + ".bed", pre + ".bim", pre + ".fam", pops=["Pop_C", "Pop_D"])

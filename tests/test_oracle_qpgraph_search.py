@@ -152,3 +152,19 @@ def test_reference_search_fixture_is_pinned():
 def test_search_argument_errors():
     with pytest.raises(ValueError):
         qg.run_qpgraph_search(type("H", (), {"pops": POPS5})(), POPS5[:2])
+
+
+def test_enumeration_edge_cases(ref):
+    # one and two leaves: a single (edge-free / cherry) tree, no admixture graph, as the reference
+    for n in (1, 2):
+        leaves = [f"L{i}" for i in range(n)]
+        ours, theirs = qg.enumerate_graphs(leaves, 1), ref.qg_enumerate(leaves, 1)
+        assert [(a["nadmix"], a["hash"], a["edges"]) for a in ours] == \
+               [(b["nadmix"], b["hash"], b["edges"]) for b in theirs]
+    assert qg.graph_hash([]) == ref.qg_graph_hash([])
+    with pytest.raises(ValueError):
+        qg.enumerate_graphs([], 1)
+    with pytest.raises(ValueError):
+        qg.enumerate_graphs([f"L{i}" for i in range(10)], 1)
+    with pytest.raises(ValueError):
+        qg.topology_neighbors({"edges": [("R", "A"), ("R", "B")], "nadmix": 2}, ["A", "B"], 1)
