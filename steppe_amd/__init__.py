@@ -30,12 +30,23 @@ from ._core import (  # noqa: F401
 __version__ = "0.1.0"
 
 
-def extract_f2(packed, n_snp, pop_offsets, block_id, n_block, pops=None, out="", device=0,
-               ploidy="auto", precision=None, sample_ploidy=None):
-    """Facade over ``_core.run_extract_f2`` (bindings/steppe/__init__.py:493-559)."""
-    return run_extract_f2(packed, n_snp, pop_offsets, block_id, n_block, pops=pops, out=out,
-                          device=device, ploidy=ploidy, precision=precision,
-                          sample_ploidy=sample_ploidy)
+def extract_f2(prefix, *args, pops=None, out=None, device=0, blgsize=0.05, maf=0.0, maxmiss=0.0,
+               autosomes_only=True, drop_monomorphic=False, transversions_only=False, ploidy="auto",
+               strand_mode="drop", precision=None, **kw):
+    """Facade over ``_core.run_extract_f2`` (bindings/steppe/__init__.py:493-559):
+    ``extract_f2(prefix, pops=[...], out=None, ...)`` reads ``<prefix>.{geno,snp,ind}`` (or
+    the PLINK ``.bed/.bim/.fam``) and returns a resident F2Handle, or the directory path
+    when ``out`` is given.  Called with an in-memory packed tile instead of a prefix
+    (``extract_f2(packed, n_snp, pop_offsets, block_id, n_block, pops=..., ...)``) it runs
+    the tile form."""
+    import os
+    if isinstance(prefix, (str, os.PathLike)):
+        if args:
+            raise TypeError("extract_f2(prefix, ...): pass the options as keywords")
+        return run_extract_f2(str(prefix), pops, out or "", device, blgsize, maf, maxmiss, autosomes_only,
+                              drop_monomorphic, transversions_only, ploidy, strand_mode, precision)
+    return run_extract_f2(prefix, *args, pops=pops, out=out or "", device=device, ploidy=ploidy,
+                          precision=precision, **kw)
 
 
 def qpadm(f2, target, left, right, **kw):

@@ -87,6 +87,9 @@ def test_prefix_call_and_output_dir(cref, tmp_path):
     pops = ["Pop_C", "Pop_A", "Pop_I"]
     h = _core.run_extract_f2(pre, pops, "", 0, 0.05, 0.0, 1.0, True, False, False, "auto", "drop", "fp64")
     assert h.pops == sorted(pops)
+    # the package facade takes the reference's keyword form
+    hf = st.extract_f2(pre, pops=pops, maxmiss=1.0, precision="fp64")
+    assert hf.pops == h.pops and np.array_equal(hf._f2_numpy(), h._f2_numpy())
     out = str(tmp_path / "f2dir")
     path, info = _core.extract_f2_files(*_paths(pre), pops, out=out, maxmiss=1.0, precision="fp64")
     assert path == out
