@@ -27,6 +27,8 @@ from ._core import (  # noqa: F401
     write_f2_dir,
 )
 
+import enum
+
 __version__ = "0.1.0"
 
 
@@ -49,14 +51,130 @@ def extract_f2(prefix, *args, pops=None, out=None, device=0, blgsize=0.05, maf=0
                           precision=precision, **kw)
 
 
-def qpadm(f2, target, left, right, **kw):
-    """Facade over ``_core.run_qpadm`` (bindings/steppe/__init__.py qpadm)."""
-    return run_qpadm(f2, target, left, right, **kw)
+class Status(enum.Enum):
+    """Result status values of the fits (bindings/steppe/__init__.py:146-160)."""
+    OK = "ok"
+    DEVICE_OOM = "device_oom"
+    RANK_DEFICIENT = "rank_deficient"
+    NON_SPD_COVARIANCE = "non_spd_covariance"
+    CHISQ_UNDEFINED = "chisq_undefined"
+    INVALID_CONFIG = "invalid_config"
 
 
-def qpadm_search(f2, target, lefts, right, **kw):
-    """Facade over ``_core.run_qpadm_search``."""
-    return run_qpadm_search(f2, target, lefts, right, **kw)
+def _frame(cols: dict):
+    import pandas as pd
+    return pd.DataFrame({k: list(v) for k, v in cols.items()})
+
+
+class QpAdmResult:
+    """One qpAdm fit with the reference facade's attributes (bindings/steppe/__init__.py:
+    193-270): weight / se / z lists, p, chisq, dof, est_rank, f4rank, status, precision,
+    model_index, feasible, and the weights / rankdrop / popdrop DataFrames."""
+
+    def __init__(self, d: dict, *, target, left):
+        self._d = d
+        self.target, self.left = target, list(left)
+        self.weight, self.se, self.z = list(d["weight"]), list(d["se"]), list(d["z"])
+        self.p, self.chisq, self.dof = d["p"], d["chisq"], d["dof"]
+        self.est_rank, self.f4rank = d["est_rank"], d["f4rank"]
+        self.status = Status(d["status"])
+        self.precision, self.model_index = d["precision"], d["model_index"]
+        pf = d.get("popdrop_feasible") or []
+        w = self.weight
+        self.feasible = bool(pf[0]) if pf else (bool(w) and all(0.0 <= x <= 1.0 for x in w))
+
+    @property
+    def weights(self):
+        n = len(self.weight)
+        nan = float("nan")
+        return _frame({"target": [self.target] * n, "left": self.left, "weight": self.weight,
+                       "se": self.se if len(self.se) == n else [nan] * n,
+                       "z": self.z if len(self.z) == n else [nan] * n})
+
+    @property
+    def rankdrop(self):
+        d = self._d
+        return _frame({c: d["rankdrop_" + c] for c in ("f4rank", "dof", "chisq", "p", "dofdiff", "chisqdiff",
+                                                       "p_nested")})
+
+    @property
+    def popdrop(self):
+        d = self._d
+        return _frame({c: d["popdrop_" + c] for c in ("pat", "wt", "dof", "f4rank", "chisq", "p", "feasible")})
+
+    def __repr__(self) -> str:
+        return (f"QpAdmResult(target={self.target!r}, left={self.left!r}, weight={self.weight!r}, p={self.p!r}, "
+                f"f4rank={self.f4rank}, feasible={self.feasible}, status={self.status.name})")
+
+
+def qpadm(f2, target=None, left=None, right=None, *, rank=-1, fudge=1e-4, als_iterations=20, rank_alpha=0.05,
+          allow_negative_weights=True):
+    """One qpAdm model (bindings/steppe/__init__.py:604-640): a QpAdmResult."""
+    d = run_qpadm(f2, target, left, right, rank=rank, fudge=fudge, als_iterations=als_iterations,
+                  rank_alpha=rank_alpha, allow_negative_weights=allow_negative_weights)
+    return QpAdmResult(d, target=target, left=left)
+
+
+def qpadm_search(f2, target=None, models=None, right=None, *, rank=-1, fudge=1e-4, als_iterations=20,
+                 rank_alpha=0.05, allow_negative_weights=True, jackknife="all", as_dataframe=False, **kw):
+    """Batched qpAdm over candidate source sets (bindings/steppe/__init__.py:1103-1160):
+    ``models`` lists source sets (a list of labels, or {"left": [...]}); a QpAdmResult
+    per model in input order, or one DataFrame with ``as_dataframe``."""
+    if models is None:
+        models = kw.pop("lefts", None)
+    lefts = [list(m["left"]) if isinstance(m, dict) else list(m) for m in models]
+    ds = run_qpadm_search(f2, target, lefts, right, rank=rank, fudge=fudge, als_iterations=als_iterations,
+                          rank_alpha=rank_alpha, allow_negative_weights=allow_negative_weights,
+                          jackknife=jackknife, **kw)
+    res = [QpAdmResult(d, target=target, left=left) for d, left in zip(ds, lefts)]
+    if not as_dataframe:
+        return res
+    return _frame({"model": list(range(len(res))), "left": [",".join(r.left) for r in res],
+                   "p": [r.p for r in res], "chisq": [r.chisq for r in res], "f4rank": [r.f4rank for r in res],
+                   "feasible": [r.feasible for r in res], "status": [r.status.value for r in res],
+                   "weight": [r.weight for r in res], "se": [r.se for r in res]})
+
+
+class QpGraphResult:
+    """One qpGraph fit with the reference facade's attributes (bindings/steppe/__init__.py:
+    646-693): score, restart_spread, status, worst_residual_z, and the weights / edges
+    DataFrames."""
+
+    def __init__(self, d: dict):
+        self._d = d
+
+    score = property(lambda self: float(self._d["score"]))
+    restart_spread = property(lambda self: float(self._d["restart_spread"]))
+    status = property(lambda self: Status(self._d["status"]))
+    worst_residual_z = property(lambda self: float(self._d["worst_residual_z"]))
+
+    @property
+    def weights(self):
+        d = self._d
+        return _frame({"from": d["admix_from"], "to": d["admix_to"], "weight": d["weight"], "low": d["weight_lo"],
+                       "high": d["weight_hi"]})
+
+    @property
+    def edges(self):
+        d = self._d
+        return _frame({"from": d["edge_from"], "to": d["edge_to"], "length": d["edge_length"]})
+
+    def __repr__(self) -> str:
+        return (f"QpGraphResult(score={self.score:.6f}, nadmix={len(self._d['weight'])}, "
+                f"nedge={len(self._d['edge_length'])}, status={self.status.name})")
+
+
+def _qpgraph_fit(f2, edges, *, numstart=10, fudge=1e-4, diag_f3=1e-5, constrained=True):
+    """One admixture-graph fit on the device (bindings/steppe/__init__.py:696-718): a
+    QpGraphResult.  Reached as ``steppe_amd.qpgraph(f2, edges, ...)``: the qpgraph
+    submodule is callable, so the reference's ``steppe.qpgraph(...)`` call and
+    ``from steppe_amd import qpgraph`` (the module) both work."""
+    from .qpgraph import run_qpgraph
+    return QpGraphResult(run_qpgraph(f2, edges, numstart=numstart, fudge=fudge, diag_f3=diag_f3,
+                                     constrained=constrained))
+
+
+from . import qpgraph  # noqa: E402  (the callable submodule, after the facade it calls)
 
 
 class FstatTable:

@@ -21,6 +21,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -528,3 +529,15 @@ def run_qpgraph_search(f2, pops, max_nadmix=1, numstart=10, fudge=1e-4, diag_f3=
         return d
     finally:
         L.stp_qpgraph_search_free(h)
+
+
+class _CallableModule(type(sys.modules[__name__])):
+    """This module, callable as the reference facade's ``qpgraph(f2, edges, ...)``
+    (bindings/steppe/__init__.py:696): returns a QpGraphResult."""
+
+    def __call__(self, f2, edges, **kw):
+        from . import _qpgraph_fit
+        return _qpgraph_fit(f2, edges, **kw)
+
+
+sys.modules[__name__].__class__ = _CallableModule
