@@ -232,9 +232,39 @@ def f4ratio(f2, tuples, *, as_dataframe=False):
     return res.table if as_dataframe else res
 
 
+class QpWaveResult:
+    """A qpWave rank sweep with the reference facade's attributes
+    (bindings/steppe/__init__.py:273-318): f4rank, est_rank, status, precision, and the
+    rankdrop / per_rank DataFrames."""
+
+    def __init__(self, d: dict, *, left):
+        self._d = d
+        self.left = list(left)
+        self.f4rank, self.est_rank = d["f4rank"], d["est_rank"]
+        self.status = Status(d["status"])
+        self.precision = d["precision"]
+
+    @property
+    def rankdrop(self):
+        d = self._d
+        return _frame({c: d["rankdrop_" + c] for c in ("f4rank", "dof", "chisq", "p", "dofdiff", "chisqdiff",
+                                                       "p_nested")})
+
+    @property
+    def per_rank(self):
+        d = self._d
+        return _frame({"rank": range(len(d["rank_chisq"])), "chisq": d["rank_chisq"], "dof": d["rank_dof"],
+                       "p": d["rank_p"]})
+
+    def __repr__(self) -> str:
+        return (f"QpWaveResult(left={self.left!r}, f4rank={self.f4rank}, est_rank={self.est_rank}, "
+                f"status={self.status.name})")
+
+
 def qpwave(f2, *, left, right, fudge=1e-4, rank_alpha=0.05):
-    """qpWave rank sweep, left[0] the reference row (bindings/steppe/__init__.py:632)."""
-    return run_qpwave(f2, left, right, fudge, rank_alpha)
+    """qpWave rank sweep, left[0] the reference row (bindings/steppe/__init__.py:632-643):
+    a QpWaveResult."""
+    return QpWaveResult(run_qpwave(f2, left, right, fudge, rank_alpha), left=left)
 
 
 def f4_sweep(f2, pops=None, *, min_z=3.0, top_k=0, sure=False, all_combinations=False, method="auto"):

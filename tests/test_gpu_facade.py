@@ -34,6 +34,9 @@ def test_qpadm_facade():
     assert len(res) == 2 and res[0].weight == d["weight"]
     df = st.qpadm_search(h, target="England_BellBeaker", right=RIGHT, models=[["Turkey_N"]], as_dataframe=True)
     assert len(df) == 1
+    qw = st.qpwave(h, left=["England_BellBeaker", "Czechia_EBA_CordedWare", "Turkey_N"], right=RIGHT)
+    dw = _core.run_qpwave(h, ["England_BellBeaker", "Czechia_EBA_CordedWare", "Turkey_N"], RIGHT)
+    assert qw.f4rank == dw["f4rank"] and list(qw.per_rank["chisq"]) == dw["rank_chisq"]
 
 
 def test_qpgraph_facade():
