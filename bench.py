@@ -65,8 +65,8 @@ sys.path.insert(0, ROOT)
 SEED = 20261202
 MI355X_PEAK = {"hbm_GBs": 8000.0, "fp64_TFLOPs": 78.6, "bf16_TFLOPs": 2500.0}
 SPLIT_K = 33 * 2 / 8.0   # executed BF16 FLOP per algorithmic FLOP in the split form
-PROFILES = {("c3", "split_bf16"): ("profiles/f2_split_c3.md", "f2_split_kernel"),
-            ("c3", "fp64"): ("profiles/f2_fp64_c3.md", "f2_fp64_kernel")}
+PROFILES = {("c3", "split_bf16"): ("profiles/f2_split_c3.md", "f2_split_kernel", "f2_split.hip"),
+            ("c3", "fp64"): ("profiles/f2_fp64_c3.md", "f2_fp64_kernel", "f2_fp64.hip")}
 METRIC = "f2 pop-pair·SNPs/sec at ≤1e-9 rel err (split-BF16 vs FP64); qpAdm fits/sec"
 
 
@@ -75,50 +75,44 @@ def log(*a):
 
 
 def workload_params(name: str):
-    rng = np.random.default_rng(SEED)
-    M = 1_200_000
-    if name == "c2":
-        P = 200
-    elif name == "c3":
-        P = 1000
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    sizes = rng.integers(1, 61, P)
-    haploid = rng.random(P) < 0.4
-    missing = np.where(haploid, rng.beta(2.0, 2.0, P), 0.05).astype(np.float32)
-    fst = rng.uniform(0.02, 0.15, P).astype(np.float32)
     from steppe_amd import synth
-    if name == "c2":
-        bs = synth.uniform_block_sizes(M, 200, 3000, rng)
-    else:
-        bs = synth.lognormal_block_sizes(M, 1500.0, 0.6, 50, 8000, rng)
-    return dict(P=P, M=M, sizes=sizes, haploid=haploid, missing=missing, fst=fst, block_sizes=bs)
+    return synth.baseline_workload(name, SEED)
 
 
 def build_panel(ctx, wp):
     """Generate the packed panel directly in HBM (returns the torch buffer)."""
-    import torch
-    from steppe_amd import _core
-    P, M = wp["P"], wp["M"]
-    sizes = wp["sizes"].astype(np.int64)
-    n_ind = int(sizes.sum())
-    bpr = ((M + 3) // 4 + 255) // 256 * 256
-    ind_pop = np.repeat(np.arange(P, dtype=np.int32), sizes)
-    ind_hap = np.repeat(wp["haploid"].astype(np.int8), sizes)
-    packed = torch.empty(n_ind * bpr, dtype=torch.uint8, device="cuda")
-    torch.cuda.synchronize()
-    _core.check(_core.lib().stpx_synth_panel(
-        ctx.h, SEED, P, M, n_ind, bpr, _core._ptr(np.ascontiguousarray(wp["fst"])),
-        _core._ptr(np.ascontiguousarray(wp["missing"])), _core._ptr(ind_pop), _core._ptr(ind_hap),
-        C.c_void_p(packed.data_ptr())), "synth")
-    offs = np.zeros(P + 1, np.uint64)
-    offs[1:] = np.cumsum(sizes)
-    return packed, n_ind, bpr, offs
+    from steppe_amd import synth
+    dp = synth.device_panel(ctx, wp)
+    return dp.packed, dp.n_ind, dp.bytes_per_record, dp.pop_offsets
 
 
-def cpu_baseline(packed, n_ind, bpr, offs, wp, target_s: float = 15.0):
+def host_cpu_info() -> dict:
+    """The host CPU model and the oracle build's compiler and flags."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info = {"cpu_model": model, "compiler": None, "flags": None}
+    for path in (os.path.join(ROOT, "oracle", "_ref", "build_info.txt"),):
+        try:
+            for line in open(path):
+                k, _, v = line.partition(":")
+                if k.strip() in ("compiler", "flags"):
+                    info[k.strip()] = v.strip()
+        except OSError:
+            pass
+    return info
+
+
+def cpu_baseline(packed, n_ind, bpr, offs, wp, dev_f2=None, dev_vp=None, target_s: float = 15.0):
     """Single-thread reference CpuBackend (or the restated oracle) on a sample:
-    all populations x the leading blocks totalling ~target_s seconds of work."""
+    all populations x the leading blocks totalling ~target_s seconds of work.  When the
+    device tensors are given ([nb, P, P] views of f2 and vpair), the reference's blocks
+    are compared with them: max relative f2 error (floor 1e-6) and vpair equality."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     from steppe_amd import synth
@@ -150,40 +144,55 @@ def cpu_baseline(packed, n_ind, bpr, offs, wp, target_s: float = 15.0):
     k = max(1, min(k, len(bs)))
     m, q, v, n, bid = sample(k)
     t0 = time.perf_counter()
-    lib.f2_blocks(q, v, n, bid, k)
+    fr, vr, _ = lib.f2_blocks(q, v, n, bid, k)
     dt = time.perf_counter() - t0
     value = P * (P + 1) / 2 * m / dt
-    return {"value": value, "unit": "pop-pair·SNPs/s", "cores": 1, "kind": kind,
-            "sample": f"all {P} populations x first {k} blocks ({m} SNPs), f2 blocks from decoded "
-                      f"Q/V/N, {dt:.1f} s single-threaded ({os.path.basename(lib.path)})"}
+    out = {"value": value, "unit": "pop-pair·SNPs/s", "cores": 1, "kind": kind,
+           "sample": f"all {P} populations x first {k} blocks ({m} SNPs), f2 blocks from decoded "
+                     f"Q/V/N, {dt:.1f} s single-threaded ({os.path.basename(lib.path)})"}
+    out.update(host_cpu_info())
+    if dev_f2 is not None:
+        import torch
+        worst, vp_equal = 0.0, True
+        for b in range(k):
+            want = torch.from_numpy(np.ascontiguousarray(fr[:, :, b].T))     # [j, i]
+            got = dev_f2[b].cpu()
+            worst = max(worst, ((got - want).abs() / want.abs().clamp_min(1e-6)).max().item())
+            vp_equal &= bool(torch.equal(dev_vp[b].cpu(), torch.from_numpy(np.ascontiguousarray(vr[:, :, b].T))))
+        out["compared_blocks"] = k
+        out["max_rel_err_device_vs_reference"] = worst
+        out["vpair_equal"] = vp_equal
+    return out
+
+
+def source_sha(name: str) -> str:
+    import hashlib
+    with open(os.path.join(ROOT, "steppe_amd", "csrc", name), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
 def profiled_traffic(workload: str, precision: str):
-    """HBM bytes per launch of the f2 kernel from the committed counter summary
-    (the 'traffic GB' column of its row), or None."""
+    """HBM bytes per launch of the f2 kernel from the committed counter summary (the
+    'traffic GB' column of its row), used only when the summary names the same kernel
+    source hash as the file in the tree (a summary of an older kernel gives None)."""
     key = PROFILES.get((workload, precision))
     if not key:
-        return None
-    path, kernel = key
+        return None, None
+    path, kernel, src = key
+    want = source_sha(src)
+    traffic, sha = None, None
     try:
         for line in open(os.path.join(ROOT, path)):
+            if line.startswith("Kernel source:"):
+                sha = line.split("sha256")[-1].strip().strip("`. ")
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             if len(cells) >= 6 and cells[0] == f"`{kernel}`":
-                return float(cells[5]) * 1e9
+                traffic = float(cells[5]) * 1e9
     except OSError:
-        return None
-    return None
-
-
-def random_models(rng, P: int, count: int):
-    """1 target + 2-5 sources, 6-12 right populations, all distinct."""
-    models = []
-    for _ in range(count):
-        ns = int(rng.integers(2, 6))
-        nrt = int(rng.integers(6, 13))
-        pick = rng.choice(P, 1 + ns + nrt, replace=False)
-        models.append((int(pick[0]), [int(x) for x in pick[1:1 + ns]], [int(x) for x in pick[1 + ns:]]))
-    return models
+        return None, None
+    if sha != want:
+        return None, f"{path} (kernel source {sha}, tree {want}: stale, not used)"
+    return traffic, f"{path} (kernel source sha256 {want})"
 
 
 _C2 = {}
@@ -218,7 +227,8 @@ def qpadm_bench(ctx, n_models: int, cpu: bool):
     from steppe_amd import _core
     h = c2_tensor(ctx)
     P, nb = h.P, h.n_block
-    models = random_models(np.random.default_rng(SEED + 4), P, n_models)
+    from steppe_amd import synth
+    models = synth.random_qpadm_models(np.random.default_rng(SEED + 4), P, n_models)
     _core.qpadm_models(h, models[:64], raw=True)          # warm-up (code load, buffers)
     ctx.timing(True)
     ctx.read_timing(reset=True)
@@ -491,20 +501,6 @@ def tier_bench(ctx, tile, block_id, nb: int, P: int, prec: int):
     return out
 
 
-def random_graphs(rng, count: int):
-    """count random admixture graphs, 8-12 leaves and 1-4 admixture events each, with
-    their simulated f2 over 300 blocks."""
-    from steppe_amd import synth
-    out = []
-    for _ in range(count):
-        L = int(rng.integers(8, 13))
-        A = int(rng.integers(1, 5))
-        edges, theta, drift = synth.random_admixture_graph(rng, L, A)
-        f2, bs, names = synth.simulate_graph_f2(edges, theta, drift, n_block=300, rng=rng)
-        out.append((edges, f2, bs, names))
-    return out
-
-
 def qpgraph_bench(ctx, n_graphs: int, cpu: bool, numstart: int = 10):
     """C5: n_graphs random graphs, each fitted from ``numstart`` restarts by the device
     L-BFGS on the score + adjoint (one workgroup per restart, to convergence); the f3
@@ -512,7 +508,8 @@ def qpgraph_bench(ctx, n_graphs: int, cpu: bool, numstart: int = 10):
     region."""
     from steppe_amd import _core, qpgraph
     t0 = time.perf_counter()
-    graphs = random_graphs(np.random.default_rng(SEED + 5), n_graphs)
+    from steppe_amd import synth
+    graphs = synth.random_graph_cases(np.random.default_rng(SEED + 5), n_graphs)
     handles = [_core.upload_f2(f2, None, bs, names) for _, f2, bs, names in graphs]
     log(f"[bench] qpgraph: {n_graphs} graphs simulated and resident in {time.perf_counter() - t0:.1f} s")
     # warm-up: code load and staging buffers
@@ -736,9 +733,10 @@ def main():
     peak = MI355X_PEAK["fp64_TFLOPs"] if fp64 else MI355X_PEAK["bf16_TFLOPs"]
     kf = 1.0 if fp64 else SPLIT_K
     achieved = kf * f_alg / f2_s / 1e12 if f2_s > 0 else 0.0
+    traffic, traffic_src = profiled_traffic(args.workload, args.precision)
     roofline = {"kernel": "f2_fp64_kernel" if fp64 else "f2_split_kernel",
                 "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": profiled_traffic(args.workload, args.precision),
+                "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                 "k": kf, "algorithmic_TFLOPs": f_alg / f2_s / 1e12 if f2_s > 0 else 0.0,
                 "work": ("8*W FLOP per launch, W = P(P+1)/2*M" if fp64 else
                          "k*8*W FLOP per launch (k = 8.25 BF16 FLOP per algorithmic FLOP), W = P(P+1)/2*M")}
@@ -747,13 +745,47 @@ def main():
     stages["decode"]["hbm_GBs"] = dec_bytes / (avg["decode"] / 1e3) / 1e9 if avg["decode"] else None
     stages["decode"]["algorithmic_bytes"] = dec_bytes
 
+    # The split form against the plain FP64 form on the same counts: the whole tensor's
+    # max relative error (floor 1e-6), vpair equality, and the FP64 step time.
+    accuracy = None
+    if not fp64:
+        f2d = torch.empty_like(f2)
+        vpd = torch.empty_like(vp)
+        torch.cuda.synchronize()
+
+        def step64():
+            _core.check(L.stp_decode_counts(ctx.h, C.byref(tile), C.c_void_p(a2.data_ptr()),
+                                            C.c_void_p(nn.data_ptr()), ld), "decode")
+            _core.check(L.stp_f2_blocks_from_counts(ctx.h, C.c_void_p(a2.data_ptr()), C.c_void_p(nn.data_ptr()),
+                                                    ld, P, M, bid_p, nb, _core.STP_PREC_FP64,
+                                                    C.c_void_p(f2d.data_ptr()), C.c_void_p(vpd.data_ptr())), "f2")
+        step64()
+        n64 = 3
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n64):
+            step64()
+        torch.cuda.synchronize()
+        ms64 = (time.perf_counter() - t0) / n64 * 1e3
+        worst = 0.0
+        fs_, fd_ = f2.view(nb, P, P), f2d.view(nb, P, P)
+        for b0 in range(0, nb, 64):
+            x, y = fs_[b0:b0 + 64], fd_[b0:b0 + 64]
+            worst = max(worst, ((x - y).abs() / y.abs().clamp_min(1e-6)).max().item())
+        accuracy = {"max_rel_err_split_vs_fp64": worst, "floor": 1e-6, "elements": P * P * nb,
+                    "vpair_equal": bool(torch.equal(vp, vpd)), "fp64_ms_per_step": ms64,
+                    "split_ms_per_step": ms_step, "fp64_over_split_step": ms64 / ms_step}
+        log(f"[bench] split vs FP64: max rel err {worst:.3e}; FP64 step {ms64:.1f} ms ({ms64 / ms_step:.2f}x split)")
+        del f2d, vpd
+        torch.cuda.empty_cache()
+
     tiers = None
     if not args.no_tiers:
         tiers = tier_bench(ctx, tile, block_id, nb, P, prec)
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(packed, n_ind, bpr, offs, wp)
+        cpu = cpu_baseline(packed, n_ind, bpr, offs, wp, f2.view(nb, P, P), vp.view(nb, P, P))
         log(f"[bench] cpu baseline {cpu}")
     qp = None
     if not args.no_qpadm:
@@ -776,6 +808,9 @@ def main():
     if not args.no_qpgraph_search:
         qs = qpgraph_search_bench(ctx, not args.no_cpu_baseline)
 
+    if cpu and accuracy is not None and "max_rel_err_device_vs_reference" in cpu:
+        accuracy["max_rel_err_vs_reference"] = cpu["max_rel_err_device_vs_reference"]
+        accuracy["reference_blocks_compared"] = cpu["compared_blocks"]
     out = {
         "metric": METRIC, "value": value, "unit": "pop-pair·SNPs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -786,7 +821,7 @@ def main():
                                f"{nb} jackknife blocks, decode + f2 ({args.precision})",
                    "P": P, "M": M, "n_block": nb, "n_individuals": n_ind,
                    "precision": args.precision, "parallelism": "single GPU"},
-        "roofline": roofline, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg, "fstats": fs, "frontend": fe,
+        "roofline": roofline, "accuracy": accuracy, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg, "fstats": fs, "frontend": fe,
         "tiers": tiers, "qpgraph_search": qs,
     }
     print(json.dumps(out), flush=True)

@@ -320,3 +320,116 @@ def simulate_graph_f2(edges, theta, drift, n_block: int = 300, rng=None, noise: 
     z = z + z.transpose(0, 2, 1)
     out = np.transpose(f2[None, :, :] + z, (1, 2, 0))
     return np.asfortranarray(out), np.full(n_block, block_size, np.int32), names
+
+
+# ---------------------------------------------------------------------------------------
+# the large workloads, generated in device memory
+# ---------------------------------------------------------------------------------------
+WORKLOAD_M = 1_200_000
+
+
+def baseline_workload(name: str, seed: int = SEED) -> dict:
+    """Parameters of the two large panels (host side; the genotypes themselves are drawn
+    on the device by stpx_synth_panel):
+      c2: 200 populations, sizes U{1..60}, 1.2M SNPs, blocks U{200..3000} SNPs;
+      c3: 1000 populations, sizes U{1..60}, 1.2M SNPs, blocks LogNormal(ln 1500, 0.6)
+          clipped to [50, 8000].
+    Both: 40% pseudo-haploid populations with missingness ~ Beta(2,2) (diploid ones 5%),
+    F_ST ~ U(0.02, 0.15) per population."""
+    rng = np.random.default_rng(seed)
+    if name == "c2":
+        P = 200
+    elif name == "c3":
+        P = 1000
+    else:
+        raise ValueError(f"unknown workload {name!r} (c2 or c3)")
+    M = WORKLOAD_M
+    sizes = rng.integers(1, 61, P)
+    haploid = rng.random(P) < 0.4
+    missing = np.where(haploid, rng.beta(2.0, 2.0, P), 0.05).astype(np.float32)
+    fst = rng.uniform(0.02, 0.15, P).astype(np.float32)
+    if name == "c2":
+        bs = uniform_block_sizes(M, 200, 3000, rng)
+    else:
+        bs = lognormal_block_sizes(M, 1500.0, 0.6, 50, 8000, rng)
+    return dict(name=name, seed=seed, P=P, M=M, sizes=sizes, haploid=haploid, missing=missing, fst=fst,
+                block_sizes=bs)
+
+
+@dataclass
+class DevicePanel:
+    """A packed panel in device memory (a torch uint8 buffer of n_ind x bytes_per_record)."""
+    packed: object
+    n_ind: int
+    bytes_per_record: int
+    pop_offsets: np.ndarray
+    block_id: np.ndarray
+    n_block: int
+
+
+def device_panel(ctx, wp: dict) -> DevicePanel:
+    """Draws the workload's genotypes directly in HBM with stpx_synth_panel."""
+    import ctypes as C
+
+    import torch
+
+    from steppe_amd import _core
+    P, M = wp["P"], wp["M"]
+    sizes = np.asarray(wp["sizes"], np.int64)
+    n_ind = int(sizes.sum())
+    bpr = ((M + 3) // 4 + 255) // 256 * 256
+    ind_pop = np.repeat(np.arange(P, dtype=np.int32), sizes)
+    ind_hap = np.repeat(np.asarray(wp["haploid"]).astype(np.int8), sizes)
+    packed = torch.empty(n_ind * bpr, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    _core.check(_core.lib().stpx_synth_panel(
+        ctx.h, wp["seed"], P, M, n_ind, bpr, _core._ptr(np.ascontiguousarray(wp["fst"])),
+        _core._ptr(np.ascontiguousarray(wp["missing"])), _core._ptr(ind_pop), _core._ptr(ind_hap),
+        C.c_void_p(packed.data_ptr())), "synth")
+    offs = np.zeros(P + 1, np.uint64)
+    offs[1:] = np.cumsum(sizes)
+    bs = wp["block_sizes"]
+    return DevicePanel(packed=packed, n_ind=n_ind, bytes_per_record=bpr, pop_offsets=offs,
+                       block_id=np.ascontiguousarray(blocks_from_sizes(bs)), n_block=len(bs))
+
+
+def host_subpanel(dp: DevicePanel, snp0: int, snp1: int, sample_ploidy) -> tuple:
+    """The SNP columns [snp0, snp1) of a device panel as a host Panel (for the oracle).
+    The records are cut at byte boundaries, so the host panel starts at SNP
+    4 * (snp0 // 4); returns (panel, lead) with lead = the SNPs before snp0 in it.
+    Its block_id puts the lead SNPs in block 0 and [snp0, snp1) in block 1."""
+    b0 = snp0 // 4
+    b1 = (snp1 + 3) // 4
+    lead = snp0 - 4 * b0
+    m = snp1 - 4 * b0
+    host = dp.packed.view(dp.n_ind, dp.bytes_per_record)[:, b0:b1].cpu().numpy()
+    bid = np.zeros(m, np.int32)
+    bid[lead:] = 1
+    panel = Panel(packed=np.ascontiguousarray(host), n_snp=m, pop_offsets=dp.pop_offsets,
+                  sample_ploidy=np.asarray(sample_ploidy, np.int32), block_id=bid, n_block=2)
+    return panel, lead
+
+
+def random_qpadm_models(rng, P: int, count: int) -> list:
+    """count models of 1 target + 2-5 sources and 6-12 right populations, all distinct,
+    drawn uniformly from P populations: (target, sources, right)."""
+    models = []
+    for _ in range(count):
+        ns = int(rng.integers(2, 6))
+        nrt = int(rng.integers(6, 13))
+        pick = rng.choice(P, 1 + ns + nrt, replace=False)
+        models.append((int(pick[0]), [int(x) for x in pick[1:1 + ns]], [int(x) for x in pick[1 + ns:]]))
+    return models
+
+
+def random_graph_cases(rng, count: int, n_block: int = 300) -> list:
+    """count random admixture graphs of 8-12 leaves and 1-4 admixture events, each with
+    its own f2 tensor simulated on it over n_block blocks: (edges, f2, block_sizes, names)."""
+    out = []
+    for _ in range(count):
+        L = int(rng.integers(8, 13))
+        A = int(rng.integers(1, 5))
+        edges, theta, drift = random_admixture_graph(rng, L, A)
+        f2, bs, names = simulate_graph_f2(edges, theta, drift, n_block=n_block, rng=rng)
+        out.append((edges, f2, bs, names))
+    return out
