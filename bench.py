@@ -8,8 +8,9 @@
 
 One step = one pass of the hot path over the synthetic panel already resident in
 HBM: decode the packed 2-bit genotypes to per-population counts, then build the
-per-jackknife-block f2 and vpair tensors [n_block x P x P] in HBM.  Prints one
-JSON line.  The metric is f2 pop-pair.SNPs/s: W / t with W = P(P+1)/2 * M
+per-jackknife-block f2 and vpair tensors [n_block x P x P] in HBM.  The timed steps
+replay one captured HIP graph of the whole step (stp_f2_graph_*; --eager times the
+separate calls instead).  Prints one JSON line.  The metric is f2 pop-pair.SNPs/s: W / t with W = P(P+1)/2 * M
 (unordered population pairs incl. the diagonal, times SNPs).
 
 Workloads (synthetic, seeded; population frequencies Balding-Nichols, genotypes
@@ -663,6 +664,7 @@ def main():
     ap.add_argument("--no-tiers", action="store_true")
     ap.add_argument("--no-qpgraph-search", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="time eager calls instead of the captured graph")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -710,17 +712,42 @@ def main():
     for i in range(args.warmup):
         step()
         log(f"[bench] warmup {i+1}/{args.warmup} done")
+    # Eager steps (one stp_decode_counts + one stp_f2_blocks_from_counts call each), timed
+    # for comparison with the captured graph.
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) / 3 * 1e3
     ctx.timing(True)
+    graph = None
+    run = step
+    if not args.eager:
+        # The whole step (ploidy detection, decode, prep, pair counts, contraction, fix-up)
+        # as one captured HIP graph, with the stage events captured inside it.
+        graph = _core.F2Graph(ctx, tile, block_id, nb, prec, f2.data_ptr(), vp.data_ptr(), a2.data_ptr(),
+                              nn.data_ptr(), ld)
+        graph.launch()
+        run = graph.launch
     ctx.read_timing(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        run()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     tm = ctx.read_timing(reset=True)
+    stage_source = "graph" if graph is not None else "eager"
+    if tm["f2"][1] == 0:
+        # the runtime did not capture the stage events: time the kernels on eager steps
+        for _ in range(args.steps):
+            step()
+        tm = ctx.read_timing(reset=True)
+        stage_source = "eager steps after the timed graph replays"
     ctx.timing(False)
-    log(f"[bench] {args.steps} steps in {dt:.3f} s; stages {tm}")
+    log(f"[bench] {args.steps} {'graph replays' if graph else 'eager steps'} in {dt:.3f} s "
+        f"(eager step {eager_ms:.2f} ms); stages ({stage_source}) {tm}")
 
     W = P * (P + 1) / 2 * M
     value = W * args.steps / dt
@@ -753,12 +780,17 @@ def main():
         vpd = torch.empty_like(vp)
         torch.cuda.synchronize()
 
-        def step64():
-            _core.check(L.stp_decode_counts(ctx.h, C.byref(tile), C.c_void_p(a2.data_ptr()),
-                                            C.c_void_p(nn.data_ptr()), ld), "decode")
-            _core.check(L.stp_f2_blocks_from_counts(ctx.h, C.c_void_p(a2.data_ptr()), C.c_void_p(nn.data_ptr()),
-                                                    ld, P, M, bid_p, nb, _core.STP_PREC_FP64,
-                                                    C.c_void_p(f2d.data_ptr()), C.c_void_p(vpd.data_ptr())), "f2")
+        if graph is not None:
+            g64 = _core.F2Graph(ctx, tile, block_id, nb, _core.STP_PREC_FP64, f2d.data_ptr(), vpd.data_ptr(),
+                                a2.data_ptr(), nn.data_ptr(), ld)
+            step64 = g64.launch
+        else:
+            def step64():
+                _core.check(L.stp_decode_counts(ctx.h, C.byref(tile), C.c_void_p(a2.data_ptr()),
+                                                C.c_void_p(nn.data_ptr()), ld), "decode")
+                _core.check(L.stp_f2_blocks_from_counts(ctx.h, C.c_void_p(a2.data_ptr()), C.c_void_p(nn.data_ptr()),
+                                                        ld, P, M, bid_p, nb, _core.STP_PREC_FP64,
+                                                        C.c_void_p(f2d.data_ptr()), C.c_void_p(vpd.data_ptr())), "f2")
         step64()
         n64 = 3
         torch.cuda.synchronize()
@@ -776,6 +808,8 @@ def main():
                     "vpair_equal": bool(torch.equal(vp, vpd)), "fp64_ms_per_step": ms64,
                     "split_ms_per_step": ms_step, "fp64_over_split_step": ms64 / ms_step}
         log(f"[bench] split vs FP64: max rel err {worst:.3e}; FP64 step {ms64:.1f} ms ({ms64 / ms_step:.2f}x split)")
+        if graph is not None:
+            g64.close()
         del f2d, vpd
         torch.cuda.empty_cache()
 
@@ -787,6 +821,8 @@ def main():
         log("[bench] cpu baseline ...")
         cpu = cpu_baseline(packed, n_ind, bpr, offs, wp, f2.view(nb, P, P), vp.view(nb, P, P))
         log(f"[bench] cpu baseline {cpu}")
+    if graph is not None:
+        graph.close()
     qp = None
     if not args.no_qpadm:
         del packed, a2, nn, f2, vp
@@ -820,7 +856,8 @@ def main():
         "config": {"workload": f"{args.workload}: {P} populations x {M} SNPs, {n_ind} individuals, "
                                f"{nb} jackknife blocks, decode + f2 ({args.precision})",
                    "P": P, "M": M, "n_block": nb, "n_individuals": n_ind,
-                   "precision": args.precision, "parallelism": "single GPU"},
+                   "precision": args.precision, "parallelism": "single GPU",
+                   "graph": graph is not None, "eager_ms_per_step": eager_ms, "stage_timing": stage_source},
         "roofline": roofline, "accuracy": accuracy, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg, "fstats": fs, "frontend": fe,
         "tiers": tiers, "qpgraph_search": qs,
     }

@@ -125,6 +125,24 @@ int stp_extract_f2(stp_context* ctx, const stp_tile* tile, const int32_t* block_
  * each chunk out while the next is computed.  Every tier holds the same values.  Fits
  * (qpAdm, qpGraph, f-statistics) need a resident handle and return STP_ERR_INVALID for
  * the others; stp_f2_handle_download reads every tier. */
+/* A resident extraction of one fixed shape captured once as a HIP graph and replayed
+ * (replaces the host-driven chunk loop of the reference's f2 engine, which synchronises
+ * after every size-bucket chunk: src/device/cuda/cuda_backend_f2_blocks.cu:301-347).
+ * The tile must be device-resident (packed_on_device); its records, d_f2 and d_vpair
+ * [P x P x n_block] must stay allocated while the graph exists.  d_a2 / d_n (uint16
+ * [P x ld], ld >= n_snp, ld % 16 == 0) receive the counts, or are both NULL for counts
+ * held by the graph.  Create runs the extraction once eagerly, then captures ploidy
+ * detection (when the tile asks for it), decode, operand prep, pair counts, contraction
+ * and the fix-up of cancellation-prone entries; each launch replays them with one graph
+ * launch and returns when the tensors are complete.  Every launch rewrites the same
+ * values as stp_decode_counts + stp_f2_blocks_from_counts on the same inputs. */
+typedef struct stp_f2_graph stp_f2_graph;
+int stp_f2_graph_create(stp_context* ctx, const stp_tile* tile, const int32_t* block_id, int32_t n_block,
+                        int32_t precision, uint16_t* d_a2, uint16_t* d_n, int64_t ld, double* d_f2,
+                        double* d_vpair, stp_f2_graph** out);
+int stp_f2_graph_launch(stp_f2_graph* g);
+int stp_f2_graph_free(stp_f2_graph* g);
+
 #define STP_TIER_AUTO (-1)
 #define STP_TIER_RESIDENT 0
 #define STP_TIER_HOST 1
@@ -237,11 +255,15 @@ typedef struct stp_qpadm_options {
  * left[left_offsets[i] .. left_offsets[i+1]) and right set
  * right[right_offsets[i] .. right_offsets[i+1]) (right[...][0] is the base outgroup),
  * all population indices of the handle.  Results go to out[i] (host array of
- * n_models records), in input order, with out[i].model_index = i.  Limits: at most
- * STP_QPADM_MAX_NL sources, at least as many right populations beyond the base as
- * sources, (sources x (right - 1)) <= 64 and right - 1 <= 32; violations return
- * STP_ERR_INVALID.  Per-model domain outcomes (rank deficient, non-SPD covariance,
- * undefined chi-square) are reported in out[i].status, as in the reference. */
+ * n_models records), in input order, with out[i].model_index = i.  Models with
+ * (sources x (right - 1)) <= 64 and right - 1 <= 32 run on the batched kernel, larger
+ * ones (the reference's large path, model_search.cpp:85-106) on the large-model kernels
+ * with the same arithmetic.  Limits: at most STP_QPADM_MAX_NL sources, sources <= right
+ * populations beyond the base + 1 (the reference reads past its SVD's right vectors for
+ * more: seed_AB, src/device/cpu/cpu_backend.cpp:2738-2744), sources x (right - 1) <= 1024
+ * and right - 1 <= 128; violations return STP_ERR_INVALID.  Per-model domain outcomes
+ * (rank deficient, non-SPD covariance, undefined chi-square) are reported in
+ * out[i].status, as in the reference. */
 int stp_qpadm_batch(stp_context* ctx, const stp_f2_handle* f2, int32_t n_models, const int32_t* target,
                     const int32_t* left_offsets, const int32_t* left, const int32_t* right_offsets,
                     const int32_t* right, const stp_qpadm_options* options, stp_qpadm_result* out);
@@ -495,11 +517,12 @@ int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_s
  * around the decode, prep and f2 contraction launches. */
 int stpx_timing(stp_context* ctx, int32_t enable);
 /* Accumulated milliseconds and launch counts per stage [decode, prep, f2, count, qpadm,
- * qpgraph, fstat, frontend] (f2 = the contraction kernel, count = the pair-count kernel,
- * qpadm = the batched fit kernels of one stp_qpadm_batch call, qpgraph = one
- * stp_qpgraph_eval or stp_qpgraph_fit launch, fstat = the kernels of one f4 / f3 /
- * f4-ratio / sweep call, frontend = each GENO transpose block, the keep mask and the
- * compaction of stp_extract_f2_files); reset clears them. */
+ * qpgraph, fstat, frontend, fixup] (arrays of 9; f2 = the contraction kernel, count = the
+ * pair-count kernel, qpadm = the batched fit kernels of one stp_qpadm_batch call,
+ * qpgraph = one stp_qpgraph_eval or stp_qpgraph_fit launch, fstat = the kernels of one
+ * f4 / f3 / f4-ratio / sweep call, frontend = each GENO transpose block, the keep mask and
+ * the compaction of stp_extract_f2_files, fixup = the FP64 recomputation of the f2
+ * entries the contraction flagged as cancellation-prone); reset clears them. */
 int stpx_timing_read(stp_context* ctx, double* ms, int64_t* count, int32_t reset);
 
 /* ---- synthetic panels (bench/test input; no counterpart in the reference) ------------ */

@@ -157,6 +157,9 @@ def lib():
         "stp_compute_f2_blocks": (C.c_int, [vp, vp, vp, vp, i32, i64, vp, i32, i32, vp, vp, vp]),
         "stp_extract_f2": (C.c_int, [vp, C.POINTER(Tile), vp, i32, i32, C.POINTER(vp)]),
         "stp_extract_f2_tiered": (C.c_int, [vp, C.POINTER(Tile), vp, i32, i32, i32, C.c_char_p, i64, C.POINTER(vp)]),
+        "stp_f2_graph_create": (C.c_int, [vp, C.POINTER(Tile), vp, i32, i32, vp, vp, i64, vp, vp, C.POINTER(vp)]),
+        "stp_f2_graph_launch": (C.c_int, [vp]),
+        "stp_f2_graph_free": (C.c_int, [vp]),
         "stp_f2_handle_tier": (C.c_int, [vp, C.POINTER(i32)]),
         "stp_f2_handle_upload": (C.c_int, [vp, vp, vp, vp, i32, i32, C.POINTER(vp)]),
         "stp_f2_handle_shape": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32)]),
@@ -224,7 +227,7 @@ class Context:
             cls._by_device[int(device)] = ctx
         return ctx
 
-    STAGES = ("decode", "prep", "f2", "count", "qpadm", "qpgraph", "fstat", "frontend")
+    STAGES = ("decode", "prep", "f2", "count", "qpadm", "qpgraph", "fstat", "frontend", "fixup")
 
     def timing(self, enable: bool = True):
         check(lib().stpx_timing(self.h, 1 if enable else 0))
@@ -284,6 +287,41 @@ def tile_from_panel(panel, ploidy="sample"):
     if ploidy == "auto":
         return make_tile(panel.packed, panel.n_snp, panel.pop_offsets, None, detect=True)
     return make_tile(panel.packed, panel.n_snp, panel.pop_offsets, None, ploidy=int(ploidy))
+
+
+class F2Graph:
+    """A resident extraction of one fixed shape captured as a HIP graph
+    (stp_f2_graph_create): ``launch()`` replays ploidy detection, decode, prep, pair
+    counts, contraction and fix-up into the device buffers given at creation.
+
+    ``tile`` must be device-resident; ``block_id`` is the SNP -> block map; ``f2`` and
+    ``vpair`` are device pointers to [P x P x n_block] FP64 buffers; ``a2`` / ``n`` are
+    device pointers to uint16 [P x ld] counts, or None for counts held by the graph."""
+
+    def __init__(self, ctx: "Context", tile, block_id, n_block: int, precision, f2: int, vpair: int,
+                 a2: int | None = None, n: int | None = None, ld: int = 0):
+        self.ctx = ctx
+        self._bid = np.ascontiguousarray(block_id, dtype=np.int32)
+        self._tile = tile
+        h = C.c_void_p()
+        check(lib().stp_f2_graph_create(ctx.h, C.byref(tile), _ptr(self._bid), int(n_block), parse_precision(precision),
+                                        C.c_void_p(a2) if a2 else None, C.c_void_p(n) if n else None, int(ld),
+                                        C.c_void_p(int(f2)), C.c_void_p(int(vpair)), C.byref(h)), "f2 graph create")
+        self.h = h
+
+    def launch(self):
+        check(lib().stp_f2_graph_launch(self.h), "f2 graph launch")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().stp_f2_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class F2Handle:
@@ -448,6 +486,9 @@ def read_f2(dir: str, device: int = 0) -> F2Handle:
 
 
 def parse_precision(precision) -> int:
+    if isinstance(precision, (int, np.integer)) and not isinstance(precision, bool) and \
+            int(precision) in (STP_PREC_FP64, STP_PREC_SPLIT_BF16):
+        return int(precision)
     if precision not in _PRECISIONS:
         raise ValueError("precision must be one of 'split_bf16'/'emu', 'fp64'/'native' "
                          f"(got '{precision}')")

@@ -22,12 +22,12 @@ OBJDIR = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(LIBDIR, "libsteppe_hip.so")
 
 ARCH = os.environ.get("STEPPE_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["decode.hip", "f2_fp64.hip", "f2_split.hip", "synth.hip", "qpadm.hip", "qpgraph.hip", "fstats.hip",
-           "frontend.hip", "engine.cpp", "qpadm_host.cpp", "qpgraph_host.cpp", "qpgraph_enum.cpp",
+SOURCES = ["decode.hip", "f2_fp64.hip", "f2_split.hip", "f2_fixup.hip", "synth.hip", "qpadm.hip", "qpadm_large.hip", "qpgraph.hip", "fstats.hip",
+           "frontend.hip", "engine.cpp", "f2_graph.cpp", "qpadm_host.cpp", "qpgraph_host.cpp", "qpgraph_enum.cpp",
            "qpgraph_search.cpp", "fstats_host.cpp", "geno_io.cpp", "frontend_host.cpp", "tiered.cpp", "capi.cpp"]
 # Built without FP contraction: each product is rounded before it is added, as in the
 # reference's fit chain (plain double sums in a fixed order).
-NO_CONTRACT = {"qpadm.hip", "qpgraph.hip", "fstats.hip", "frontend.hip"}
+NO_CONTRACT = {"qpadm.hip", "qpadm_large.hip", "qpgraph.hip", "fstats.hip", "frontend.hip", "f2_fixup.hip"}
 
 
 def hipcc() -> str:
@@ -43,8 +43,19 @@ def common_flags() -> list:
             "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 
-def _compile(src: str) -> str:
+def _newest_header() -> float:
+    heads = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    inc = os.path.join(ROOT, "include")
+    heads += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    return max(os.path.getmtime(h) for h in heads)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+    if incremental and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t > os.path.getmtime(os.path.join(CSRC, src)) and t > _newest_header():
+            return obj
     cmd = [hipcc(), *common_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
     if src in NO_CONTRACT:
         cmd.insert(-4, "-ffp-contract=off")
@@ -56,10 +67,12 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every source (incremental=True: only those older than their object
+    file or any header) and link the library."""
     os.makedirs(OBJDIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda f: _compile(f, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -72,5 +85,5 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

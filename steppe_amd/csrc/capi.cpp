@@ -24,6 +24,11 @@ struct stp_f2_handle {
     std::unique_ptr<stp::F2Blocks> blocks;
 };
 
+struct stp_f2_graph {
+    stp_context* ctx = nullptr;
+    std::unique_ptr<stp::F2Graph> g;
+};
+
 struct stp_qpgraph_topology {
     stp::QgTopology t;
 };
@@ -193,6 +198,32 @@ int stp_extract_f2(stp_context* ctx, const stp_tile* tile, const int32_t* block_
         h->ctx = ctx;
         h->blocks.reset(backend(ctx).extract_f2(*tile, block_id, n_block, precision));
         *out = h.release();
+    });
+}
+
+int stp_f2_graph_create(stp_context* ctx, const stp_tile* tile, const int32_t* block_id, int32_t n_block,
+                        int32_t precision, uint16_t* d_a2, uint16_t* d_n, int64_t ld, double* d_f2, double* d_vpair,
+                        stp_f2_graph** out) {
+    return guard([&] {
+        STP_REQUIRE(tile && block_id && out, "stp_f2_graph_create: null argument");
+        auto g = std::make_unique<stp_f2_graph>();
+        g->ctx = ctx;
+        g->g.reset(backend(ctx).f2_graph_create(*tile, block_id, n_block, precision, d_a2, d_n, ld, d_f2, d_vpair));
+        *out = g.release();
+    });
+}
+
+int stp_f2_graph_launch(stp_f2_graph* g) {
+    return guard([&] {
+        STP_REQUIRE(g && g->g, "stp_f2_graph_launch: null graph");
+        backend(g->ctx).f2_graph_launch(*g->g);
+    });
+}
+
+int stp_f2_graph_free(stp_f2_graph* g) {
+    return guard([&] {
+        if (g) backend(g->ctx).activate();
+        delete g;
     });
 }
 

@@ -6,6 +6,8 @@
 //            one prep launch and one persistent contraction launch, no per-block or
 //            per-bucket host loop (the reference loops buckets x chunks with a sync
 //            per chunk, cuda_backend_f2_blocks.cu:319-341).
+#include <cstdlib>
+
 #include "engine.hpp"
 
 #include <algorithm>
@@ -16,10 +18,6 @@
 #include "steppe_hip.h"
 
 namespace stp {
-
-namespace {
-constexpr int kMaxPopForU16 = 16383;  // a2 <= 4 * size must fit uint16
-}
 
 // ---------------------------------------------------------------------------------------
 F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_block) {
@@ -174,7 +172,7 @@ void HipBackend::read_timing(double* ms, std::int64_t* count, bool reset) {
     }
 }
 
-BlockTable HipBackend::PlanDev::table() const {
+BlockTable F2Work::PlanDev::table() const {
     return BlockTable{s0.get(), len.get(), k0.get(), kpad.get()};
 }
 
@@ -185,7 +183,7 @@ static void h2d(DevBuf<T>& buf, const std::vector<T>& v, hipStream_t st) {
         STP_HIP(hipMemcpyAsync(buf.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
 }
 
-void HipBackend::upload_plan(const F2Plan& plan, PlanDev& pd) {
+void HipBackend::upload_plan(const F2Plan& plan, F2Work::PlanDev& pd) {
     h2d(pd.s0, plan.s0, stream_);
     h2d(pd.k0, plan.k0, stream_);
     h2d(pd.len, plan.len, stream_);
@@ -202,7 +200,7 @@ const F2Plan& HipBackend::plan_for(int P, std::int64_t M, const std::int32_t* bl
     plan_valid_ = false;
     plan_ = make_f2_plan(P, M, block_id, n_block);
     plan_bid_.assign(block_id, block_id + M);
-    upload_plan(plan_, plan_dev_);
+    upload_plan(plan_, work_.plan);
     sync();
     plan_valid_ = true;
     return plan_;
@@ -305,60 +303,99 @@ void HipBackend::decode_af(const stp_tile& t, double* q, double* v, double* n) {
 }
 
 // ---------------------------------------------------------------------------------------
-void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, double* d_f2, double* d_vpair) {
+void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, double* d_f2, double* d_vpair,
+                        F2Work& w) {
     STP_REQUIRE(precision == STP_PREC_FP64 || precision == STP_PREC_SPLIT_BF16,
                 "f2: unknown precision form " + std::to_string(precision));
     const std::int64_t rows16 = static_cast<std::int64_t>(plan.G) * kRowGroup;
     const std::size_t plane = static_cast<std::size_t>(rows16) * static_cast<std::size_t>(plan.K_total);
     const std::int64_t n_chunks = plan.K_total / kKPad;
-    const BlockTable bt = plan_dev_.table();
+    const BlockTable bt = w.plan.table();
     const int n_items = static_cast<int>(plan.items.size());
     // Persistent grids are whole multiples of the die count so every die queue has
     // its own workers.
     const int n_wg = static_cast<int>(
         round_up(std::max(1, std::min<int>(2 * n_cu_, static_cast<int>(ceil_div(n_items, 4)))), kXcds));
-    planeV8_.ensure(std::max<std::size_t>(plane, 16));
+    w.planeV8.ensure(std::max<std::size_t>(plane, 16));
+    w.counters.ensure(2 * kXcds);
     if (precision == STP_PREC_FP64) {
-        planeA_.ensure(std::max<std::size_t>(plane, 1));
-        planeV_.ensure(std::max<std::size_t>(plane, 1));
-        planeQ_.ensure(std::max<std::size_t>(plane, 1));
+        w.planeA.ensure(std::max<std::size_t>(plane, 1));
+        w.planeV.ensure(std::max<std::size_t>(plane, 1));
+        w.planeQ.ensure(std::max<std::size_t>(plane, 1));
         const std::int64_t KS = plan.K_total / 4;
         mark_begin(kPrep);
         if (src.a2)
-            launch_prep_fp64_counts(src.a2, src.n, src.ld, plan.P, plan.G, bt, plan_dev_.chunk_block.get(), n_chunks,
-                                    KS, planeA_.get(), planeV_.get(), planeQ_.get(), planeV8_.get(), stream_);
+            launch_prep_fp64_counts(src.a2, src.n, src.ld, plan.P, plan.G, bt, w.plan.chunk_block.get(), n_chunks,
+                                    KS, w.planeA.get(), w.planeV.get(), w.planeQ.get(), w.planeV8.get(), stream_);
         else
-            launch_prep_fp64_qvn(src.q, src.v, src.nq, plan.P, plan.G, bt, plan_dev_.chunk_block.get(), n_chunks, KS,
-                                 planeA_.get(), planeV_.get(), planeQ_.get(), planeV8_.get(), stream_);
+            launch_prep_fp64_qvn(src.q, src.v, src.nq, plan.P, plan.G, bt, w.plan.chunk_block.get(), n_chunks, KS,
+                                 w.planeA.get(), w.planeV.get(), w.planeQ.get(), w.planeV8.get(), stream_);
         mark_end(kPrep);
+        const FixupList fl = fixup_list(plan, kFixupTauFp64, w);
         mark_begin(kF2);
-        launch_f2_fp64(planeA_.get(), planeV_.get(), planeQ_.get(), planeV8_.get(), bt, plan_dev_.items.get(),
-                       plan.q64, counter_.get(), plan.P, plan.G, KS, d_f2, d_vpair, n_wg, stream_);
+        launch_f2_fp64(w.planeA.get(), w.planeV.get(), w.planeQ.get(), w.planeV8.get(), bt, w.plan.items.get(),
+                       plan.q64, w.counters.get(), plan.P, plan.G, KS, d_f2, d_vpair, fl, n_wg, stream_);
         mark_end(kF2);
+        run_fixup(plan, src, fl, d_f2, d_vpair, w);
         return;
     }
     // Split-BF16: 13 digit planes of 2 bytes per (row, SNP), rows padded to 32.
     const std::int64_t KS16 = plan.K_total / 16;
     const std::size_t split_bytes = static_cast<std::size_t>(plan.G32) * 32 * plan.K_total * 26;
-    planeSplit_.ensure(std::max<std::size_t>(split_bytes, 16));
+    w.planeSplit.ensure(std::max<std::size_t>(split_bytes, 16));
     mark_begin(kPrep);
     if (src.a2)
-        launch_prep_split_counts(src.a2, src.n, src.ld, plan.P, plan.G32, plan.G, bt, plan_dev_.chunk_block.get(),
-                                 n_chunks, KS16, planeSplit_.get(), planeV8_.get(), stream_);
+        launch_prep_split_counts(src.a2, src.n, src.ld, plan.P, plan.G32, plan.G, bt, w.plan.chunk_block.get(),
+                                 n_chunks, KS16, w.planeSplit.get(), w.planeV8.get(), stream_);
     else
-        launch_prep_split_qvn(src.q, src.v, src.nq, plan.P, plan.G32, plan.G, bt, plan_dev_.chunk_block.get(), n_chunks,
-                              KS16, planeSplit_.get(), planeV8_.get(), stream_);
+        launch_prep_split_qvn(src.q, src.v, src.nq, plan.P, plan.G32, plan.G, bt, w.plan.chunk_block.get(), n_chunks,
+                              KS16, w.planeSplit.get(), w.planeV8.get(), stream_);
     mark_end(kPrep);
     mark_begin(kCount);
-    launch_count(planeV8_.get(), bt, plan_dev_.items.get(), plan.q64, counter_.get(), plan.P, plan.G,
+    launch_count(w.planeV8.get(), bt, w.plan.items.get(), plan.q64, w.counters.get(), plan.P, plan.G,
                  plan.K_total / 64, d_vpair, n_wg, stream_);
     mark_end(kCount);
     const int n_items128 = static_cast<int>(plan.items128.size());
+    const FixupList fl = fixup_list(plan, kFixupTauSplit, w);
     mark_begin(kF2);
-    launch_f2_split(planeSplit_.get(), bt, plan_dev_.items128.get(), plan.q128, counter_.get() + kXcds, plan.P,
-                    plan.G32, KS16, d_vpair, d_f2,
+    launch_f2_split(w.planeSplit.get(), bt, w.plan.items128.get(), plan.q128, w.counters.get() + kXcds, plan.P,
+                    plan.G32, KS16, d_vpair, d_f2, fl,
                     static_cast<int>(round_up(std::max(1, std::min(n_cu_, n_items128)), kXcds)), stream_);
     mark_end(kF2);
+    run_fixup(plan, src, fl, d_f2, d_vpair, w);
+}
+
+// The flag list of one contraction: capacity of 1/16 of the upper-triangle entries
+// (at least 64 Ki, at most 16 Mi entries); the count is cleared on the stream.
+// STEPPE_F2_FIXUP_CAP lowers the capacity (tests use it to exercise the overflow scan).
+FixupList HipBackend::fixup_list(const F2Plan& plan, double tau, F2Work& w) {
+    const std::int64_t upper = static_cast<std::int64_t>(plan.P) * (plan.P + 1) / 2 * plan.n_block;
+    std::int64_t cap = std::min<std::int64_t>(std::max<std::int64_t>(upper / 16, 1 << 16), 1 << 24);
+    w.fix_count.ensure(1);
+    w.fix_entries.ensure(static_cast<std::size_t>(3 * cap));
+    if (const char* env = std::getenv("STEPPE_F2_FIXUP_CAP")) {
+        const long long v = std::strtoll(env, nullptr, 10);
+        if (v > 0) cap = std::min<std::int64_t>(cap, v);
+    }
+    launch_zero_u32(w.fix_count.get(), 1, stream_);
+    FixupList fl;
+    fl.count = w.fix_count.get();
+    fl.entries = w.fix_entries.get();
+    fl.cap = static_cast<unsigned>(cap);
+    fl.tau = tau;
+    return fl;
+}
+
+void HipBackend::run_fixup(const F2Plan& plan, const F2Source& src, const FixupList& fl, double* d_f2,
+                           const double* d_vpair, const F2Work& w) {
+    const BlockTable bt = w.plan.table();
+    const int n_wg = 2 * n_cu_;
+    mark_begin(kFixup);
+    if (src.a2)
+        launch_f2_fixup_counts(src.a2, src.n, src.ld, bt, fl, plan.P, plan.n_block, d_vpair, d_f2, n_wg, stream_);
+    else
+        launch_f2_fixup_qvn(src.q, src.v, src.nq, bt, fl, plan.P, plan.n_block, d_vpair, d_f2, n_wg, stream_);
+    mark_end(kFixup);
 }
 
 void HipBackend::f2_from_counts(const std::uint16_t* d_a2, const std::uint16_t* d_n, std::int64_t ld, int P,

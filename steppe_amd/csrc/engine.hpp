@@ -19,6 +19,7 @@
 #include "frontend.hpp"
 #include "fstats.hpp"
 #include "geno_io.hpp"
+#include "qpadm.hpp"
 #include "qpgraph.hpp"
 #include "qpgraph_enum.hpp"
 #include "steppe_hip.h"
@@ -44,7 +45,8 @@ void launch_prep_fp64_qvn(const double* q, const double* v, const double* n, int
                           double* pA, double* pV, double* pQ, void* pV8, hipStream_t st);
 void launch_f2_fp64(const double* pA, const double* pV, const double* pQ, const void* pV8, BlockTable bt,
                     const F2Item* items, const ItemQueues& queues, unsigned* counters, int P, int G,
-                    std::int64_t KS, double* f2, double* vpair, int n_workgroups, hipStream_t st);
+                    std::int64_t KS, double* f2, double* vpair, const FixupList& fl, int n_workgroups,
+                    hipStream_t st);
 void launch_prep_split_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P, int G32,
                               int G16, BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks,
                               std::int64_t KS16, std::uint8_t* planes, void* pV8, hipStream_t st);
@@ -56,11 +58,20 @@ void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const Ite
                   hipStream_t st);
 void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
                      unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
-                     int n_workgroups, hipStream_t st);
+                     const FixupList& fl, int n_workgroups, hipStream_t st);
+void launch_f2_fixup_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, BlockTable bt,
+                            const FixupList& fl, int P, int n_block, const double* vpair, double* f2,
+                            int n_workgroups, hipStream_t st);
+void launch_f2_fixup_qvn(const double* q, const double* v, const double* n, BlockTable bt, const FixupList& fl,
+                         int P, int n_block, const double* vpair, double* f2, int n_workgroups, hipStream_t st);
 void launch_synth_panel(std::uint64_t seed, int P, std::int64_t M, std::int64_t n_ind, std::int64_t bpr,
                         const float* fst, const float* pop_missing, const std::int32_t* ind_pop,
                         const std::int8_t* ind_haploid, float* freq_scratch, std::uint8_t* packed,
                         hipStream_t st);
+
+// Largest population (individuals) the 16-bit counts hold: a2 <= 4 * size fits uint16.
+constexpr int kMaxPopForU16 = 16383;
+
 
 // ---- block plan ----------------------------------------------------------------------
 // Host description of one f2 launch: block ranges, padded K layout, LPT work items.
@@ -78,6 +89,56 @@ struct F2Plan {
 // Validates block_id (contiguous, non-decreasing, in [0, n_block); the reference's
 // core::block_ranges throws on the same conditions, block_partition_rule.hpp:54-96).
 F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_block);
+
+// Device workspace of one f2 contraction: the block plan, the operand planes, the work
+// queue cursors and the fix-up list.  The backend keeps one for its eager calls; a
+// captured f2 graph owns its own, so that later calls cannot move the buffers its
+// kernels were captured with.
+struct F2Work {
+    struct PlanDev {
+        DevBuf<std::int64_t> s0, k0;
+        DevBuf<std::int32_t> len, kpad, chunk_block;
+        DevBuf<F2Item> items, items128;
+        BlockTable table() const;
+    };
+    PlanDev plan;
+    DevBuf<double> planeA, planeV, planeQ;      // FP64 operand planes
+    DevBuf<std::uint8_t> planeV8;               // int8 V plane (pair counts)
+    DevBuf<std::uint8_t> planeSplit;            // split-BF16 digit planes
+    DevBuf<unsigned> counters;                  // work-queue cursors: pair counts, contraction
+    DevBuf<unsigned> fix_count;                 // cancellation-prone entries flagged
+    DevBuf<std::int32_t> fix_entries;
+};
+
+// A resident extraction of one fixed shape captured as a HIP graph: ploidy detection,
+// decode, operand prep, pair counts, contraction and fix-up of a device-resident tile
+// into caller-owned f2 / vpair buffers, replayed with one launch.  The graph owns its
+// workspace (plan, planes, queue cursors, fix-up list, population offsets, ploidy);
+// the caller keeps the tile's packed records and the output buffers alive.
+struct F2Graph {
+    int precision = 0;
+    int P = 0;
+    std::int64_t M = 0;
+    std::size_t bpr = 0, n_ind = 0;
+    const std::uint8_t* packed = nullptr;
+    int ploidy_mode = 0;                     // 0 detect on device, 1 given per sample, 2 fixed
+    int fixed_ploidy = 0;
+    DevBuf<std::uint64_t> pop_offsets;
+    DevBuf<std::int8_t> ploidy;
+    std::uint16_t* a2 = nullptr;
+    std::uint16_t* n = nullptr;
+    std::int64_t ld = 0;
+    DevBuf<std::uint16_t> own_a2, own_n;
+    double* f2 = nullptr;
+    double* vpair = nullptr;
+    F2Plan plan;
+    F2Work work;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool timed = false;                      // the stage events are part of the graph
+    int device = 0;
+    ~F2Graph();
+};
 
 // Where an f2 result lives (the reference's OutputTier, src/device/tier_select.hpp:12-17):
 // in HBM, in host memory, or in an STPF2BK1 f2.bin file written block range by block range.
@@ -254,6 +315,11 @@ public:
                      double* vpair, std::int32_t* block_sizes);
     F2Blocks* extract_f2(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision,
                          const TierRequest& tier = TierRequest{kTierResident, "", 0});
+    // Capture / replay of a fixed-shape resident extraction (f2_graph.cpp).  a2 / n may be
+    // null (the graph then holds its own counts).
+    F2Graph* f2_graph_create(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision,
+                             std::uint16_t* a2, std::uint16_t* n, std::int64_t ld, double* f2, double* vpair);
+    void f2_graph_launch(F2Graph& g);
     // Genotype files -> resident f2 (frontend_host.cpp): read the packed triple, build the
     // individual-major tile in HBM, decode, keep / compact SNPs, block, contract.
     F2Blocks* extract_f2_files(const ExtractRequest& req, ExtractInfo& info);
@@ -299,7 +365,10 @@ public:
     void qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOptions& o, QgFleetOut& out);
 
     // Per-stage device timing with HIP events on this backend's stream (opt-in).
-    enum Stage { kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kQpgraph = 5, kFstat = 6, kFront = 7, kStages = 8 };
+    enum Stage {
+        kDecode = 0, kPrep = 1, kF2 = 2, kCount = 3, kQpadm = 4, kQpgraph = 5, kFstat = 6, kFront = 7, kFixup = 8,
+        kStages = 9
+    };
     void set_timing(bool on);
     void read_timing(double* ms, std::int64_t* count, bool reset);
 
@@ -318,13 +387,7 @@ private:
                        const std::int32_t* d_subset, const std::vector<std::int64_t>& binom, std::int64_t K,
                        int filter, double& tau, std::vector<SweepHit>& res, std::int64_t& rechecked);
     void qpgraph_upload_topologies(QgSet& s, const std::vector<const QgTopology*>& topos);
-    struct PlanDev {
-        DevBuf<std::int64_t> s0, k0;
-        DevBuf<std::int32_t> len, kpad, chunk_block;
-        DevBuf<F2Item> items, items128;
-        BlockTable table() const;
-    };
-    void upload_plan(const F2Plan& plan, PlanDev& pd);
+    void upload_plan(const F2Plan& plan, F2Work::PlanDev& pd);
     // The plan for (P, M, block_id, n_block), rebuilt and re-uploaded only when the
     // partition differs from the previous call's (compared element by element).
     const F2Plan& plan_for(int P, std::int64_t M, const std::int32_t* block_id, int n_block);
@@ -332,6 +395,8 @@ private:
     std::vector<std::int32_t> plan_bid_;
     bool plan_valid_ = false;
     void prepare_tile(const stp_tile& t);
+    // The launches of one graph replay, on this backend's stream (captured or eager).
+    void record_f2_graph(F2Graph& g);
     // Where the f2 prep reads frequencies from: device counts (a2, n, ld) or device
     // FP64 Q/V/N [P x M].
     struct F2Source {
@@ -342,7 +407,14 @@ private:
         const double* v = nullptr;
         const double* nq = nullptr;
     };
-    void run_f2(const F2Plan& plan, int precision, const F2Source& src, double* d_f2, double* d_vpair);
+    void run_f2(const F2Plan& plan, int precision, const F2Source& src, double* d_f2, double* d_vpair) {
+        run_f2(plan, precision, src, d_f2, d_vpair, work_);
+    }
+    // The contraction on a given workspace (whose plan must already hold `plan`).
+    void run_f2(const F2Plan& plan, int precision, const F2Source& src, double* d_f2, double* d_vpair, F2Work& w);
+    FixupList fixup_list(const F2Plan& plan, double tau, F2Work& w);
+    void run_fixup(const F2Plan& plan, const F2Source& src, const FixupList& fl, double* d_f2, const double* d_vpair,
+                   const F2Work& w);
     // f2 over device counts into the requested tier (tiered.cpp): the whole tensor in HBM,
     // or block ranges computed into a HBM slab and streamed to host memory or an f2.bin.
     F2Blocks* f2_counts_tiered(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P,
@@ -359,13 +431,10 @@ private:
     DevBuf<std::uint64_t> pop_offsets_;
     DevBuf<std::int8_t> ploidy_;
     DevBuf<std::uint16_t> a2_, n_;
-    DevBuf<double> planeA_, planeV_, planeQ_;
-    DevBuf<std::uint8_t> planeV8_;
-    DevBuf<std::uint8_t> planeSplit_;
     DevBuf<unsigned> counter_;
     DevBuf<double> qvn_;
     DevBuf<float> synth_freq_;
-    PlanDev plan_dev_;
+    F2Work work_;
 
     // Stage timing: events bracket each stage launch; elapsed times are folded
     // into the totals at the next sync.
@@ -378,6 +447,13 @@ private:
     DevBuf<double> qpT_, qp_scratch_;
     DevBuf<std::int32_t> qp_i32_, qp_keep_;
     DevBuf<stp_qpadm_result> qp_out_;
+    // Large-model qpAdm working set (grow-only): model and task lists, fit tables,
+    // per-model setup scratch, per-wavefront fit scratch, leave-one-block-out weights.
+    void qpadm_large(QpLargeArgs a, const std::vector<QpLargeModel>& large, int rank_opt);
+    DevBuf<QpLargeModel> ql_models_;
+    DevBuf<QpLargeTask> ql_tasks_;
+    DevBuf<QpLargeTables> ql_tables_;
+    DevBuf<double> ql_setup_, ql_fit_, ql_wm_;
     // f-statistics working set (grow-only): survivor sizes / index tuples, outputs,
     // sweep combinatorics and hit buffers.
     DevBuf<std::int32_t> fs_i32_, fs_idx_;

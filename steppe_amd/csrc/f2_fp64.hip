@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void f2_fp64_kernel(
     const double* __restrict__ pA, const double* __restrict__ pV, const double* __restrict__ pQ,
     const i4* __restrict__ pV8, BlockTable bt, const F2Item* __restrict__ items, ItemQueues queues,
     unsigned* __restrict__ counters, int P, int G, std::int64_t KS, double* __restrict__ f2,
-    double* __restrict__ vpair) {
+    double* __restrict__ vpair, FixupList fl) {
     const int lane = threadIdx.x & 63;
     // v_mfma_f64_16x16x4f64 returns D[(l/16) + 4e][l%16] in lane l, item e, while the
     // int8 count MFMA returns D[4(l/16) + e][l%16].  Feeding the left operand's rows
@@ -257,13 +257,16 @@ __global__ __launch_bounds__(256) void f2_fp64_kernel(
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int row = kRowGroup * gI + 4 * (lane >> 4) + e;
-                    if (row >= P || col >= P || (gI == gJ && row > col)) continue;
+                    const bool ok = row < P && col < P && !(gI == gJ && row > col);
                     const double vp = static_cast<double>(cnt[x][y][e]);
                     const double val = vp > 0.0 ? acc[x][y][e] / vp : 0.0;
-                    f2b[row + static_cast<std::int64_t>(P) * col] = val;
-                    f2b[col + static_cast<std::int64_t>(P) * row] = val;
-                    vpb[row + static_cast<std::int64_t>(P) * col] = vp;
-                    vpb[col + static_cast<std::int64_t>(P) * row] = vp;
+                    if (ok) {
+                        f2b[row + static_cast<std::int64_t>(P) * col] = val;
+                        f2b[col + static_cast<std::int64_t>(P) * row] = val;
+                        vpb[row + static_cast<std::int64_t>(P) * col] = vp;
+                        vpb[col + static_cast<std::int64_t>(P) * row] = vp;
+                    }
+                    fixup_append(fl, ok && vp > 0.0 && fabs(val) < fl.tau, b, row, col);
                 }
             }
     }
@@ -301,11 +304,12 @@ void launch_prep_fp64_qvn(const double* q, const double* v, const double* n, int
 
 void launch_f2_fp64(const double* pA, const double* pV, const double* pQ, const void* pV8, BlockTable bt,
                     const F2Item* items, const ItemQueues& queues, unsigned* counters, int P, int G,
-                    std::int64_t KS, double* f2, double* vpair, int n_workgroups, hipStream_t st) {
+                    std::int64_t KS, double* f2, double* vpair, const FixupList& fl, int n_workgroups,
+                    hipStream_t st) {
     if (queues.off[kXcds] <= 0) return;
     launch_zero_u32(counters, kXcds, st);
     f2_fp64_kernel<<<static_cast<unsigned>(n_workgroups), 256, 0, st>>>(
-        pA, pV, pQ, static_cast<const i4*>(pV8), bt, items, queues, counters, P, G, KS, f2, vpair);
+        pA, pV, pQ, static_cast<const i4*>(pV8), bt, items, queues, counters, P, G, KS, f2, vpair, fl);
     STP_LAUNCH_CHECK();
 }
 

@@ -53,6 +53,30 @@ struct ItemQueues {
     std::int32_t off[kXcds + 1];
 };
 
+// Cancellation-prone entries.  Where a block's f2 is close to zero, the per-SNP terms
+// (q_i - q_j)^2 - h_i - h_j cancel almost completely and the contraction's rounding
+// (FP64 MFMA sums, or the split form's 48-bit digits) is large relative to the result.
+// The contraction epilogues append every upper-triangle entry with 0 < vpair and
+// |f2| < tau to this list; f2_fixup.hip then recomputes those entries from the
+// frequencies with per-SNP FP64 terms and a double-double sum.  Entries past `cap`
+// are not stored, and a scan over the whole tensor handles that case.
+struct FixupList {
+    unsigned* count;          // entries flagged by the launch (may exceed cap)
+    std::int32_t* entries;    // 3 per entry: block, row, column (row <= column)
+    unsigned cap;
+    double tau;
+};
+
+// |f2| thresholds below which the contraction's result is recomputed.  Split path: its
+// 48-bit digits leave up to ~1.4e-14 absolute (measured over 684M entries; ~6e-14 worst
+// case), so entries below 1e-4 are recomputed to keep every entry within 1e-9 relative.
+// FP64 path: its sums are off by ~1e-18 on entries that cancel to ~0 and by ~1e-16 at
+// |f2| ~ 1e-4, so the same threshold keeps it within 1e-12 relative; a larger one would
+// send whole tensors of closely related populations (f2 ~ 1e-3 everywhere) through the
+// recomputation.
+constexpr double kFixupTauFp64 = 1e-4;
+constexpr double kFixupTauSplit = 1e-4;
+
 #ifdef __HIPCC__
 // Claims the next item for a worker homed on die `home`: its own queue first, then
 // the other dies' queues in turn once it is drained (`xq` counts the queues already
@@ -69,6 +93,26 @@ __device__ inline int claim_item(unsigned* counters, const ItemQueues& q, int ho
         ++xq;
     }
     return -1;
+}
+
+// Appends (b, row, col) for every lane of the wavefront whose flag is set; all 64 lanes
+// call it together.  One atomic per wavefront with a flagged lane.
+__device__ inline void fixup_append(const FixupList& fl, bool flag, int b, int row, int col) {
+    const unsigned long long m = __ballot(flag);
+    if (m == 0ull) return;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(fl.count, static_cast<unsigned>(__popcll(m)));
+    base = __shfl(base, leader);
+    if (flag) {
+        const unsigned k = base + static_cast<unsigned>(__popcll(m & ((1ull << lane) - 1ull)));
+        if (k < fl.cap) {
+            fl.entries[3 * static_cast<std::int64_t>(k)] = b;
+            fl.entries[3 * static_cast<std::int64_t>(k) + 1] = row;
+            fl.entries[3 * static_cast<std::int64_t>(k) + 2] = col;
+        }
+    }
 }
 #endif
 

@@ -419,7 +419,7 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                                                        const F2Item* __restrict__ items, ItemQueues queues,
                                                        unsigned* __restrict__ counters, int P, int G32,
                                                        std::int64_t KS16, const double* __restrict__ vpair,
-                                                       double* __restrict__ f2) {
+                                                       double* __restrict__ f2, FixupList fl) {
     // One LDS object per ring slot: the compiler then knows that reads of one slot do
     // not depend on the direct loads still landing in the others.  Two q-digit slots
     // alternate between K-steps; one slot each for e0..e2 and for e3..e5 + V.
@@ -600,18 +600,22 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const int row = 32 * gI + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-                if (row >= P || col >= P || (gI == gJ && row > col)) continue;
-                const double s = (static_cast<double>(H[x][e]) + static_cast<double>(L[x][0][e])) * 0x1p-14 +
-                                 static_cast<double>(L[x][1][e]) * 0x1p-22 +
-                                 static_cast<double>(L[x][2][e]) * 0x1p-30 +
-                                 static_cast<double>(L[x][3][e]) * 0x1p-38 +
-                                 static_cast<double>(L[x][4][e]) * 0x1p-46;
-                const double num = -2.0 * s;
-                const std::int64_t ij = row + static_cast<std::int64_t>(P) * col;
-                const double vp = vpb[ij];
-                const double val = vp > 0.0 ? num / vp : 0.0;
-                f2b[ij] = val;
-                f2b[col + static_cast<std::int64_t>(P) * row] = val;
+                const bool ok = row < P && col < P && !(gI == gJ && row > col);
+                double val = 0.0, vp = 0.0;
+                if (ok) {
+                    const double s = (static_cast<double>(H[x][e]) + static_cast<double>(L[x][0][e])) * 0x1p-14 +
+                                     static_cast<double>(L[x][1][e]) * 0x1p-22 +
+                                     static_cast<double>(L[x][2][e]) * 0x1p-30 +
+                                     static_cast<double>(L[x][3][e]) * 0x1p-38 +
+                                     static_cast<double>(L[x][4][e]) * 0x1p-46;
+                    const double num = -2.0 * s;
+                    const std::int64_t ij = row + static_cast<std::int64_t>(P) * col;
+                    vp = vpb[ij];
+                    val = vp > 0.0 ? num / vp : 0.0;
+                    f2b[ij] = val;
+                    f2b[col + static_cast<std::int64_t>(P) * row] = val;
+                }
+                fixup_append(fl, ok && vp > 0.0 && fabs(val) < fl.tau, b, row, col);
             }
         }
     }
@@ -658,11 +662,11 @@ void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const Ite
 
 void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
                      unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
-                     int n_workgroups, hipStream_t st) {
+                     const FixupList& fl, int n_workgroups, hipStream_t st) {
     if (queues.off[kXcds] <= 0) return;
     launch_zero_u32(counters, kXcds, st);
     f2_split_kernel<<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues, counters, P,
-                                                                        G32, KS16, vpair, f2);
+                                                                        G32, KS16, vpair, f2, fl);
     STP_LAUNCH_CHECK();
 }
 

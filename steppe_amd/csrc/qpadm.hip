@@ -42,6 +42,7 @@
 #include "qpadm.hpp"
 #include "stp_qpadm_result.h"
 #include "jackknife_dev.hpp"
+#include "qpadm_record.hpp"
 #include "wave_linalg.hpp"
 
 namespace stp {
@@ -52,9 +53,9 @@ using namespace wl;
 
 constexpr int kQWaves = 4;
 constexpr int kQThreads = 64 * kQWaves;
-constexpr int kMaxNl = STP_QPADM_MAX_NL;
-constexpr int kMaxRows = kMaxNl + 1;   // popdrop rows
-constexpr int kMaxRanks = kMaxNl + 1;  // fits per row: ranks 0..8
+constexpr int kMaxNl = kQpMaxNl;
+constexpr int kMaxRows = kQpMaxRows;
+constexpr int kMaxRanks = kQpMaxRanks;
 
 // ---------------------------------------------------------------------------------
 // per-wave LDS slice and one ALS fit
@@ -351,17 +352,6 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq, FitShape s, int i
     return true;
 }
 
-__device__ bool feasible_w(const double* w, int n) {
-    bool any = false;
-    for (int i = 0; i < n; ++i) {
-        const double v = w[i];
-        if (v != v) continue;
-        any = true;
-        if (v < 0.0 || v > 1.0) return false;
-    }
-    return any;
-}
-
 // ---------------------------------------------------------------------------------
 // the batched kernel
 // ---------------------------------------------------------------------------------
@@ -541,109 +531,9 @@ __global__ __launch_bounds__(kQThreads) void qpadm_kernel(QpadmArgs g) {
         // ---- results (wave 0; lane 0 writes the record fields) ----
         if (tid < 64) {
             double se = 0.0;
-            if (do_se && lane < nl) {
-                const double scale = static_cast<double>(nbs - 1) / sqrt(static_cast<double>(nbs));
-                double mean = 0.0;
-                for (int b = 0; b < nbs; ++b) mean += sWM[static_cast<std::int64_t>(b) * nl + lane] * scale;
-                mean /= nbs;
-                dd acc{0.0, 0.0};
-                for (int b = 0; b < nbs; ++b) {
-                    const double d = sWM[static_cast<std::int64_t>(b) * nl + lane] * scale - mean;
-                    acc = dd_fma(acc, d, d);
-                }
-                se = sqrt(dd_div_d(acc, static_cast<double>(nbs - 1)));
-            }
-            const int rmax = min(nl, nr) - 1;
-            // p of each swept rank, one lane per rank
-            if (main_ok && lane <= rmax) s_prank[lane] = chisq_upper(s_chisq[0][lane], dof_of(nl, nr, lane));
-            __builtin_amdgcn_wave_barrier();
-            if (lane == 0) {
-                stp_qpadm_result* o = out;
-                o->status = 0;
-                o->model_index = mi;
-                o->est_rank = rank;
-                o->dof = dof;
-                o->nl = 0;
-                o->n_rank = 0;
-                o->n_popdrop = 0;
-                o->f4rank = 0;
-                o->p = 0.0;
-                o->chisq = 0.0;
-                const bool lanes_write_se = main_ok && do_se;
-                for (int i = 0; i < kMaxNl; ++i) {
-                    o->weight[i] = 0.0;
-                    if (!lanes_write_se || i >= nl) {
-                        o->se[i] = 0.0;
-                        o->z[i] = 0.0;
-                    }
-                }
-                if (!main_ok) {
-                    o->status = 2;  // RankDeficient
-                } else {
-                    o->nl = nl;
-                    o->chisq = s_chisq[0][rank];
-                    o->p = chisq_upper(s_chisq[0][rank], dof);
-                    for (int i = 0; i < nl; ++i) o->weight[i] = wmain[i];
-                    // rank sweep of the full model
-                    o->n_rank = rmax + 1;
-                    int f4 = rmax;
-                    for (int rr = rmax; rr >= 0; --rr)
-                        if (s_prank[rr] > g.alpha) f4 = rr;
-                    o->f4rank = f4;
-                    for (int rr = 0; rr <= rmax; ++rr) {
-                        o->rank_chisq[rr] = s_chisq[0][rr];
-                        o->rank_dof[rr] = dof_of(nl, nr, rr);
-                    }
-                    for (int kk = 0; kk <= rmax; ++kk) {
-                        const int rr = rmax - kk;
-                        o->rankdrop_f4rank[kk] = rr;
-                        o->rankdrop_dof[kk] = dof_of(nl, nr, rr);
-                        o->rankdrop_chisq[kk] = s_chisq[0][rr];
-                        o->rankdrop_p[kk] = s_prank[rr];
-                        if (rr >= 1) {
-                            const int ddf = dof_of(nl, nr, rr - 1) - dof_of(nl, nr, rr);
-                            const double cd = s_chisq[0][rr - 1] - s_chisq[0][rr];
-                            o->rankdrop_dofdiff[kk] = ddf;
-                            o->rankdrop_chisqdiff[kk] = cd;
-                            o->rankdrop_p_nested[kk] = chisq_upper(cd, ddf);
-                        } else {
-                            o->rankdrop_dofdiff[kk] = INT_MIN;
-                            o->rankdrop_chisqdiff[kk] = __builtin_nan("");
-                            o->rankdrop_p_nested[kk] = __builtin_nan("");
-                        }
-                    }
-                    // popdrop rows: the full model, then each source dropped (last first)
-                    o->n_popdrop = nrows;
-                    for (int k = 0; k < nrows; ++k) {
-                        const int drop = k == 0 ? -1 : nl - k;
-                        const int nlr = k == 0 ? nl : nl - 1;
-                        const int rmx = min(nlr, nr) - 1, rf = nlr - 1;
-                        for (int c = 0; c < nl; ++c) o->popdrop_pat[k][c] = (c == drop) ? '1' : '0';
-                        o->popdrop_pat[k][nl] = '\0';
-                        o->popdrop_wt[k] = drop >= 0 ? 1 : 0;
-                        o->popdrop_f4rank[k] = rf;
-                        const int pdof = dof_of(nlr, nr, rf);
-                        o->popdrop_dof[k] = pdof;
-                        o->popdrop_chisq[k] = rf <= rmx ? s_chisq[k][rf] : 0.0;
-                        o->popdrop_p[k] = rf <= rmx ? chisq_upper(s_chisq[k][rf], pdof) : 0.0;
-                        // feasible: every weight of the row's fit in [0, 1] (the dropped
-                        // source is NaN and ignored); a failed fit is infeasible
-                        bool feas = s_ok[k][rf] != 0 && nlr > 0;
-                        if (feas)
-                            for (int ii = 0; ii < nlr; ++ii) {
-                                const double v = s_wrf[k][ii];
-                                if (!(v >= 0.0 && v <= 1.0)) feas = false;
-                            }
-                        o->popdrop_feasible[k] = feas ? 1 : 0;
-                    }
-                    o->status = dof <= 0 ? 4 : 0;  // ChisqUndefined
-                }
-            }
-            if (main_ok && do_se && lane < nl) {
-                // each lane owns its weight's SE and z (no other lane writes these fields)
-                out->se[lane] = se;
-                out->z[lane] = se > 0.0 ? wmain[lane] / se : 0.0;
-            }
+            if (do_se && lane < nl) se = loo_se(sWM, nbs, nl, lane);
+            write_qpadm_record(out, mi, nl, nr, rank, g.alpha, s_chisq, s_ok, s_wrf, wmain, main_ok, do_se, se,
+                               s_prank);
         }
         __syncthreads();
     }

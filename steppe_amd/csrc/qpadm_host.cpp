@@ -26,6 +26,15 @@ struct ModelShape {
 
 // The normal-matrix bucket of a model: rank sweeps and LOO fits run at ranks up to
 // nl - 1, so the largest systems are nl*(nl-1) (A-step) and (nl-1)*nr (B-step).
+// Models the batched kernel fits: one f4 entry per lane, one normal-matrix row per lane.
+bool fits_batched(int nl, int nr) { return nl * nr <= kMaxQpadmM && nr <= kMaxQpadmNr; }
+
+// Largest normal system of a large model's fits (rank sweep and refits up to nl - 1).
+int large_t(int nl, int nr, int rank) {
+    const int rmax = std::max(nl - 1, rank);
+    return std::max({nl * rmax, rmax * nr, nl});
+}
+
 int bucket_of(int nl, int nr, int rank) {
     const int rmax = std::max(nl - 1, rank);
     const int t = std::max({nl * rmax, rmax * nr, nl});
@@ -60,10 +69,15 @@ void HipBackend::qpadm_batch(const F2Blocks& h, int n_models, const std::int32_t
         STP_REQUIRE(nl >= 1, tag + "needs at least one source population");
         STP_REQUIRE(nr >= 1, tag + "needs at least two right populations");
         STP_REQUIRE(nl <= STP_QPADM_MAX_NL, tag + "more than " + std::to_string(STP_QPADM_MAX_NL) + " sources");
-        STP_REQUIRE(nl <= nr, tag + "needs at least as many right populations beyond the first as sources");
-        STP_REQUIRE(nl * nr <= kMaxQpadmM && nr <= kMaxQpadmNr,
-                    tag + "model too large for the device fit (sources x (right - 1) must be <= 64)");
+        // The reference seeds the rank nl - 1 fit from nl - 1 right singular vectors of the
+        // nl x nr f4 matrix (seed_AB, cpu_backend.cpp:2738-2744), which exist for nl <= nr + 1.
+        STP_REQUIRE(nl <= nr + 1, tag + "needs at least sources - 1 right populations beyond the first");
         STP_REQUIRE(opt.rank < nl, tag + "rank must be below the number of sources");
+        if (!fits_batched(nl, nr)) {
+            STP_REQUIRE(nl * nr <= kQpLargeMaxM && nr <= kQpLargeMaxNr && large_t(nl, nr, opt.rank) <= kQpLargeMaxT,
+                        tag + "model too large (sources x (right - 1) must be <= " + std::to_string(kQpLargeMaxM) +
+                            ", right - 1 <= " + std::to_string(kQpLargeMaxNr) + ")");
+        }
         STP_REQUIRE(target[i] >= 0 && target[i] < P, tag + "target index out of range");
         for (int k = left_off[i]; k < left_off[i + 1]; ++k)
             STP_REQUIRE(left[k] >= 0 && left[k] < P, tag + "left index out of range");
@@ -100,8 +114,9 @@ void HipBackend::qpadm_batch(const F2Blocks& h, int n_models, const std::int32_t
     for (int s : bsz) STP_REQUIRE(s > 0, "qpadm: surviving blocks must have positive sizes");
 
     // ---- device arena: survivors, sizes, models, per-bucket claim orders ----
-    std::vector<int> order(n_models);
-    std::iota(order.begin(), order.end(), 0);
+    std::vector<int> order, large;
+    for (int i = 0; i < n_models; ++i) (fits_batched(shape[i].nl, shape[i].nr) ? order : large).push_back(i);
+    const int n_small = static_cast<int>(order.size());
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         if (shape[a].T != shape[b].T) return shape[a].T < shape[b].T;
         return shape[a].cost > shape[b].cost;
@@ -134,10 +149,10 @@ void HipBackend::qpadm_batch(const F2Blocks& h, int n_models, const std::int32_t
     // Each launch group fills every CU to the kernel's occupancy; every workgroup has its
     // own scratch slice (X, LOO and the LOO weights), sized for the largest group.
     std::size_t scratch_elems = 0;
-    for (int slot = 0; slot < n_models;) {
+    for (int slot = 0; slot < n_small;) {
         const int T = shape[order[slot]].T;
         int end = slot, m_max = 0, nr_max = 0;
-        for (; end < n_models && shape[order[end]].T == T; ++end) {
+        for (; end < n_small && shape[order[end]].T == T; ++end) {
             m_max = std::max(m_max, shape[order[end]].m);
             nr_max = std::max(nr_max, shape[order[end]].nr);
         }
@@ -151,11 +166,11 @@ void HipBackend::qpadm_batch(const F2Blocks& h, int n_models, const std::int32_t
     mark_begin(kQpadm);
     int slot = 0;
     int bucket_id = 0;
-    while (slot < n_models) {
+    while (slot < n_small) {
         const int T = shape[order[slot]].T;
         int end = slot;
         int m_max = 0, nr_max = 0;
-        while (end < n_models && shape[order[end]].T == T) {
+        while (end < n_small && shape[order[end]].T == T) {
             m_max = std::max(m_max, shape[order[end]].m);
             nr_max = std::max(nr_max, shape[order[end]].nr);
             ++end;
@@ -196,10 +211,132 @@ void HipBackend::qpadm_batch(const F2Blocks& h, int n_models, const std::int32_t
         ++bucket_id;
         STP_REQUIRE(bucket_id < 32, "qpadm: too many launch groups");
     }
+    if (!large.empty()) {
+        QpLargeArgs a{};
+        a.f2T = qpT_.get();
+        a.bsz = qp_i32_.get() + o_bsz;
+        a.P = P;
+        a.nbs = nbs;
+        a.n_total = n_total;
+        a.target = qp_i32_.get() + o_tgt;
+        a.left = qp_i32_.get() + o_left;
+        a.right = qp_i32_.get() + o_right;
+        a.out = qp_out_.get();
+        a.fudge = opt.fudge;
+        a.iters = opt.als_iterations;
+        a.alpha = opt.rank_alpha;
+        a.jackknife = opt.jackknife;
+        a.se_require_p = opt.se_require_p;
+        a.p_se_threshold = opt.p_se_threshold;
+        std::vector<QpLargeModel> lm;
+        for (int mi : large) {
+            QpLargeModel md{};
+            md.mi = mi;
+            md.l0 = left_off[mi];
+            md.nl = shape[mi].nl;
+            md.r0 = right_off[mi];
+            md.nr = shape[mi].nr;
+            md.m = md.nl * md.nr;
+            md.rank = opt.rank < 0 ? md.nl - 1 : opt.rank;
+            lm.push_back(md);
+        }
+        qpadm_large(a, lm, opt.rank);
+    }
     mark_end(kQpadm);
     STP_HIP(hipMemcpyAsync(out, qp_out_.get(), static_cast<std::size_t>(n_models) * sizeof(stp_qpadm_result),
                            hipMemcpyDeviceToHost, stream_));
     sync();
+}
+
+// Models beyond the batched kernel: chunks of models whose setup scratch fits a fixed
+// budget; per chunk the setup kernel, the fit-table fits, the leave-one-block-out refits
+// and the record kernel, each one launch.
+void HipBackend::qpadm_large(QpLargeArgs a, const std::vector<QpLargeModel>& large, int rank_opt) {
+    constexpr std::size_t kSetupBudget = std::size_t{1} << 29;   // doubles (4 GiB)
+    constexpr std::size_t kFitBudget = std::size_t{1} << 28;     // doubles (2 GiB)
+    const int nbs = a.nbs;
+    std::size_t i0 = 0;
+    while (i0 < large.size()) {
+        // ---- the chunk's models and their setup scratch ----
+        std::vector<QpLargeModel> models;
+        std::size_t setup = 0;
+        int m_max = 0, nr_max = 0, t_max = 0;
+        for (; i0 < large.size(); ++i0) {
+            QpLargeModel md = large[i0];
+            const int nl = md.nl, nr = md.nr, m = md.m;
+            const std::size_t need = qpadm_large_setup_doubles(m, nbs);
+            if (!models.empty() && (setup + need > kSetupBudget || models.size() >= 4096)) break;
+            md.base = static_cast<std::int64_t>(setup);
+            models.push_back(md);
+            setup += need;
+            m_max = std::max(m_max, m);
+            nr_max = std::max(nr_max, nr);
+            t_max = std::max(t_max, large_t(nl, nr, rank_opt));
+        }
+        const int nm = static_cast<int>(models.size());
+        // ---- fit tasks: every model's fit table, then its leave-one-block-out refits ----
+        std::vector<QpLargeTask> tasks;
+        for (int sl = 0; sl < nm; ++sl) {
+            const int nl = models[sl].nl, nr = models[sl].nr, rank = models[sl].rank;
+            const int nrows = nl >= 2 ? nl + 1 : 1;
+            for (int k = 0; k < nrows; ++k) {
+                const int nlr = k == 0 ? nl : nl - 1;
+                const int rmax = std::min(nlr, nr) - 1, rf = nlr - 1;
+                for (int rr = 0; rr <= rmax; ++rr) tasks.push_back({sl, k, rr, -1});
+                if (rf > rmax) tasks.push_back({sl, k, rf, -1});
+                if (k == 0 && rank > rmax && rank != rf) tasks.push_back({sl, 0, rank, -1});
+            }
+        }
+        const int n_table = static_cast<int>(tasks.size());
+        if (a.jackknife != 0 && nbs >= 2)
+            for (int sl = 0; sl < nm; ++sl)
+                for (int b = 0; b < nbs; ++b) tasks.push_back({sl, 0, 0, b});
+        const int n_loo = static_cast<int>(tasks.size()) - n_table;
+
+        ql_models_.ensure(static_cast<std::size_t>(nm));
+        STP_HIP(hipMemcpyAsync(ql_models_.get(), models.data(), nm * sizeof(QpLargeModel), hipMemcpyHostToDevice,
+                               stream_));
+        ql_tasks_.ensure(tasks.size());
+        STP_HIP(hipMemcpyAsync(ql_tasks_.get(), tasks.data(), tasks.size() * sizeof(QpLargeTask),
+                               hipMemcpyHostToDevice, stream_));
+        ql_tables_.ensure(static_cast<std::size_t>(nm));
+        launch_zero_u32(reinterpret_cast<unsigned*>(ql_tables_.get()),
+                        static_cast<int>(nm * sizeof(QpLargeTables) / sizeof(unsigned)), stream_);
+        ql_setup_.ensure(setup);
+        ql_wm_.ensure(static_cast<std::size_t>(nm) * nbs * STP_QPADM_MAX_NL);
+        const std::size_t per_wave = qpadm_large_fit_doubles(m_max, nr_max, t_max);
+        const int waves_per_wg = qpadm_large_fit_waves();
+        int n_wg = std::min<int>(2 * n_cu_, static_cast<int>(ceil_div(std::max(n_table, n_loo), waves_per_wg)));
+        n_wg = std::max(1, std::min<int>(n_wg, static_cast<int>(kFitBudget / (per_wave * waves_per_wg))));
+        ql_fit_.ensure(per_wave * waves_per_wg * n_wg);
+        counter_.ensure(64);
+
+        a.models = ql_models_.get();
+        a.n_models = nm;
+        a.tables = ql_tables_.get();
+        a.setup = ql_setup_.get();
+        a.fit = ql_fit_.get();
+        a.fit_stride = static_cast<std::int64_t>(per_wave);
+        a.m_max = m_max;
+        a.nr_max = nr_max;
+        a.t_max = t_max;
+        a.wm = ql_wm_.get();
+        a.counter = counter_.get() + 30;
+        launch_qpadm_large_setup(a, stream_);
+        a.tasks = ql_tasks_.get();
+        a.n_tasks = n_table;
+        launch_zero_u32(a.counter, 1, stream_);
+        launch_qpadm_large_fits(a, n_wg, stream_);
+        if (n_loo > 0) {
+            a.tasks = ql_tasks_.get() + n_table;
+            a.n_tasks = n_loo;
+            launch_zero_u32(a.counter, 1, stream_);
+            launch_qpadm_large_fits(a, n_wg, stream_);
+        }
+        launch_qpadm_large_finish(a, stream_);
+        // the chunk's task list (host memory) must outlive its copy
+        sync();
+    }
 }
 
 }  // namespace stp

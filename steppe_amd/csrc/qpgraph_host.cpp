@@ -659,7 +659,9 @@ void HipBackend::qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOption
     std::vector<std::int32_t> iters(m);
     qpgraph_set_fit(s, m, ig.data(), x0.data(), o, th.data(), sc.data(), iters.data());
     // best restart per graph (first of equal scores, over every start); the spread and
-    // per-coordinate range are over the reference's restarts, as it reports them
+    // per-coordinate range are over the reference's restarts and the best one, so that,
+    // as in the reference (where the best is always one of its restarts), the reported
+    // weights lie inside their range and the best score inside the spread
     std::vector<std::int32_t> best_g;
     std::vector<double> best_th;
     for (int g = 0; g < n; ++g) {
@@ -678,9 +680,11 @@ void HipBackend::qpgraph_set_fit_fleet(QgSet& s, int numstart, const QgFitOption
             if (b < 0 || sc[i] < sc[b]) b = i;
         }
         out.iterations[g] = it;
+        smin = std::min(smin, sc[b]);
+        smax = std::max(smax, sc[b]);
         out.restart_spread[g] = (std::isfinite(smin) && std::isfinite(smax)) ? smax - smin : 0.0;
         for (int j = 0; j < d; ++j) {
-            double lo = 1.0, hi = 0.0;
+            double lo = th[static_cast<std::size_t>(b) * D + j], hi = lo;
             for (int i = first[g]; i < ref_end; ++i) {
                 lo = std::min(lo, th[static_cast<std::size_t>(i) * D + j]);
                 hi = std::max(hi, th[static_cast<std::size_t>(i) * D + j]);

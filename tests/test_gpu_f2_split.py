@@ -87,3 +87,52 @@ def test_split_extract_end_to_end(engine, cref):
     fr, vr, br = cref.f2_blocks(q, v, n, p.block_id, p.n_block)
     assert np.array_equal(h._vpair_numpy(), vr)
     assert _rel_err(h._f2_numpy(), fr) <= SPLIT_RTOL
+
+
+def _near_zero_panel(seed=11):
+    """Q/V/N with exactly-zero and near-zero block f2 entries: populations 1..5 copy the
+    single pseudo-haploid individual of population 0 (h = 0, so f2 = 0 on every block),
+    population 6 differs from it at one SNP per block, the rest are random."""
+    rng = np.random.default_rng(seed)
+    P, M = 48, 4096
+    n = rng.integers(1, 120, (P, M)).astype(np.float64)
+    q = np.round(rng.random((P, M)) * 2 * n) / (2 * n)
+    n[:7] = 1.0
+    q[0] = (rng.random(M) > 0.5).astype(np.float64)
+    q[1:7] = q[0]
+    q[6, ::512] = 1.0 - q[0, ::512]
+    v = (rng.random((P, M)) > 0.05).astype(np.float64)
+    v[:7] = 1.0
+    q, n = q * v, n * v
+    bid = np.repeat(np.arange(8, dtype=np.int32), 512)
+    return q, v, n, bid
+
+
+@pytest.mark.parametrize("prec", [_core.STP_PREC_FP64, _core.STP_PREC_SPLIT_BF16])
+def test_cancellation_prone_entries_recomputed(engine, cref, prec):
+    # Entries whose terms cancel to (near) zero are recomputed in FP64 by the fix-up
+    # kernel: they match the reference's long-double sum to its rounding, far inside the
+    # FP64 path's 1e-12 bar even where the exact value is 0.
+    q, v, n, bid = _near_zero_panel()
+    f2, vp, _ = _f2_qvn_gpu(engine, q, v, n, bid, 8, prec)
+    fr, vr, _ = cref.f2_blocks(q, v, n, bid, 8)
+    assert np.array_equal(vp, vr)
+    assert np.all(fr[0, 1:6, :] == 0.0) and np.all(f2[0, 1:6, :] == 0.0)
+    small = np.abs(fr) < 1e-4
+    assert small.sum() >= 5 * 8
+    err = float(np.max(np.abs(f2 - fr)[small] / np.maximum(np.abs(fr[small]), 1e-6)))
+    _say(f"  {_core.PRECISION_NAMES[prec]}: {int(small.sum())} entries below 1e-4, max rel err {err:.2e}")
+    assert err <= 1e-12
+    assert _rel_err(f2, fr) <= (1e-12 if prec == _core.STP_PREC_FP64 else SPLIT_RTOL)
+
+
+@pytest.mark.parametrize("prec", [_core.STP_PREC_FP64, _core.STP_PREC_SPLIT_BF16])
+def test_fixup_list_overflow_scan(engine, monkeypatch, prec):
+    # With room for a single flagged entry the list overflows and the scan kernel
+    # recomputes every entry below the threshold: the same tensor as the list path.
+    q, v, n, bid = _near_zero_panel(seed=12)
+    f2_list, vp_list, _ = _f2_qvn_gpu(engine, q, v, n, bid, 8, prec)
+    monkeypatch.setenv("STEPPE_F2_FIXUP_CAP", "1")
+    f2_scan, vp_scan, _ = _f2_qvn_gpu(engine, q, v, n, bid, 8, prec)
+    assert np.array_equal(vp_list, vp_scan)
+    assert np.array_equal(f2_list, f2_scan)

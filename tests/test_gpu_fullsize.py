@@ -306,7 +306,14 @@ def test_c5_five_thousand_graphs(engine, cref):
     th, sc, it = gset.fit(ig, x0)
     start = gset.eval(ig, x0, mode=0)["score"]
     assert np.all(it >= 0) and np.all(sc <= start)
-    assert np.array_equal(gset.eval(ig, th, mode=0)["score"], sc)
+    # the fit kernel's score at its returned theta is the score kernel's, up to the
+    # rounding of its own evaluation order (the inner NNLS can end on the other side of a
+    # near-tie in its active set, which moves the score in the 12th digit)
+    again = gset.eval(ig, th, mode=0)["score"]
+    dev = np.abs(again - sc) / np.abs(sc)
+    _say(f"[c5] fit-kernel score vs score kernel at the same theta: max rel {dev.max():.1e}, "
+         f"{int(np.sum(dev > 0))} of {n} not bit-equal")
+    assert dev.max() <= 1e-10
 
     # the oracle's finite-difference search on spread graphs
     pick = np.linspace(0, n - 1, 9).round().astype(int).tolist()
