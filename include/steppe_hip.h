@@ -89,6 +89,17 @@ int stp_detect_ploidy(stp_context* ctx, const stp_tile* tile, int32_t* out_ploid
 int stp_decode_counts(stp_context* ctx, const stp_tile* tile, uint16_t* d_a2, uint16_t* d_n,
                       int64_t ld);
 
+/* stp_decode_counts that allocates no device memory.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The tile must be device-resident (packed_on_device);
+ * the population offsets and the per-sample ploidy go through a fixed device scratch array
+ * of about 5.5 MiB that each GPU holds until the process ends, so one call takes at most
+ * 65536 populations and 2^20 individuals (larger inputs: STP_ERR_INVALID).  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU race.  Outputs are the
+ * same as stp_decode_counts on the same inputs. */
+int stp_decode_counts_noalloc(stp_context* ctx, const stp_tile* tile, uint16_t* d_a2, uint16_t* d_n,
+                              int64_t ld);
+
 /* Floating decode, bit-identical to the reference's DecodeResult{q, v, n}
  * (src/device/backend.hpp:250-256): host [P x M] arrays at i + P*s. */
 int stp_decode_af(stp_context* ctx, const stp_tile* tile, double* q, double* v, double* n);

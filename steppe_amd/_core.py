@@ -152,6 +152,7 @@ def lib():
         "stp_device_info": (C.c_int, [vp, C.c_char_p, C.c_int, C.POINTER(i32)]),
         "stp_detect_ploidy": (C.c_int, [vp, C.POINTER(Tile), vp]),
         "stp_decode_counts": (C.c_int, [vp, C.POINTER(Tile), vp, vp, i64]),
+        "stp_decode_counts_noalloc": (C.c_int, [vp, C.POINTER(Tile), vp, vp, i64]),
         "stp_decode_af": (C.c_int, [vp, C.POINTER(Tile), vp, vp, vp]),
         "stp_f2_blocks_from_counts": (C.c_int, [vp, vp, vp, i64, i32, i64, vp, i32, i32, vp, vp]),
         "stp_compute_f2_blocks": (C.c_int, [vp, vp, vp, vp, i32, i64, vp, i32, i32, vp, vp, vp]),

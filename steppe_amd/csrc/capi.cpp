@@ -164,6 +164,13 @@ int stp_decode_counts(stp_context* ctx, const stp_tile* tile, uint16_t* d_a2, ui
     });
 }
 
+int stp_decode_counts_noalloc(stp_context* ctx, const stp_tile* tile, uint16_t* d_a2, uint16_t* d_n, int64_t ld) {
+    return guard([&] {
+        STP_REQUIRE(tile && d_a2 && d_n, "stp_decode_counts_noalloc: null argument");
+        backend(ctx).decode_counts_noalloc(*tile, d_a2, d_n, ld);
+    });
+}
+
 int stp_decode_af(stp_context* ctx, const stp_tile* tile, double* q, double* v, double* n) {
     return guard([&] {
         STP_REQUIRE(tile && q && v && n, "stp_decode_af: null argument");

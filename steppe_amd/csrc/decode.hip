@@ -26,6 +26,7 @@
 // and the kernel runs at HBM speed.
 
 #include "common.hpp"
+#include "engine.hpp"
 
 namespace stp {
 namespace {
@@ -200,7 +201,45 @@ __global__ void counts_to_qvn_kernel(const std::uint16_t* __restrict__ a2, const
     }
 }
 
+// Per-sample ploidy from the caller's int32 values: 1 or 2 as given, anything else 0.
+__global__ void ploidy_from_i32_kernel(const std::int32_t* __restrict__ in, std::int64_t n,
+                                       std::int8_t* __restrict__ out) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int v = in[i];
+        out[i] = static_cast<std::int8_t>((v == 1 || v == 2) ? v : 0);
+    }
+}
+
 }  // namespace
+
+// ---- scratch of the allocation-free decode (stp_decode_counts_noalloc) -------------------
+// Device copies of the population offsets and the per-sample ploidy, in a fixed array that
+// every GPU the library runs on holds until the process ends.  Layout: offsets
+// (uint64, kNoallocMaxPops + 1), then ploidy (int8, kNoallocMaxIndividuals), then the
+// caller's int32 ploidy values staged for conversion (kNoallocMaxIndividuals int32).
+constexpr std::size_t kNoallocOffsBytes = (8 * (static_cast<std::size_t>(kNoallocMaxPops) + 1) + 255) / 256 * 256;
+constexpr std::size_t kNoallocBytes =
+    kNoallocOffsBytes + static_cast<std::size_t>(kNoallocMaxIndividuals) * (1 + 4);
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+NoallocScratch resolve_noalloc_scratch() {
+    void* ptr = nullptr;
+    STP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    NoallocScratch s;
+    auto* base = static_cast<unsigned char*>(ptr);
+    s.offs = reinterpret_cast<std::uint64_t*>(base);
+    s.ploidy = reinterpret_cast<std::int8_t*>(base + kNoallocOffsBytes);
+    s.ploidy_i32 = reinterpret_cast<std::int32_t*>(base + kNoallocOffsBytes + kNoallocMaxIndividuals);
+    s.bytes = kNoallocBytes;
+    return s;
+}
+
+void launch_ploidy_from_i32(const std::int32_t* in, std::int64_t n, std::int8_t* out, hipStream_t st) {
+    if (n <= 0) return;
+    ploidy_from_i32_kernel<<<static_cast<unsigned>(ceil_div(n, 256)), 256, 0, st>>>(in, n, out);
+    STP_LAUNCH_CHECK();
+}
 
 void launch_counts_to_qvn(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P,
                           std::int64_t M, double* q, double* v, double* nout, hipStream_t st) {

@@ -111,6 +111,7 @@ HipBackend::HipBackend(int device) : device_(device) {
     name_ = prop.gcnArchName;
     STP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     counter_.alloc(64);
+    noalloc_ = resolve_noalloc_scratch();
 }
 
 HipBackend::~HipBackend() {
@@ -281,6 +282,46 @@ void HipBackend::decode_counts(const stp_tile& t, std::uint16_t* d_a2, std::uint
                          static_cast<std::int64_t>(t.n_snp), pop_offsets_.get(), ploidy_.get(), t.n_pop, d_a2,
                          d_n, ld, stream_);
     mark_end(kDecode);
+    sync();
+}
+
+void HipBackend::decode_counts_noalloc(const stp_tile& t, std::uint16_t* d_a2, std::uint16_t* d_n, std::int64_t ld) {
+    activate();
+    STP_REQUIRE(t.packed_on_device, "decode (no allocation): the tile must be resident on the device");
+    STP_REQUIRE(t.n_pop >= 0 && t.n_pop <= kNoallocMaxPops,
+                "decode (no allocation): at most " + std::to_string(kNoallocMaxPops) + " populations");
+    STP_REQUIRE(t.n_individuals <= static_cast<std::size_t>(kNoallocMaxIndividuals),
+                "decode (no allocation): at most " + std::to_string(kNoallocMaxIndividuals) + " individuals");
+    STP_REQUIRE(t.n_pop == 0 || t.pop_offsets != nullptr, "decode: pop_offsets is required");
+    STP_REQUIRE(t.bytes_per_record >= (t.n_snp + 3) / 4, "decode: bytes_per_record < ceil(n_snp/4)");
+    STP_REQUIRE(t.n_individuals == 0 || t.packed != nullptr, "decode: packed is required");
+    for (int p = 0; p < t.n_pop; ++p) {
+        STP_REQUIRE(t.pop_offsets[p] <= t.pop_offsets[p + 1], "decode: pop_offsets must be non-decreasing");
+        STP_REQUIRE(t.pop_offsets[p + 1] - t.pop_offsets[p] <= static_cast<std::uint64_t>(kMaxPopForU16),
+                    "decode: a population exceeds 16383 individuals (uint16 count limit)");
+    }
+    if (t.n_pop > 0)
+        STP_REQUIRE(t.pop_offsets[t.n_pop] <= t.n_individuals, "decode: pop_offsets exceed n_individuals");
+    const std::int64_t n_ind = static_cast<std::int64_t>(t.n_individuals);
+    if (t.n_pop > 0)
+        STP_HIP(hipMemcpyAsync(noalloc_.offs, t.pop_offsets, (t.n_pop + 1) * sizeof(std::uint64_t),
+                               hipMemcpyHostToDevice, stream_));
+    if (t.sample_ploidy) {
+        if (n_ind > 0)
+            STP_HIP(hipMemcpyAsync(noalloc_.ploidy_i32, t.sample_ploidy, n_ind * sizeof(std::int32_t),
+                                   hipMemcpyHostToDevice, stream_));
+        launch_ploidy_from_i32(noalloc_.ploidy_i32, n_ind, noalloc_.ploidy, stream_);
+    } else if (t.detect_ploidy_on_device) {
+        launch_detect_ploidy(t.packed, static_cast<std::int64_t>(t.bytes_per_record),
+                             static_cast<std::int64_t>(t.n_snp), n_ind, noalloc_.ploidy, stream_);
+    } else {
+        launch_fill_ploidy(noalloc_.ploidy, n_ind, (t.ploidy == 1 || t.ploidy == 2) ? t.ploidy : 0, stream_);
+    }
+    mark_begin(kDecode);
+    launch_decode_counts(t.packed, static_cast<std::int64_t>(t.bytes_per_record), static_cast<std::int64_t>(t.n_snp),
+                         noalloc_.offs, noalloc_.ploidy, t.n_pop, d_a2, d_n, ld, stream_);
+    mark_end(kDecode);
+    // the host arrays copied above belong to the caller: complete before returning
     sync();
 }
 

@@ -72,6 +72,19 @@ void launch_synth_panel(std::uint64_t seed, int P, std::int64_t M, std::int64_t 
 // Largest population (individuals) the 16-bit counts hold: a2 <= 4 * size fits uint16.
 constexpr int kMaxPopForU16 = 16383;
 
+// The allocation-free decode's fixed device scratch (decode.hip): at most this many
+// populations and individuals per call.
+constexpr int kNoallocMaxPops = 65536;
+constexpr std::int64_t kNoallocMaxIndividuals = std::int64_t{1} << 20;
+struct NoallocScratch {
+    std::uint64_t* offs = nullptr;       // population offsets (device address)
+    std::int8_t* ploidy = nullptr;       // per-sample ploidy
+    std::int32_t* ploidy_i32 = nullptr;  // staged caller ploidy values
+    std::size_t bytes = 0;
+};
+// This device's address of the scratch array (the device must be current).
+NoallocScratch resolve_noalloc_scratch();
+void launch_ploidy_from_i32(const std::int32_t* in, std::int64_t n, std::int8_t* out, hipStream_t st);
 
 // ---- block plan ----------------------------------------------------------------------
 // Host description of one f2 launch: block ranges, padded K layout, LPT work items.
@@ -304,6 +317,9 @@ public:
     std::vector<int> detect_ploidy(const stp_tile& t);
     // Decode into caller-owned device buffers.
     void decode_counts(const stp_tile& t, std::uint16_t* d_a2, std::uint16_t* d_n, std::int64_t ld);
+    // The same with no device allocation: population offsets and ploidy go through the
+    // fixed scratch array resolved at construction (device-resident tiles only).
+    void decode_counts_noalloc(const stp_tile& t, std::uint16_t* d_a2, std::uint16_t* d_n, std::int64_t ld);
     void decode_af(const stp_tile& t, double* q, double* v, double* n);
     // f2 over device counts into caller-owned device outputs.
     void f2_from_counts(const std::uint16_t* d_a2, const std::uint16_t* d_n, std::int64_t ld, int P,
@@ -423,6 +439,7 @@ private:
 
     int device_ = 0;
     int n_cu_ = 0;
+    NoallocScratch noalloc_;   // this device's address of the allocation-free decode's scratch
     std::string name_;
     hipStream_t stream_ = nullptr;
 
