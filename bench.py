@@ -665,6 +665,7 @@ def main():
     ap.add_argument("--no-qpgraph-search", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--eager", action="store_true", help="time eager calls instead of the captured graph")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default launch configuration")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -712,6 +713,16 @@ def main():
     for i in range(args.warmup):
         step()
         log(f"[bench] warmup {i+1}/{args.warmup} done")
+    # The contraction's launch configuration for this workload, chosen by timing every
+    # candidate (stp_f2_autotune, outside the timed region; the graph below uses it).
+    tune = None
+    if not args.no_autotune:
+        tune = {"split_bf16" if not prec == _core.STP_PREC_FP64 else "fp64":
+                _core.f2_autotune(ctx, a2.data_ptr(), nn.data_ptr(), ld, P, M, block_id, nb, prec, f2.data_ptr(),
+                                  vp.data_ptr())}
+        for k, r in tune.items():
+            log(f"[bench] autotune {k}: chosen {r['candidates'][r['chosen']]['config']} "
+                f"({r['candidates'][r['chosen']]['ms']:.2f} ms; default {r['candidates'][0]['ms']:.2f} ms)")
     # Eager steps (one stp_decode_counts + one stp_f2_blocks_from_counts call each), timed
     # for comparison with the captured graph.
     torch.cuda.synchronize()
@@ -780,6 +791,12 @@ def main():
         vpd = torch.empty_like(vp)
         torch.cuda.synchronize()
 
+        if tune is not None:
+            tune["fp64"] = _core.f2_autotune(ctx, a2.data_ptr(), nn.data_ptr(), ld, P, M, block_id, nb,
+                                             _core.STP_PREC_FP64, f2d.data_ptr(), vpd.data_ptr())
+            r = tune["fp64"]
+            log(f"[bench] autotune fp64: chosen {r['candidates'][r['chosen']]['config']} "
+                f"({r['candidates'][r['chosen']]['ms']:.2f} ms; default {r['candidates'][0]['ms']:.2f} ms)")
         if graph is not None:
             g64 = _core.F2Graph(ctx, tile, block_id, nb, _core.STP_PREC_FP64, f2d.data_ptr(), vpd.data_ptr(),
                                 a2.data_ptr(), nn.data_ptr(), ld)
@@ -859,7 +876,7 @@ def main():
                    "precision": args.precision, "parallelism": "single GPU",
                    "graph": graph is not None, "eager_ms_per_step": eager_ms, "stage_timing": stage_source},
         "roofline": roofline, "accuracy": accuracy, "stages": stages, "cpu_baseline": cpu, "qpadm": qp, "qpgraph": qg, "fstats": fs, "frontend": fe,
-        "tiers": tiers, "qpgraph_search": qs,
+        "tiers": tiers, "qpgraph_search": qs, "autotune": tune,
     }
     print(json.dumps(out), flush=True)
 

@@ -113,6 +113,21 @@ int stp_f2_blocks_from_counts(stp_context* ctx, const uint16_t* d_a2, const uint
                               int64_t ld, int32_t P, int64_t M, const int32_t* block_id,
                               int32_t n_block, int32_t precision, double* d_f2, double* d_vpair);
 
+/* Autotuner of the f2 contraction's launch configuration (replaces the reference's fixed
+ * size buckets and VRAM chunk budget, src/device/cuda/cuda_backend_f2_blocks.cu:55-74,
+ * src/device/vram_budget.hpp:63): runs stp_f2_blocks_from_counts on these device inputs
+ * under every candidate configuration (work-item schedule over the dies, FP64 wavefront
+ * tile 64x64 or 32x64, FP64 K-step prefetch, FP64 and pair-count grids), keeps the fastest
+ * for the workload's key (device, precision, P, block-size histogram) in the context and
+ * in the tune cache file (STEPPE_TUNE_CACHE, else ~/.cache/steppe_amd/f2_tune.tsv), which
+ * later f2 calls and graphs of the same key use.  Every candidate writes the same tensor;
+ * d_f2 / d_vpair hold it on return.  report (NUL-terminated JSON: key, chosen index,
+ * candidates with their milliseconds) is truncated to report_len; report_needed receives
+ * the full size.  Either may be NULL. */
+int stp_f2_autotune(stp_context* ctx, const uint16_t* d_a2, const uint16_t* d_n, int64_t ld, int32_t P, int64_t M,
+                    const int32_t* block_id, int32_t n_block, int32_t precision, double* d_f2, double* d_vpair,
+                    char* report, int32_t report_len, int32_t* report_needed);
+
 /* Per-block f2 from host FP64 Q/V/N [P x M] (reference: ComputeBackend::compute_f2_blocks,
  * src/device/backend.hpp:687).  Host outputs f2, vpair [P x P x n_block], block_sizes[n_block]. */
 int stp_compute_f2_blocks(stp_context* ctx, const double* Q, const double* V, const double* N,
@@ -523,6 +538,12 @@ int stp_f2bin_read_header(const char* path, int32_t* P, int32_t* n_block);
 int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_sizes);
 
 /* ---- instrumentation (no counterpart in the reference) ---------------------------- */
+
+/* The automatic tier choice of stp_extract_f2_tiered (host only, no device): the
+ * contraction's device working set for this partition and precision, and the tier it
+ * picks given free device and host memory in bytes. */
+int stpx_f2_tier_choice(int32_t P, int64_t M, const int32_t* block_id, int32_t n_block, int32_t precision,
+                        uint64_t free_vram, uint64_t free_host, uint64_t* working_bytes, int32_t* tier);
 
 /* Enable per-stage device timing: HIP events recorded on the context's own stream
  * around the decode, prep and f2 contraction launches. */

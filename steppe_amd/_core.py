@@ -156,6 +156,8 @@ def lib():
         "stp_decode_af": (C.c_int, [vp, C.POINTER(Tile), vp, vp, vp]),
         "stp_f2_blocks_from_counts": (C.c_int, [vp, vp, vp, i64, i32, i64, vp, i32, i32, vp, vp]),
         "stp_compute_f2_blocks": (C.c_int, [vp, vp, vp, vp, i32, i64, vp, i32, i32, vp, vp, vp]),
+        "stp_f2_autotune": (C.c_int, [vp, vp, vp, i64, i32, i64, vp, i32, i32, vp, vp, C.c_char_p, i32,
+                                      C.POINTER(i32)]),
         "stp_extract_f2": (C.c_int, [vp, C.POINTER(Tile), vp, i32, i32, C.POINTER(vp)]),
         "stp_extract_f2_tiered": (C.c_int, [vp, C.POINTER(Tile), vp, i32, i32, i32, C.c_char_p, i64, C.POINTER(vp)]),
         "stp_f2_graph_create": (C.c_int, [vp, C.POINTER(Tile), vp, i32, i32, vp, vp, i64, vp, vp, C.POINTER(vp)]),
@@ -181,6 +183,8 @@ def lib():
         "stp_f4ratio": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
         "stp_fstat_sweep": (C.c_int, [vp, vp, C.POINTER(SweepRequest), i64, vp, vp, vp, vp, vp, C.POINTER(i64),
                                       C.POINTER(C.c_uint64), C.POINTER(i32), C.POINTER(i32)]),
+        "stpx_f2_tier_choice": (C.c_int, [i32, i64, vp, i32, i32, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64),
+                                          C.POINTER(i32)]),
         "stpx_timing": (C.c_int, [vp, i32]),
         "stpx_timing_read": (C.c_int, [vp, vp, vp, i32]),
         "stpx_synth_panel": (C.c_int, [vp, C.c_uint64, i32, i64, i64, i64, vp, vp, vp, vp, vp]),
@@ -288,6 +292,19 @@ def tile_from_panel(panel, ploidy="sample"):
     if ploidy == "auto":
         return make_tile(panel.packed, panel.n_snp, panel.pop_offsets, None, detect=True)
     return make_tile(panel.packed, panel.n_snp, panel.pop_offsets, None, ploidy=int(ploidy))
+
+
+def f2_autotune(ctx: "Context", a2: int, n: int, ld: int, P: int, M: int, block_id, n_block: int, precision,
+                f2: int, vpair: int) -> dict:
+    """Time every launch configuration of the f2 contraction on these device counts
+    (stp_f2_autotune) and keep the fastest for the workload; returns the sweep table."""
+    bid = np.ascontiguousarray(block_id, dtype=np.int32)
+    need = C.c_int32(0)
+    buf = C.create_string_buffer(1 << 16)
+    check(lib().stp_f2_autotune(ctx.h, C.c_void_p(int(a2)), C.c_void_p(int(n)), int(ld), int(P), int(M), _ptr(bid),
+                                int(n_block), parse_precision(precision), C.c_void_p(int(f2)), C.c_void_p(int(vpair)),
+                                buf, len(buf), C.byref(need)), "f2 autotune")
+    return json.loads(buf.value.decode())
 
 
 class F2Graph:

@@ -23,7 +23,7 @@ LIB = os.path.join(LIBDIR, "libsteppe_hip.so")
 
 ARCH = os.environ.get("STEPPE_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["decode.hip", "f2_fp64.hip", "f2_split.hip", "f2_fixup.hip", "synth.hip", "qpadm.hip", "qpadm_large.hip", "qpgraph.hip", "fstats.hip",
-           "frontend.hip", "engine.cpp", "f2_graph.cpp", "qpadm_host.cpp", "qpgraph_host.cpp", "qpgraph_enum.cpp",
+           "frontend.hip", "engine.cpp", "f2_graph.cpp", "autotune.cpp", "qpadm_host.cpp", "qpgraph_host.cpp", "qpgraph_enum.cpp",
            "qpgraph_search.cpp", "fstats_host.cpp", "geno_io.cpp", "frontend_host.cpp", "tiered.cpp", "capi.cpp"]
 # Built without FP contraction: each product is rounded before it is added, as in the
 # reference's fit chain (plain double sums in a fixed order).

@@ -187,6 +187,22 @@ int stp_f2_blocks_from_counts(stp_context* ctx, const uint16_t* d_a2, const uint
     });
 }
 
+int stp_f2_autotune(stp_context* ctx, const uint16_t* d_a2, const uint16_t* d_n, int64_t ld, int32_t P, int64_t M,
+                    const int32_t* block_id, int32_t n_block, int32_t precision, double* d_f2, double* d_vpair,
+                    char* report, int32_t report_len, int32_t* report_needed) {
+    return guard([&] {
+        STP_REQUIRE(d_a2 && d_n && d_f2 && d_vpair, "stp_f2_autotune: null argument");
+        const std::string r =
+            backend(ctx).f2_autotune(d_a2, d_n, ld, P, M, block_id, n_block, precision, d_f2, d_vpair);
+        if (report_needed) *report_needed = static_cast<int32_t>(r.size() + 1);
+        if (report && report_len > 0) {
+            const std::size_t k = std::min<std::size_t>(r.size(), static_cast<std::size_t>(report_len - 1));
+            std::memcpy(report, r.data(), k);
+            report[k] = '\0';
+        }
+    });
+}
+
 int stp_compute_f2_blocks(stp_context* ctx, const double* Q, const double* V, const double* N, int32_t P,
                           int64_t M, const int32_t* block_id, int32_t n_block, int32_t precision, double* f2,
                           double* vpair, int32_t* block_sizes) {
@@ -873,6 +889,17 @@ int stp_f2bin_read(const char* path, double* f2, double* vpair, int32_t* block_s
             f.read(reinterpret_cast<char*>(block_sizes), static_cast<std::streamsize>(4 * h.n_block));
         }
         if (!f) throw FormatError(std::string("f2.bin: read failed: ") + path);
+    });
+}
+
+int stpx_f2_tier_choice(int32_t P, int64_t M, const int32_t* block_id, int32_t n_block, int32_t precision,
+                        uint64_t free_vram, uint64_t free_host, uint64_t* working_bytes, int32_t* tier) {
+    return guard([&] {
+        STP_REQUIRE(working_bytes && tier, "stpx_f2_tier_choice: null output");
+        const stp::F2Plan plan = stp::make_f2_plan(P, M, block_id, n_block);
+        const std::size_t result = 2 * static_cast<std::size_t>(P) * P * n_block * sizeof(double);
+        *working_bytes = stp::f2_working_set_bytes(plan, precision);
+        *tier = stp::choose_f2_tier(result, *working_bytes, free_vram, free_host);
     });
 }
 

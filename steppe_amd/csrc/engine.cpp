@@ -20,12 +20,13 @@
 namespace stp {
 
 // ---------------------------------------------------------------------------------------
-F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_block) {
+F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_block, const F2Tune& tune) {
     STP_REQUIRE(P > 0, "f2: P must be positive");
     STP_REQUIRE(M >= 0, "f2: M must be non-negative");
     STP_REQUIRE(n_block > 0, "f2: n_block must be positive");
     STP_REQUIRE(M == 0 || block_id != nullptr, "f2: block_id is required");
     F2Plan pl;
+    pl.tune = tune;
     pl.P = P;
     pl.M = M;
     pl.n_block = n_block;
@@ -71,7 +72,10 @@ F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_b
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pl.len[a] > pl.len[b]; });
     std::vector<std::vector<int>> die_blocks(kXcds);
-    {
+    if (tune.sched == 1) {
+        // one shared queue: every worker drains it (the claim falls through to it)
+        die_blocks[0] = order;
+    } else {
         std::vector<std::int64_t> load(kXcds, 0);
         for (int b : order) {
             const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
@@ -194,12 +198,17 @@ void HipBackend::upload_plan(const F2Plan& plan, F2Work::PlanDev& pd) {
     h2d(pd.items128, plan.items128, stream_);
 }
 
-const F2Plan& HipBackend::plan_for(int P, std::int64_t M, const std::int32_t* block_id, int n_block) {
+const F2Plan& HipBackend::plan_for(int P, std::int64_t M, const std::int32_t* block_id, int n_block, int precision) {
     const bool same = plan_valid_ && P == plan_.P && M == plan_.M && n_block == plan_.n_block &&
                       (M == 0 || std::memcmp(plan_bid_.data(), block_id, static_cast<std::size_t>(M) * 4) == 0);
-    if (same) return plan_;
+    if (same) {
+        const F2Tune t = tune_for(plan_, precision);
+        if (t.str() == plan_.tune.str()) return plan_;
+    }
     plan_valid_ = false;
     plan_ = make_f2_plan(P, M, block_id, n_block);
+    const F2Tune t = tune_for(plan_, precision);
+    if (t.str() != plan_.tune.str()) plan_ = make_f2_plan(P, M, block_id, n_block, t);
     plan_bid_.assign(block_id, block_id + M);
     upload_plan(plan_, work_.plan);
     sync();
@@ -353,10 +362,16 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
     const std::int64_t n_chunks = plan.K_total / kKPad;
     const BlockTable bt = w.plan.table();
     const int n_items = static_cast<int>(plan.items.size());
+    const F2Tune& tn = plan.tune;
     // Persistent grids are whole multiples of the die count so every die queue has
     // its own workers.
-    const int n_wg = static_cast<int>(
-        round_up(std::max(1, std::min<int>(2 * n_cu_, static_cast<int>(ceil_div(n_items, 4)))), kXcds));
+    auto grid = [&](int per_cu, std::int64_t units) {
+        return static_cast<int>(
+            round_up(std::max<std::int64_t>(1, std::min<std::int64_t>(static_cast<std::int64_t>(per_cu) * n_cu_,
+                                                                      ceil_div(units, 4))),
+                     kXcds));
+    };
+    const int n_wg_count = grid(tn.count_wg_per_cu, n_items);
     w.planeV8.ensure(std::max<std::size_t>(plane, 16));
     w.counters.ensure(2 * kXcds);
     if (precision == STP_PREC_FP64) {
@@ -372,10 +387,17 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
             launch_prep_fp64_qvn(src.q, src.v, src.nq, plan.P, plan.G, bt, w.plan.chunk_block.get(), n_chunks, KS,
                                  w.planeA.get(), w.planeV.get(), w.planeQ.get(), w.planeV8.get(), stream_);
         mark_end(kPrep);
+        mark_begin(kCount);
+        launch_count(w.planeV8.get(), bt, w.plan.items.get(), plan.q64, w.counters.get(), plan.P, plan.G,
+                     plan.K_total / 64, d_vpair, n_wg_count, stream_);
+        mark_end(kCount);
         const FixupList fl = fixup_list(plan, kFixupTauFp64, w);
+        const int units = n_items * (tn.fp64_rows == 32 ? 2 : 1);
         mark_begin(kF2);
-        launch_f2_fp64(w.planeA.get(), w.planeV.get(), w.planeQ.get(), w.planeV8.get(), bt, w.plan.items.get(),
-                       plan.q64, w.counters.get(), plan.P, plan.G, KS, d_f2, d_vpair, fl, n_wg, stream_);
+        launch_f2_fp64(w.planeA.get(), w.planeV.get(), w.planeQ.get(), bt, w.plan.items.get(), plan.q64,
+                       w.counters.get() + kXcds, plan.P, plan.G, KS, d_f2, d_vpair, fl,
+                       grid(tn.fp64_wg_per_cu, 4 * static_cast<std::int64_t>(units)), tn.fp64_rows,
+                       tn.fp64_prefetch != 0, stream_);
         mark_end(kF2);
         run_fixup(plan, src, fl, d_f2, d_vpair, w);
         return;
@@ -394,14 +416,15 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
     mark_end(kPrep);
     mark_begin(kCount);
     launch_count(w.planeV8.get(), bt, w.plan.items.get(), plan.q64, w.counters.get(), plan.P, plan.G,
-                 plan.K_total / 64, d_vpair, n_wg, stream_);
+                 plan.K_total / 64, d_vpair, n_wg_count, stream_);
     mark_end(kCount);
     const int n_items128 = static_cast<int>(plan.items128.size());
     const FixupList fl = fixup_list(plan, kFixupTauSplit, w);
     mark_begin(kF2);
     launch_f2_split(w.planeSplit.get(), bt, w.plan.items128.get(), plan.q128, w.counters.get() + kXcds, plan.P,
                     plan.G32, KS16, d_vpair, d_f2, fl,
-                    static_cast<int>(round_up(std::max(1, std::min(n_cu_, n_items128)), kXcds)), stream_);
+                    static_cast<int>(round_up(std::max(1, std::min(n_cu_, n_items128)), kXcds)), tn.split_il != 0,
+                    stream_);
     mark_end(kF2);
     run_fixup(plan, src, fl, d_f2, d_vpair, w);
 }
@@ -444,7 +467,7 @@ void HipBackend::f2_from_counts(const std::uint16_t* d_a2, const std::uint16_t* 
                                 double* d_f2, double* d_vpair) {
     activate();
     STP_REQUIRE(ld >= M, "f2: ld must be >= M");
-    const F2Plan& plan = plan_for(P, M, block_id, n_block);
+    const F2Plan& plan = plan_for(P, M, block_id, n_block, precision);
     F2Source src;
     src.a2 = d_a2;
     src.n = d_n;
@@ -457,7 +480,7 @@ void HipBackend::f2_from_qvn(const double* Q, const double* V, const double* N, 
                              const std::int32_t* block_id, int n_block, int precision, double* f2,
                              double* vpair, std::int32_t* block_sizes) {
     activate();
-    const F2Plan& plan = plan_for(P, M, block_id, n_block);
+    const F2Plan& plan = plan_for(P, M, block_id, n_block, precision);
     const std::size_t pm = static_cast<std::size_t>(P) * static_cast<std::size_t>(M);
     qvn_.ensure(std::max<std::size_t>(3 * pm, 1));
     if (pm) {

@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -43,9 +44,9 @@ void launch_prep_fp64_counts(const std::uint16_t* a2, const std::uint16_t* n, st
 void launch_prep_fp64_qvn(const double* q, const double* v, const double* n, int P, int G, BlockTable bt,
                           const std::int32_t* chunk_block, std::int64_t n_chunks, std::int64_t KS,
                           double* pA, double* pV, double* pQ, void* pV8, hipStream_t st);
-void launch_f2_fp64(const double* pA, const double* pV, const double* pQ, const void* pV8, BlockTable bt,
-                    const F2Item* items, const ItemQueues& queues, unsigned* counters, int P, int G,
-                    std::int64_t KS, double* f2, double* vpair, const FixupList& fl, int n_workgroups,
+void launch_f2_fp64(const double* pA, const double* pV, const double* pQ, BlockTable bt, const F2Item* items,
+                    const ItemQueues& queues, unsigned* counters, int P, int G, std::int64_t KS, double* f2,
+                    const double* vpair, const FixupList& fl, int n_workgroups, int wave_rows, bool prefetch,
                     hipStream_t st);
 void launch_prep_split_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P, int G32,
                               int G16, BlockTable bt, const std::int32_t* chunk_block, std::int64_t n_chunks,
@@ -58,7 +59,7 @@ void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const Ite
                   hipStream_t st);
 void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
                      unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
-                     const FixupList& fl, int n_workgroups, hipStream_t st);
+                     const FixupList& fl, int n_workgroups, bool interleave, hipStream_t st);
 void launch_f2_fixup_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, BlockTable bt,
                             const FixupList& fl, int P, int n_block, const double* vpair, double* f2,
                             int n_workgroups, hipStream_t st);
@@ -86,6 +87,20 @@ struct NoallocScratch {
 NoallocScratch resolve_noalloc_scratch();
 void launch_ploidy_from_i32(const std::int32_t* in, std::int64_t n, std::int8_t* out, hipStream_t st);
 
+// ---- launch configuration chosen by the autotuner (autotune.cpp) -----------------------
+struct F2Tune {
+    int sched = 0;            // work items: 0 per-die queues (blocks longest-first onto the
+                              // least-loaded die), 1 one shared queue (longest blocks first)
+    int fp64_rows = 64;       // FP64 wavefront tile: 64 x 64 or 32 x 64 outputs
+    int fp64_prefetch = 1;    // FP64: load the next K-step while this one's MFMAs run
+    int fp64_wg_per_cu = 2;   // FP64 contraction workgroups (4 wavefronts) per CU
+    int count_wg_per_cu = 2;  // pair-count kernel workgroups per CU
+    int split_il = 0;         // split contraction: loads issued between the MFMAs (1) or in a
+                              // burst after each phase barrier (0)
+    std::string str() const;
+    static bool parse(const std::string& s, F2Tune& out);
+};
+
 // ---- block plan ----------------------------------------------------------------------
 // Host description of one f2 launch: block ranges, padded K layout, LPT work items.
 struct F2Plan {
@@ -97,11 +112,12 @@ struct F2Plan {
     std::vector<F2Item> items128;  // 128-row tile pairs (split contraction)
     ItemQueues q64{}, q128{};      // per-die segments of items / items128
     int G32 = 0, T128 = 0;
+    F2Tune tune;                   // launch configuration (its sched built the queues)
 };
 
 // Validates block_id (contiguous, non-decreasing, in [0, n_block); the reference's
 // core::block_ranges throws on the same conditions, block_partition_rule.hpp:54-96).
-F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_block);
+F2Plan make_f2_plan(int P, std::int64_t M, const std::int32_t* block_id, int n_block, const F2Tune& tune = F2Tune{});
 
 // Device workspace of one f2 contraction: the block plan, the operand planes, the work
 // queue cursors and the fix-up list.  The backend keeps one for its eager calls; a
@@ -184,6 +200,12 @@ public:
     std::vector<std::int32_t> block_sizes;
     std::vector<std::string> labels;   // population labels when built from files
 };
+
+// Device bytes the contraction of `plan` needs besides its result (operand planes of the
+// precision, the int8 V plane, the fix-up list), and the automatic tier choice from the
+// result size, that working set and the free device / host memory (tiered.cpp).
+std::size_t f2_working_set_bytes(const F2Plan& plan, int precision);
+int choose_f2_tier(std::size_t result_bytes, std::size_t working_bytes, std::size_t free_vram, std::size_t free_host);
 
 // Tier of an f2 computation: -1 chooses from free memory as the reference's
 // select_output_tier does (src/device/tier_select.hpp:57-80: resident when the result
@@ -336,6 +358,14 @@ public:
     F2Graph* f2_graph_create(const stp_tile& t, const std::int32_t* block_id, int n_block, int precision,
                              std::uint16_t* a2, std::uint16_t* n, std::int64_t ld, double* f2, double* vpair);
     void f2_graph_launch(F2Graph& g);
+    // Autotuner (autotune.cpp): times every candidate launch configuration of the f2
+    // contraction on these device inputs and outputs, keeps the fastest for the workload's
+    // key (device, precision, P, block-size histogram) and writes the table to the tune
+    // cache file.  Returns the sweep as JSON.
+    std::string f2_autotune(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P, std::int64_t M,
+                            const std::int32_t* block_id, int n_block, int precision, double* f2, double* vpair);
+    // The configuration for a plan's key: the tuned one, else the defaults.
+    F2Tune tune_for(const F2Plan& plan, int precision);
     // Genotype files -> resident f2 (frontend_host.cpp): read the packed triple, build the
     // individual-major tile in HBM, decode, keep / compact SNPs, block, contract.
     F2Blocks* extract_f2_files(const ExtractRequest& req, ExtractInfo& info);
@@ -406,10 +436,16 @@ private:
     void upload_plan(const F2Plan& plan, F2Work::PlanDev& pd);
     // The plan for (P, M, block_id, n_block), rebuilt and re-uploaded only when the
     // partition differs from the previous call's (compared element by element).
-    const F2Plan& plan_for(int P, std::int64_t M, const std::int32_t* block_id, int n_block);
+    const F2Plan& plan_for(int P, std::int64_t M, const std::int32_t* block_id, int n_block, int precision);
     F2Plan plan_;
     std::vector<std::int32_t> plan_bid_;
     bool plan_valid_ = false;
+    // Tuned configurations by key, loaded from / saved to the tune cache file.
+    std::string tune_key(const F2Plan& plan, int precision) const;
+    void tune_load();
+    void tune_save() const;
+    std::map<std::string, F2Tune> tuned_;
+    bool tune_loaded_ = false;
     void prepare_tile(const stp_tile& t);
     // The launches of one graph replay, on this backend's stream (captured or eager).
     void record_f2_graph(F2Graph& g);

@@ -17,8 +17,8 @@
 // i.e. three rank-K products accumulated into ONE FP64 accumulator (left operand
 // rows [A, V, -2q'], right rows [V, A, q']), so the cancellation between the
 // squared and cross terms happens inside the FP64 sum of centred values instead of
-// between three separately rounded GEMM outputs.  The pair count is an exact int8
-// MFMA product V V^T (int32 accumulation).
+// between three separately rounded GEMM outputs.  The pair count is the exact int8
+// MFMA product V V^T of the count kernel (f2_split.hip), run before the contraction.
 
 #include "common.hpp"
 #include "f2_layout.hpp"
@@ -151,104 +151,124 @@ __global__ __launch_bounds__(256) void prep_fp64_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// contraction: persistent wavefronts pull (block, 64x64 tile pair) items from a
-// device counter (items are ordered block-major, largest blocks first) and
-// accumulate the 4x4 grid of 16x16 sub-tiles over the block's K range.
+// contraction: persistent wavefronts pull work units from per-die queues (items are
+// (block, 64x64 tile pair), ordered largest blocks first; a unit is an item, or half
+// of one for 32x64 wavefront tiles) and accumulate TI x 4 sub-tiles of 16 x 16 over the
+// block's K range: per 4-SNP step three v_mfma_f64_16x16x4 per sub-tile,
+// A_i V_j + V_i A_j - 2 q'_i q'_j into one accumulator.  With PF the next step's
+// operands are loaded before this step's MFMAs (one step of software pipelining).
+// vpair comes from the pair-count kernel, which runs first.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void f2_fp64_kernel(
-    const double* __restrict__ pA, const double* __restrict__ pV, const double* __restrict__ pQ,
-    const i4* __restrict__ pV8, BlockTable bt, const F2Item* __restrict__ items, ItemQueues queues,
-    unsigned* __restrict__ counters, int P, int G, std::int64_t KS, double* __restrict__ f2,
-    double* __restrict__ vpair, FixupList fl) {
+template <int TI, bool PF>
+__global__ __launch_bounds__(256, TI == 4 ? 2 : 3) void f2_fp64_kernel(
+    const double* __restrict__ pA, const double* __restrict__ pV, const double* __restrict__ pQ, BlockTable bt,
+    const F2Item* __restrict__ items, ItemQueues queues, unsigned* __restrict__ counters, int P, int G,
+    std::int64_t KS, double* __restrict__ f2, const double* __restrict__ vpair, FixupList fl) {
+    constexpr int kHalves = 4 / TI;
     const int lane = threadIdx.x & 63;
-    // v_mfma_f64_16x16x4f64 returns D[(l/16) + 4e][l%16] in lane l, item e, while the
-    // int8 count MFMA returns D[4(l/16) + e][l%16].  Feeding the left operand's rows
-    // in the order pi(i) = 4(i%4) + i/4 makes the FP64 accumulator land in the same
-    // lanes/items as the count: lane l of the left fragment reads plane lane lane_a.
+    // v_mfma_f64_16x16x4f64 returns D[(l/16) + 4e][l%16] in lane l, item e.  Feeding the
+    // left operand's rows in the order pi(i) = 4(i%4) + i/4 puts D[i][j] in lane
+    // j + 16*(i/4), item i%4: lane l of the left fragment reads plane lane lane_a.
     const int lane_a = (lane & 48) + 4 * (lane & 3) + ((lane >> 2) & 3);
-    const std::int64_t KS64 = KS / 16;
     const std::int64_t slab = static_cast<std::int64_t>(P) * P;
 
     const int home = static_cast<int>(blockIdx.x % kXcds);
     int xq = 0;
     for (;;) {
-        int it = 0;
-        if (lane == 0) it = claim_item(counters, queues, home, xq);
+        int un = 0;
+        if (lane == 0) un = claim_item(counters, queues, home, xq);
         // Uniform values in scalar registers keep every branch below uniform.
-        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-        if (it < 0) break;
-        const F2Item w = items[it];
+        un = __builtin_amdgcn_readfirstlane(__shfl(un, 0));
+        if (un < 0) break;
+        const F2Item w = items[un / kHalves];
+        const int half = un - (un / kHalves) * kHalves;
         const int b = __builtin_amdgcn_readfirstlane(w.b);
-        const int gI0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(w.ti));
+        const int gI0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(w.ti)) + TI * half;
         const int gJ0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(w.tj));
         const std::int64_t t_beg = bt.k0[b] / 4;
         const int nsteps = bt.kpad[b] / 4;
 
-        bool rowI[4], rowJ[4];
+        bool rowI[TI], rowJ[4];
 #pragma unroll
-        for (int x = 0; x < 4; ++x) { rowI[x] = gI0 + x < G; rowJ[x] = gJ0 + x < G; }
+        for (int x = 0; x < TI; ++x) rowI[x] = gI0 + x < G;
+#pragma unroll
+        for (int y = 0; y < 4; ++y) rowJ[y] = gJ0 + y < G;
 
-        d4 acc[4][4];
-        i4 cnt[4][4];
+        d4 acc[TI][4];
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
+        for (int x = 0; x < TI; ++x)
 #pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
-                cnt[x][y] = i4{0, 0, 0, 0};
-            }
+            for (int y = 0; y < 4; ++y) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
 
-        for (int t = 0; t < nsteps; ++t) {
-            double aI[4], vI[4], qI[4], aJ[4], vJ[4], qJ[4];
+        double aI[TI], vI[TI], qI[TI], aJ[4], vJ[4], qJ[4];
+        auto load = [&](int t, double (&a_i)[TI], double (&v_i)[TI], double (&q_i)[TI], double (&a_j)[4],
+                        double (&v_j)[4], double (&q_j)[4]) {
 #pragma unroll
-            for (int x = 0; x < 4; ++x) {
-                aI[x] = vI[x] = qI[x] = aJ[x] = vJ[x] = qJ[x] = 0.0;
+            for (int x = 0; x < TI; ++x) {
+                a_i[x] = v_i[x] = q_i[x] = 0.0;
                 if (rowI[x]) {
                     const std::int64_t o = ((gI0 + x) * KS + t_beg + t) * 64 + lane_a;
-                    aI[x] = pA[o];
-                    vI[x] = pV[o];
-                    qI[x] = -2.0 * pQ[o];
-                }
-                if (rowJ[x]) {
-                    const std::int64_t o = ((gJ0 + x) * KS + t_beg + t) * 64 + lane;
-                    aJ[x] = pA[o];
-                    vJ[x] = pV[o];
-                    qJ[x] = pQ[o];
+                    a_i[x] = pA[o];
+                    v_i[x] = pV[o];
+                    q_i[x] = -2.0 * pQ[o];
                 }
             }
 #pragma unroll
-            for (int x = 0; x < 4; ++x)
+            for (int y = 0; y < 4; ++y) {
+                a_j[y] = v_j[y] = q_j[y] = 0.0;
+                if (rowJ[y]) {
+                    const std::int64_t o = ((gJ0 + y) * KS + t_beg + t) * 64 + lane;
+                    a_j[y] = pA[o];
+                    v_j[y] = pV[o];
+                    q_j[y] = pQ[o];
+                }
+            }
+        };
+        auto mfma = [&](const double (&a_i)[TI], const double (&v_i)[TI], const double (&q_i)[TI],
+                        const double (&a_j)[4], const double (&v_j)[4], const double (&q_j)[4]) {
+#pragma unroll
+            for (int x = 0; x < TI; ++x)
 #pragma unroll
                 for (int y = 0; y < 4; ++y) {
                     if (!(rowI[x] && rowJ[y] && gI0 + x <= gJ0 + y)) continue;
-                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(aI[x], vJ[y], acc[x][y], 0, 0, 0);
-                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(vI[x], aJ[y], acc[x][y], 0, 0, 0);
-                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(qI[x], qJ[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[x], v_j[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(v_i[x], a_j[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(q_i[x], q_j[y], acc[x][y], 0, 0, 0);
                 }
-            if ((t & 15) == 15) {
-                const std::int64_t u = (t_beg + t) / 16;
-                i4 cI[4], cJ[4];
+        };
+        if constexpr (PF) {
+            load(0, aI, vI, qI, aJ, vJ, qJ);
+            for (int t = 0; t < nsteps; ++t) {
+                double aI2[TI], vI2[TI], qI2[TI], aJ2[4], vJ2[4], qJ2[4];
+                // the last step re-reads its own operands (in bounds; never used)
+                load(t + 1 < nsteps ? t + 1 : t, aI2, vI2, qI2, aJ2, vJ2, qJ2);
+                mfma(aI, vI, qI, aJ, vJ, qJ);
 #pragma unroll
-                for (int x = 0; x < 4; ++x) {
-                    cI[x] = rowI[x] ? pV8[((gI0 + x) * KS64 + u) * 64 + lane] : i4{0, 0, 0, 0};
-                    cJ[x] = rowJ[x] ? pV8[((gJ0 + x) * KS64 + u) * 64 + lane] : i4{0, 0, 0, 0};
+                for (int x = 0; x < TI; ++x) {
+                    aI[x] = aI2[x];
+                    vI[x] = vI2[x];
+                    qI[x] = qI2[x];
                 }
 #pragma unroll
-                for (int x = 0; x < 4; ++x)
-#pragma unroll
-                    for (int y = 0; y < 4; ++y) {
-                        if (!(rowI[x] && rowJ[y] && gI0 + x <= gJ0 + y)) continue;
-                        cnt[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cI[x], cJ[y], cnt[x][y], 0, 0, 0);
-                    }
+                for (int y = 0; y < 4; ++y) {
+                    aJ[y] = aJ2[y];
+                    vJ[y] = vJ2[y];
+                    qJ[y] = qJ2[y];
+                }
+            }
+        } else {
+            for (int t = 0; t < nsteps; ++t) {
+                load(t, aI, vI, qI, aJ, vJ, qJ);
+                mfma(aI, vI, qI, aJ, vJ, qJ);
             }
         }
 
-        // Epilogue: with the row permutation above, D[i][j] of both accumulators sits
-        // in lane j + 16*(i/4), item i%4.
+        // Epilogue: with the row permutation above, D[i][j] sits in lane j + 16*(i/4),
+        // item i%4.
         double* f2b = f2 + slab * b;
-        double* vpb = vpair + slab * b;
+        const double* vpb = vpair + slab * b;
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
+        for (int x = 0; x < TI; ++x)
 #pragma unroll
             for (int y = 0; y < 4; ++y) {
                 const int gI = gI0 + x, gJ = gJ0 + y;
@@ -258,13 +278,12 @@ __global__ __launch_bounds__(256) void f2_fp64_kernel(
                 for (int e = 0; e < 4; ++e) {
                     const int row = kRowGroup * gI + 4 * (lane >> 4) + e;
                     const bool ok = row < P && col < P && !(gI == gJ && row > col);
-                    const double vp = static_cast<double>(cnt[x][y][e]);
-                    const double val = vp > 0.0 ? acc[x][y][e] / vp : 0.0;
+                    double vp = 0.0, val = 0.0;
                     if (ok) {
+                        vp = vpb[row + static_cast<std::int64_t>(P) * col];
+                        val = vp > 0.0 ? acc[x][y][e] / vp : 0.0;
                         f2b[row + static_cast<std::int64_t>(P) * col] = val;
                         f2b[col + static_cast<std::int64_t>(P) * row] = val;
-                        vpb[row + static_cast<std::int64_t>(P) * col] = vp;
-                        vpb[col + static_cast<std::int64_t>(P) * row] = vp;
                     }
                     fixup_append(fl, ok && vp > 0.0 && fabs(val) < fl.tau, b, row, col);
                 }
@@ -302,14 +321,25 @@ void launch_prep_fp64_qvn(const double* q, const double* v, const double* n, int
     STP_LAUNCH_CHECK();
 }
 
-void launch_f2_fp64(const double* pA, const double* pV, const double* pQ, const void* pV8, BlockTable bt,
-                    const F2Item* items, const ItemQueues& queues, unsigned* counters, int P, int G,
-                    std::int64_t KS, double* f2, double* vpair, const FixupList& fl, int n_workgroups,
+void launch_f2_fp64(const double* pA, const double* pV, const double* pQ, BlockTable bt, const F2Item* items,
+                    const ItemQueues& queues, unsigned* counters, int P, int G, std::int64_t KS, double* f2,
+                    const double* vpair, const FixupList& fl, int n_workgroups, int wave_rows, bool prefetch,
                     hipStream_t st) {
     if (queues.off[kXcds] <= 0) return;
     launch_zero_u32(counters, kXcds, st);
-    f2_fp64_kernel<<<static_cast<unsigned>(n_workgroups), 256, 0, st>>>(
-        pA, pV, pQ, static_cast<const i4*>(pV8), bt, items, queues, counters, P, G, KS, f2, vpair, fl);
+    // work units: whole items (64-row wavefront tiles) or item halves (32-row tiles)
+    ItemQueues qu = queues;
+    const int halves = wave_rows == 32 ? 2 : 1;
+    for (int x = 0; x <= kXcds; ++x) qu.off[x] = queues.off[x] * halves;
+    const dim3 grid(static_cast<unsigned>(n_workgroups));
+    auto go = [&](auto kernel) { kernel<<<grid, 256, 0, st>>>(pA, pV, pQ, bt, items, qu, counters, P, G, KS, f2, vpair, fl); };
+    if (wave_rows == 32) {
+        if (prefetch) go(f2_fp64_kernel<2, true>);
+        else go(f2_fp64_kernel<2, false>);
+    } else {
+        if (prefetch) go(f2_fp64_kernel<4, true>);
+        else go(f2_fp64_kernel<4, false>);
+    }
     STP_LAUNCH_CHECK();
 }
 

@@ -71,6 +71,8 @@ F2Graph* HipBackend::f2_graph_create(const stp_tile& t, const std::int32_t* bloc
     g->f2 = f2;
     g->vpair = vpair;
     g->plan = make_f2_plan(t.n_pop, M, block_id, n_block);
+    const F2Tune tn = tune_for(g->plan, precision);
+    if (tn.str() != g->plan.tune.str()) g->plan = make_f2_plan(t.n_pop, M, block_id, n_block, tn);
     upload_plan(g->plan, g->work.plan);
     if (a2) {
         STP_REQUIRE(ld >= M && ld % 16 == 0, "f2 graph: ld must be >= n_snp and a multiple of 16");
@@ -164,8 +166,7 @@ void HipBackend::f2_graph_launch(F2Graph& g) {
     STP_REQUIRE(g.device == device_, "f2 graph: launched on another device's context");
     STP_HIP(hipGraphLaunch(g.exec, stream_));
     if (g.timed && timing_) {
-        pending_[kDecode] = pending_[kPrep] = pending_[kF2] = pending_[kFixup] = true;
-        if (g.precision == STP_PREC_SPLIT_BF16) pending_[kCount] = true;
+        pending_[kDecode] = pending_[kPrep] = pending_[kCount] = pending_[kF2] = pending_[kFixup] = true;
     }
     sync();
 }

@@ -40,6 +40,8 @@
 // global->LDS loads into a three-slot ring, so two phases of loads are in flight
 // while one is computed.
 
+#include <utility>
+
 #include "common.hpp"
 #include "f2_layout.hpp"
 
@@ -62,6 +64,14 @@ constexpr int kDSlotBytes = kGroups * kDPlanes * kChunk;    // 48 KB
 constexpr int kE1SlotBytes = kGroups * kE1Planes * kChunk;  // 24 KB
 constexpr int kE2SlotBytes = kGroups * kE2Planes * kChunk;  // 32 KB
 constexpr int kWaves = 8;
+
+// f(std::integral_constant<int, I>) for I = 0 .. N-1, as straight-line code.
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+        (f(std::integral_constant<int, I>{}), ...);
+    }(std::make_integer_sequence<int, N>{});
+}
 
 // ---------------------------------------------------------------------------------
 // prep: frequencies -> 13 BF16 digit planes (32x32x16 MFMA fragment order) + the
@@ -415,6 +425,25 @@ __device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std:
     }
 }
 
+// Load r of this wavefront's share of one phase (the r-th iteration of stage_phase).
+template <int kind, int r>
+__device__ inline void stage_one(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, std::int64_t t16,
+                                 std::int64_t KS16, int gI0, int gJ0, int G32, int wave, int lane) {
+    constexpr int np = phase_planes<kind>();
+    static_assert(r < np, "one load per plane of the phase");
+    const int ci = wave + kWaves * r;
+    const int lg = ci / np, pp = ci - lg * np;
+    int gg = lg < 4 ? gI0 + lg : gJ0 + lg - 4;
+    if (gg >= G32) gg = 0;  // rows beyond P: any valid address; never read back
+    const std::uint8_t* src =
+        planes + ((static_cast<std::int64_t>(gg) * KS16 + t16) * kPlanes + phase_plane(kind, pp)) * kChunk + lane * 16;
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
+                                     (__attribute__((address_space(3))) void*)(slot + ci * kChunk), 16, 0, 0);
+}
+
+// IL: issue each phase's loads one at a time between its MFMAs instead of as one burst
+// after the phase barrier (same loads, same waits).
+template <bool IL>
 __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __restrict__ planes, BlockTable bt,
                                                        const F2Item* __restrict__ items, ItemQueues queues,
                                                        unsigned* __restrict__ counters, int P, int G32,
@@ -489,16 +518,17 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
             wait_vmcnt<9>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            stage_phase<2>(planes, lds_e2, kstep(step), KS16, gI0, gJ0, G32, wave, lane);
+            const std::int64_t t_e2 = kstep(step);
+            if constexpr (!IL) stage_phase<2>(planes, lds_e2, t_e2, KS16, gI0, gJ0, G32, wave, lane);
             {
                 bf8 dj[6];
 #pragma unroll
                 for (int k = 0; k < 6; ++k) dj[k] = lds_frag(slot_d, 4 + wc, k, kDPlanes, lane);
+                static_for<6>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
 #pragma unroll
-                for (int x = 0; x < 2; ++x) {
-                    if (!act[x]) continue;
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) {
+                    for (int x = 0; x < 2; ++x) {
+                        if (!act[x]) continue;
                         const bf8 di = lds_frag(slot_d, 2 * wa + x, k, kDPlanes, lane);
 #pragma unroll
                         for (int t = k; t < 5; ++t)
@@ -507,46 +537,67 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                         const bf8 d5l = (k == 0 || k == 5) ? di : scale_2m8(di);
                         L[x][4] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(d5l, dj[5 - k], L[x][4], 0, 0, 0);
                     }
-                }
+                    if constexpr (IL && k < kE2Planes)
+                        stage_one<2, k>(planes, lds_e2, t_e2, KS16, gI0, gJ0, G32, wave, lane);
+                });
             }
             // ---- e0..e2 (V from the e3..e5 slot)
             wait_vmcnt<0>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            stage_phase<0>(planes, slot_d, kstep(step + 2), KS16, gI0, gJ0, G32, wave, lane);
+            const std::int64_t t_d = kstep(step + 2);
+            if constexpr (!IL) stage_phase<0>(planes, slot_d, t_d, KS16, gI0, gJ0, G32, wave, lane);
             {
                 bf8 ej[3];
 #pragma unroll
                 for (int m = 0; m < 3; ++m) ej[m] = lds_frag(lds_e1, 4 + wc, m, kE1Planes, lane);
                 const bf8 vj = lds_frag(lds_e2, 4 + wc, 3, kE2Planes, lane);
+                static_for<2>([&](auto xc) {
+                    constexpr int x = decltype(xc)::value;
+                    static_for<3>([&](auto mc) {
+                        constexpr int m = decltype(mc)::value;
+                        if (act[x]) {
+                            const bf8 ei = lds_frag(lds_e1, 2 * wa + x, m, kE1Planes, lane);
+                            L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][m], 0, 0, 0);
+                        }
+                        if constexpr (IL) stage_one<0, 3 * x + m>(planes, slot_d, t_d, KS16, gI0, gJ0, G32, wave, lane);
+                    });
+                    if (act[x]) {
+                        const bf8 vi = lds_frag(lds_e2, 2 * wa + x, 3, kE2Planes, lane);
 #pragma unroll
-                for (int x = 0; x < 2; ++x) {
-                    if (!act[x]) continue;
-#pragma unroll
-                    for (int m = 0; m < 3; ++m) {
-                        const bf8 ei = lds_frag(lds_e1, 2 * wa + x, m, kE1Planes, lane);
-                        L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][m], 0, 0, 0);
+                        for (int m = 0; m < 3; ++m)
+                            L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vi, ej[m], L[x][m], 0, 0, 0);
                     }
-                    const bf8 vi = lds_frag(lds_e2, 2 * wa + x, 3, kE2Planes, lane);
-#pragma unroll
-                    for (int m = 0; m < 3; ++m)
-                        L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vi, ej[m], L[x][m], 0, 0, 0);
-                }
+                });
             }
             // ---- e3..e5 + V (e5 pre-scaled into level 4)
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            stage_phase<1>(planes, lds_e1, kstep(step + 1), KS16, gI0, gJ0, G32, wave, lane);
+            const std::int64_t t_e1 = kstep(step + 1);
+            if constexpr (!IL) stage_phase<1>(planes, lds_e1, t_e1, KS16, gI0, gJ0, G32, wave, lane);
             {
                 bf8 ej[3];
 #pragma unroll
                 for (int m = 0; m < 3; ++m) ej[m] = lds_frag(lds_e2, 4 + wc, m, kE2Planes, lane);
                 const bf8 vj = lds_frag(lds_e2, 4 + wc, 3, kE2Planes, lane);
+                if constexpr (IL) {
+                    // the three loads of E1(s+1) go between the first tile's MFMAs
+                    static_for<3>([&](auto mc) {
+                        constexpr int m = decltype(mc)::value;
+                        if (act[0]) {
+                            const bf8 ei = lds_frag(lds_e2, 2 * wa, m, kE2Planes, lane);
+                            const int t = m == 2 ? 4 : 3 + m;
+                            L[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[0][t], 0, 0, 0);
+                        }
+                        stage_one<1, m>(planes, lds_e1, t_e1, KS16, gI0, gJ0, G32, wave, lane);
+                    });
+                }
 #pragma unroll
                 for (int x = 0; x < 2; ++x) {
                     if (!act[x]) continue;
 #pragma unroll
                     for (int m = 0; m < 3; ++m) {
+                        if (IL && x == 0) break;   // issued above, between the loads
                         const bf8 ei = lds_frag(lds_e2, 2 * wa + x, m, kE2Planes, lane);
                         const int t = m == 2 ? 4 : 3 + m;
                         L[x][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][t], 0, 0, 0);
@@ -662,11 +713,15 @@ void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const Ite
 
 void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
                      unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
-                     const FixupList& fl, int n_workgroups, hipStream_t st) {
+                     const FixupList& fl, int n_workgroups, bool interleave, hipStream_t st) {
     if (queues.off[kXcds] <= 0) return;
     launch_zero_u32(counters, kXcds, st);
-    f2_split_kernel<<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues, counters, P,
-                                                                        G32, KS16, vpair, f2, fl);
+    if (interleave)
+        f2_split_kernel<true><<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues, counters,
+                                                                                  P, G32, KS16, vpair, f2, fl);
+    else
+        f2_split_kernel<false><<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues,
+                                                                                   counters, P, G32, KS16, vpair, f2, fl);
     STP_LAUNCH_CHECK();
 }
 

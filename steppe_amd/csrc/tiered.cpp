@@ -123,6 +123,25 @@ void HostArray::alloc(std::size_t n) {
     n_ = n;
 }
 
+std::size_t f2_working_set_bytes(const F2Plan& plan, int precision) {
+    const std::size_t K = static_cast<std::size_t>(plan.K_total);
+    const std::size_t rows16 = static_cast<std::size_t>(plan.G) * kRowGroup;
+    const std::size_t v8 = rows16 * K;   // int8 V plane
+    const std::size_t planes = precision == STP_PREC_FP64 ? 3 * sizeof(double) * rows16 * K
+                                                          : 26 * static_cast<std::size_t>(plan.G32) * 32 * K;
+    const std::size_t upper = static_cast<std::size_t>(plan.P) * (plan.P + 1) / 2 * plan.n_block;
+    const std::size_t fix = 12 * std::min<std::size_t>(std::max<std::size_t>(upper / 16, 1 << 16), 1 << 24);
+    return planes + v8 + fix;
+}
+
+int choose_f2_tier(std::size_t result_bytes, std::size_t working_bytes, std::size_t free_vram,
+                   std::size_t free_host) {
+    if (result_bytes + working_bytes <= static_cast<std::size_t>(0.70 * static_cast<double>(free_vram)))
+        return kTierResident;
+    if (result_bytes <= static_cast<std::size_t>(0.60 * static_cast<double>(free_host))) return kTierHost;
+    return kTierDisk;
+}
+
 F2Blocks* HipBackend::f2_counts_tiered(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, int P,
                                        std::int64_t M, const std::int32_t* block_id, int n_block, int precision,
                                        const TierRequest& tr) {
@@ -131,24 +150,24 @@ F2Blocks* HipBackend::f2_counts_tiered(const std::uint16_t* a2, const std::uint1
     const std::size_t result_bytes = 2 * slab * static_cast<std::size_t>(n_block) * sizeof(double);
     std::size_t free_vram = 0, total_vram = 0;
     STP_HIP(hipMemGetInfo(&free_vram, &total_vram));
+    const F2Plan full = make_f2_plan(P, M, block_id, n_block);   // validates the partition
     int tier = tr.tier;
     if (tier < 0) {
-        if (result_bytes <= static_cast<std::size_t>(0.70 * static_cast<double>(free_vram))) tier = kTierResident;
-        else if (result_bytes <= static_cast<std::size_t>(0.60 * static_cast<double>(free_host_bytes())))
-            tier = kTierHost;
-        else tier = kTierDisk;
+        // resident only when the result and the contraction's working set (planes, int8 V
+        // plane, fix-up list) fit together, as the reference's select_output_tier adds
+        // its resident working set (src/device/tier_select.hpp:62-72)
+        tier = choose_f2_tier(result_bytes, f2_working_set_bytes(full, precision), free_vram, free_host_bytes());
     }
     STP_REQUIRE(tier != kTierDisk || !tr.disk_path.empty(),
                 "f2: the result exceeds device and host memory; a disk path for the f2.bin is required");
 
-    const F2Plan full = make_f2_plan(P, M, block_id, n_block);   // validates the partition
     auto out = std::make_unique<F2Blocks>();
     out->P = P;
     out->n_block = n_block;
     out->tier = tier;
     out->block_sizes = full.len;
     if (tier == kTierResident) {
-        const F2Plan& plan = plan_for(P, M, block_id, n_block);
+        const F2Plan& plan = plan_for(P, M, block_id, n_block, precision);
         out->f2.alloc(slab * n_block);
         out->vpair.alloc(slab * n_block);
         F2Source src;
@@ -239,7 +258,7 @@ F2Blocks* HipBackend::f2_counts_tiered(const std::uint16_t* a2, const std::uint1
         if (s_hi > s_lo) {
             sub.resize(static_cast<std::size_t>(s_hi - s_lo));
             for (std::int64_t s = s_lo; s < s_hi; ++s) sub[s - s_lo] = block_id[s] - b0;
-            const F2Plan& plan = plan_for(P, s_hi - s_lo, sub.data(), b1 - b0);
+            const F2Plan& plan = plan_for(P, s_hi - s_lo, sub.data(), b1 - b0, precision);
             F2Source src;
             src.a2 = a2 + s_lo;
             src.n = n + s_lo;

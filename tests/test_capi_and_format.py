@@ -139,3 +139,32 @@ def test_ctypes_structs_match_the_header(tmp_path):
         cls = getattr(_core, pyname)
         got = C.sizeof(cls) if field == "size" else getattr(cls, field).offset
         assert got == int(val), (pyname, field, got, val)
+
+
+def _tier_choice(P, M, n_block, precision, free_vram, free_host):
+    L = _core.lib()
+    bid = np.ascontiguousarray(np.repeat(np.arange(n_block, dtype=np.int32), M // n_block))
+    wb, tier = C.c_uint64(0), C.c_int32(-9)
+    _core.check(L.stpx_f2_tier_choice(P, len(bid), _core._ptr(bid), n_block, precision, free_vram, free_host,
+                                      C.byref(wb), C.byref(tier)))
+    return wb.value, tier.value
+
+
+def test_tier_choice_counts_the_working_set():
+    # P = 4000, M = 1.2M SNPs in 700 blocks, split form: 179 GB of f2 + vpair and ~131 GB of
+    # operand planes.  With 288 GB free the result alone fits 70% of HBM (201 GB) but result +
+    # planes do not: the host tier is chosen (the reference adds its working set the same way,
+    # src/device/tier_select.hpp:62-72).
+    P, M, nb = 4000, 1_200_000, 700
+    result = 2 * P * P * nb * 8
+    wb, tier = _tier_choice(P, M, nb, _core.STP_PREC_SPLIT_BF16, 288 << 30, 2 << 40)
+    planes = 26 * ((P + 31) // 32 * 32) * (M // nb + 63) // 64 * 64 * nb
+    assert wb >= planes and result <= 0.70 * (288 << 30) < result + wb
+    assert tier == 1
+    # the same shape with twice the device memory stays resident
+    assert _tier_choice(P, M, nb, _core.STP_PREC_SPLIT_BF16, 576 << 30, 2 << 40)[1] == 0
+    # no room on the host either: the disk tier
+    assert _tier_choice(P, M, nb, _core.STP_PREC_SPLIT_BF16, 288 << 30, 64 << 30)[1] == 2
+    # the FP64 planes (24 B per population and SNP + the int8 V plane) are counted too
+    wb64, _ = _tier_choice(P, M, nb, _core.STP_PREC_FP64, 288 << 30, 2 << 40)
+    assert wb64 >= 24 * ((P + 15) // 16 * 16) * (M // nb + 63) // 64 * 64 * nb
