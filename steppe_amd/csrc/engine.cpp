@@ -404,7 +404,7 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
     }
     // Split-BF16: 13 digit planes of 2 bytes per (row, SNP), rows padded to 32.
     const std::int64_t KS16 = plan.K_total / 16;
-    const std::size_t split_bytes = static_cast<std::size_t>(plan.G32) * 32 * plan.K_total * 26;
+    const std::size_t split_bytes = static_cast<std::size_t>(plan.G32) * 32 * plan.K_total * 2 * kSplitPlanes;
     w.planeSplit.ensure(std::max<std::size_t>(split_bytes, 16));
     mark_begin(kPrep);
     if (src.a2)
@@ -423,8 +423,7 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
     mark_begin(kF2);
     launch_f2_split(w.planeSplit.get(), bt, w.plan.items128.get(), plan.q128, w.counters.get() + kXcds, plan.P,
                     plan.G32, KS16, d_vpair, d_f2, fl,
-                    static_cast<int>(round_up(std::max(1, std::min(n_cu_, n_items128)), kXcds)), tn.split_il != 0,
-                    stream_);
+                    static_cast<int>(round_up(std::max(1, std::min(n_cu_, n_items128)), kXcds)), stream_);
     mark_end(kF2);
     run_fixup(plan, src, fl, d_f2, d_vpair, w);
 }

@@ -59,7 +59,7 @@ void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const Ite
                   hipStream_t st);
 void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
                      unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
-                     const FixupList& fl, int n_workgroups, bool interleave, hipStream_t st);
+                     const FixupList& fl, int n_workgroups, hipStream_t st);
 void launch_f2_fixup_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, BlockTable bt,
                             const FixupList& fl, int P, int n_block, const double* vpair, double* f2,
                             int n_workgroups, hipStream_t st);
@@ -95,8 +95,6 @@ struct F2Tune {
     int fp64_prefetch = 1;    // FP64: load the next K-step while this one's MFMAs run
     int fp64_wg_per_cu = 2;   // FP64 contraction workgroups (4 wavefronts) per CU
     int count_wg_per_cu = 2;  // pair-count kernel workgroups per CU
-    int split_il = 0;         // split contraction: loads issued between the MFMAs (1) or in a
-                              // burst after each phase barrier (0)
     std::string str() const;
     static bool parse(const std::string& s, F2Tune& out);
 };

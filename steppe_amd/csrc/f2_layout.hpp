@@ -28,6 +28,8 @@ namespace stp {
 constexpr int kRowGroup = 16;
 constexpr int kKPad = 64;   // padded block granularity (SNPs)
 constexpr int kTileRows = 64;  // rows per wavefront output tile (4 row groups)
+// Split form: 11 digit planes of 2 bytes per (population, SNP) (f2_split.hip).
+constexpr int kSplitPlanes = 11;
 
 struct BlockTable {
     std::int64_t* s0;    // first SNP of block b

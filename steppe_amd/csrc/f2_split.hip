@@ -1,4 +1,5 @@
-// steppe_amd/csrc/f2_split.hip — per-block f2 from fixed-point BF16 slices on BF16 MFMA.
+// steppe_amd/csrc/f2_split.hip — per-block f2 from fixed-point digit slices on BF16 and
+// FP16 MFMA.
 //
 // The reference's default precision is cuBLAS's closed FP64 emulation on integer
 // units (src/device/cuda/f2_block_kernel.cu:121-144).  This is an independent
@@ -12,33 +13,40 @@
 //
 // Operands.  The estimator only sees differences q_i - q_j, so any per-SNP shift of q
 // leaves it unchanged; with q' = q - 1/2 (|q'| <= 1/2) and w = -(q'^2 - h)/2
-// (|w| <= 1/8), each is written as six balanced 8-bit digits (|digit| <= 128, exact
-// in BF16):
-//     q~ = 2^-7 sum_k d_k 2^-8k,     w~ = 2^-7 sum_m c_m 2^-8m,     k, m = 0..5
-// w is computed from q~ (not q'), so the device evaluates the estimator for the
-// perturbed frequencies q~ (|q~ - q'| <= 2^-48).  Then
-//     -num_ij / 2 = sum_s [ q~_i q~_j + w~_i v_j + v_i w~_j ]
-//                 = sum_t U_t * L_t,   U_t = 2^(-14-8t)
-//     L_t = sum_s [ sum_{k+l=t} d_k(i) d_l(j) + e_t(i) v(j) + v(i) e_t(j) ],  e_t = 2^7 c_t
-// Levels t = 0..4 each have an FP32 accumulator.  Per SNP, |L_t| grows by at most
-// (t+3) 2^14, so 128 SNPs keep every L_t below 2^24 and the MFMA sums are exact.
-// Level 5 (d_k d_{5-k} for k = 0..5, and c_5) is folded into L_4 through operands
-// scaled by 2^-8 (exact in BF16): those sums are rounded at 2^-47 per MFMA, far
-// below the result's precision.  Products with k + l >= 6 are dropped
-// (<= 2^-45 per SNP).  Every 128 SNPs each level carries its multiple of 2^8 into
-// the level above (exact FP32 operations) and L_0 carries multiples of 2^12 into an
-// FP32 high word, so a block of any length accumulates in six FP32 words per output
-// and FP64 appears only in the per-block epilogue.
+// (|w| <= 1/8):
+//   q' is written as six balanced 8-bit digits (|digit| <= 128, exact in BF16),
+//       q~ = 2^-7 sum_k d_k 2^-8k,  k = 0..5;
+//   w (computed from q~, not q') as four FP16 digits of up to 11 significant bits,
+//       w~ = A 2^-14 + B 2^-30 + C 2^-38 + D 2^-46,
+//       A = rint(w 2^14) (|A| <= 2048), B = 32 b, C = 4 c (|b|, |c| <= 1024), |D| <= 512,
+//   so |w - w~| <= 2^-47, and each digit is an integer in the units of the level it
+//   enters.  The device evaluates the estimator for the perturbed frequencies q~
+//   (|q~ - q'| <= 2^-48).  Then, with level units U_t = 2^(-14-8t),
+//     -num_ij / 2 = sum_s [ q~_i q~_j + w~_i v_j + v_i w~_j ] = sum_t U_t * L_t,
+//     L_t = sum_s [ sum_{k+l=t} d_k(i) d_l(j) + W_t(i) v(j) + v(i) W_t(j) ],
+//   W_0 = A, W_1 = 0, W_2 = B, W_3 = C, W_4 = D.
+// The digit products d_k d_l run on BF16 MFMA and the W_t v products on FP16 MFMA
+// (v in {0, 1}); both accumulate into the same FP32 level accumulators.  Levels
+// t = 0..4 each have one.  Per SNP, |L_2| grows by at most 3 * 2^15 and the others by
+// less, so 128 SNPs keep every L_t below 2^24 and the MFMA sums are exact.
+// Level 5 (d_k d_{5-k} for k = 0..5) is folded into L_4 through operands scaled by
+// 2^-8 (exact in BF16): those sums are rounded at 2^-47 per MFMA, far below the
+// result's precision.  Products with k + l >= 6 are dropped (<= 2^-45 per SNP).
+// Every 128 SNPs each level carries its multiple of 2^8 into the level above (exact
+// FP32 operations) and L_0 carries multiples of 2^12 into an FP32 high word, so a
+// block of any length accumulates in six FP32 words per output and FP64 appears only
+// in the per-block epilogue.
 //
-// MFMA work per (pair, SNP): 33 BF16 products (21 digit-by-digit, 12 digit-by-V),
-// plus the pair count V V^T, computed exactly by a separate int8 MFMA kernel.
+// MFMA work per (pair, SNP): 29 products (21 BF16 digit-by-digit, 8 FP16
+// digit-by-V), plus the pair count V V^T, computed exactly by a separate int8 MFMA
+// kernel.  Operand bytes per (population, SNP): 22 (six q digits, four w digits, V).
 //
 // Kernel shape: persistent workgroups of 8 wavefronts (2 per SIMD) own a 128x128
 // output tile of one jackknife block; each wavefront owns 64x32 (two 32x32 MFMA
 // tiles, six FP32 accumulators per output).  Each 16-SNP K-step is staged through
-// LDS in three phases (the 6 q-digit planes; e0..e2 + V; e3..e5 + V) by direct
-// global->LDS loads into a three-slot ring, so two phases of loads are in flight
-// while one is computed.
+// LDS in two phases by direct global->LDS loads: the q digits (two slots, alternating
+// between K-steps), then the w digits and V (A and V in two alternating slots, B..D
+// in one).
 
 #include <utility>
 
@@ -54,15 +62,16 @@ namespace {
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef int i4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
-constexpr int kPlanes = 13;            // d0..d5 (0-5), e0..e5 (6-11), V (12)
-constexpr int kVPlane = 12;
+constexpr int kPlanes = kSplitPlanes;  // d0..d5 (0-5), w digit A (6), V (7), w digits B, C, D (8-10)
+constexpr int kPlaneA = 6, kVPlane = 7;
 constexpr int kChunk = 1024;           // bytes per (32-row group, 16-SNP step, plane)
 constexpr int kGroups = 8;             // 32-row groups per workgroup tile: 4 left + 4 right
-constexpr int kDPlanes = 6, kE1Planes = 3, kE2Planes = 4;  // planes per phase: q digits; e0..e2; e3..e5 + V
+constexpr int kDPlanes = 6, kXPlanes = 2, kYPlanes = 3;      // planes per slot: q digits; A + V; B..D
 constexpr int kDSlotBytes = kGroups * kDPlanes * kChunk;    // 48 KB
-constexpr int kE1SlotBytes = kGroups * kE1Planes * kChunk;  // 24 KB
-constexpr int kE2SlotBytes = kGroups * kE2Planes * kChunk;  // 32 KB
+constexpr int kXSlotBytes = kGroups * kXPlanes * kChunk;    // 16 KB
+constexpr int kYSlotBytes = kGroups * kYPlanes * kChunk;    // 24 KB
 constexpr int kWaves = 8;
 
 // f(std::integral_constant<int, I>) for I = 0 .. N-1, as straight-line code.
@@ -74,11 +83,12 @@ __device__ __forceinline__ void static_for(F&& f) {
 }
 
 // ---------------------------------------------------------------------------------
-// prep: frequencies -> 13 BF16 digit planes (32x32x16 MFMA fragment order) + the
-// int8 V plane (16x16x64 fragment order, for the count kernel), in one pass.
+// prep: frequencies -> 11 digit planes (32x32x16 MFMA fragment order: six BF16 q
+// digits, then FP16 A, V, B, C, D) + the int8 V plane (16x16x64 fragment order, for
+// the count kernel), in one pass.
 //   plane chunk (g32, t16, p) holds rows 32 g32 + lane%32, SNPs 16 t16 + 8 (lane/32) + j
-//   at byte offset ((g32 * KS16 + t16) * 13 + p) * 1024 + lane * 16 + 2 j.
-//   Planes 5 (d5) and 11 (e5) are stored scaled by 2^-8 (they only enter level 5).
+//   at byte offset ((g32 * KS16 + t16) * 11 + p) * 1024 + lane * 16 + 2 j.
+//   Plane 5 (d5) is stored scaled by 2^-8 (it only enters level 5).
 // A workgroup converts one (32-row group, 64-SNP chunk) unit at a time: it stages the
 // unit's frequencies and heterozygosity terms in LDS with coalesced row reads, then
 // every thread expands one 8-SNP fragment into its digits.
@@ -201,6 +211,13 @@ __device__ inline unsigned bf16_bits(double integer_digit) {
     return __float_as_uint(static_cast<float>(integer_digit)) >> 16;
 }
 
+__device__ inline unsigned f16_bits(double digit) {
+    // The w digits are integers of at most 11 significant bits times a power of two
+    // (|A| <= 2048, 32 b, 4 c with |b|, |c| <= 1024, |D| <= 512) and V is 0 or 1: the
+    // float and FP16 conversions are exact.
+    return __builtin_bit_cast(unsigned short, static_cast<_Float16>(static_cast<float>(digit)));
+}
+
 template <class Src>
 __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32, int G16, BlockTable bt,
                                                          const std::int32_t* __restrict__ chunk_block,
@@ -252,21 +269,24 @@ __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32
                 }
                 const double qt =
                     (d[0] + (d[1] + (d[2] + (d[3] + (d[4] + d[5] / 256.0) / 256.0) / 256.0) / 256.0) / 256.0) / 128.0;
-                double y = -0.5 * (qt * qt - h) * 128.0;
-                double e[6];
-#pragma unroll
-                for (int m = 0; m < 6; ++m) {
-                    e[m] = rint(y);
-                    y = (y - e[m]) * 256.0;
-                }
+                // w in units of 2^-14, then the remainders in units of 2^-30 / 32, 2^-38 / 4
+                // and 2^-46 (every step is exact in FP64).
+                const double y = -0.5 * (qt * qt - h) * 16384.0;
+                const double wA = rint(y);
+                const double r1 = (y - wA) * 2048.0;
+                const double wb = rint(r1);
+                const double r2 = (r1 - wb) * 2048.0;
+                const double wc = rint(r2);
+                const double wD = rint((r2 - wc) * 1024.0);
                 const int sft = 16 * (j & 1);
 #pragma unroll
                 for (int k = 0; k < 5; ++k) w[k][j >> 1] |= bf16_bits(d[k]) << sft;
                 w[5][j >> 1] |= bf16_bits(d[5] / 256.0) << sft;
-#pragma unroll
-                for (int m = 0; m < 5; ++m) w[6 + m][j >> 1] |= bf16_bits(e[m] * 128.0) << sft;
-                w[11][j >> 1] |= bf16_bits(e[5] * 0.5) << sft;
-                w[kVPlane][j >> 1] |= bf16_bits(1.0) << sft;
+                w[kPlaneA][j >> 1] |= f16_bits(wA) << sft;
+                w[kVPlane][j >> 1] |= f16_bits(1.0) << sft;
+                w[8][j >> 1] |= f16_bits(32.0 * wb) << sft;
+                w[9][j >> 1] |= f16_bits(4.0 * wc) << sft;
+                w[10][j >> 1] |= f16_bits(wD) << sft;
             }
         }
         const std::int64_t t16 = 4 * c + t4;
@@ -391,26 +411,26 @@ __device__ inline void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
-// Phase kinds of a K-step: 0 = q digits d0..d5 (planes 0-5), 1 = e0..e2 (planes 6-8),
-// 2 = e3..e5 + V (planes 9-11, 12).
+// Slot kinds of a K-step: 0 = q digits d0..d5 (planes 0-5), 1 = A + V (planes 6-7),
+// 2 = B, C, D (planes 8-10).
 template <int kind>
-constexpr int phase_planes() {
-    return kind == 0 ? kDPlanes : (kind == 1 ? kE1Planes : kE2Planes);
+constexpr int slot_planes() {
+    return kind == 0 ? kDPlanes : (kind == 1 ? kXPlanes : kYPlanes);
 }
-__device__ inline int phase_plane(int kind, int pp) {
-    if (kind == 0) return pp;
-    if (kind == 1) return 6 + pp;
-    return pp < 3 ? 9 + pp : kVPlane;
+template <int kind>
+constexpr int slot_first_plane() {
+    return kind == 0 ? 0 : (kind == 1 ? kPlaneA : kVPlane + 1);
 }
 
-// Issue this wavefront's share of one phase's 1 KB chunks (direct global->LDS).
-// Every wavefront issues exactly (8 groups * planes) / 8 = planes loads per phase; the
-// compile-time count lets the compiler's wait-count tracking see how many loads
-// follow each phase's, so that it adds no wait of its own beyond the explicit ones.
+// Issue this wavefront's share of one slot's 1 KB chunks (direct global->LDS).
+// Every wavefront issues exactly (8 groups * planes) / 8 = planes loads per slot, and
+// chunk ci of a slot is always loaded by wavefront ci % 8; the compile-time count
+// lets the compiler's wait-count tracking see how many loads follow each slot's, so
+// that it adds no wait of its own beyond the explicit ones.
 template <int kind>
-__device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, std::int64_t t16,
-                                   std::int64_t KS16, int gI0, int gJ0, int G32, int wave, int lane) {
-    constexpr int np = phase_planes<kind>();
+__device__ inline void stage_slot(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, std::int64_t t16,
+                                  std::int64_t KS16, int gI0, int gJ0, int G32, int wave, int lane) {
+    constexpr int np = slot_planes<kind>();
 #pragma unroll
     for (int r = 0; r < np; ++r) {
         const int ci = wave + kWaves * r;
@@ -418,44 +438,30 @@ __device__ inline void stage_phase(const std::uint8_t* __restrict__ planes, std:
         int gg = lg < 4 ? gI0 + lg : gJ0 + lg - 4;
         if (gg >= G32) gg = 0;  // rows beyond P: any valid address; never read back
         const std::uint8_t* src =
-            planes + ((static_cast<std::int64_t>(gg) * KS16 + t16) * kPlanes + phase_plane(kind, pp)) * kChunk +
+            planes + ((static_cast<std::int64_t>(gg) * KS16 + t16) * kPlanes + slot_first_plane<kind>() + pp) * kChunk +
             lane * 16;
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
                                          (__attribute__((address_space(3))) void*)(slot + ci * kChunk), 16, 0, 0);
     }
 }
 
-// Load r of this wavefront's share of one phase (the r-th iteration of stage_phase).
-template <int kind, int r>
-__device__ inline void stage_one(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, std::int64_t t16,
-                                 std::int64_t KS16, int gI0, int gJ0, int G32, int wave, int lane) {
-    constexpr int np = phase_planes<kind>();
-    static_assert(r < np, "one load per plane of the phase");
-    const int ci = wave + kWaves * r;
-    const int lg = ci / np, pp = ci - lg * np;
-    int gg = lg < 4 ? gI0 + lg : gJ0 + lg - 4;
-    if (gg >= G32) gg = 0;  // rows beyond P: any valid address; never read back
-    const std::uint8_t* src =
-        planes + ((static_cast<std::int64_t>(gg) * KS16 + t16) * kPlanes + phase_plane(kind, pp)) * kChunk + lane * 16;
-    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
-                                     (__attribute__((address_space(3))) void*)(slot + ci * kChunk), 16, 0, 0);
+__device__ inline h8 lds_frag_h(const std::uint8_t* slot, int group, int plane, int nplanes, int lane) {
+    return *reinterpret_cast<const h8*>(slot + (group * nplanes + plane) * kChunk + lane * 16);
 }
 
-// IL: issue each phase's loads one at a time between its MFMAs instead of as one burst
-// after the phase barrier (same loads, same waits).
-template <bool IL>
 __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __restrict__ planes, BlockTable bt,
                                                        const F2Item* __restrict__ items, ItemQueues queues,
                                                        unsigned* __restrict__ counters, int P, int G32,
                                                        std::int64_t KS16, const double* __restrict__ vpair,
                                                        double* __restrict__ f2, FixupList fl) {
     // One LDS object per ring slot: the compiler then knows that reads of one slot do
-    // not depend on the direct loads still landing in the others.  Two q-digit slots
-    // alternate between K-steps; one slot each for e0..e2 and for e3..e5 + V.
+    // not depend on the direct loads still landing in the others.  The q-digit slots
+    // and the A + V slots alternate between K-steps; B..D have one slot.
     __shared__ __attribute__((aligned(16))) std::uint8_t lds_d0[kDSlotBytes];
     __shared__ __attribute__((aligned(16))) std::uint8_t lds_d1[kDSlotBytes];
-    __shared__ __attribute__((aligned(16))) std::uint8_t lds_e1[kE1SlotBytes];
-    __shared__ __attribute__((aligned(16))) std::uint8_t lds_e2[kE2SlotBytes];
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_x0[kXSlotBytes];
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_x1[kXSlotBytes];
+    __shared__ __attribute__((aligned(16))) std::uint8_t lds_y[kYSlotBytes];
     __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     // Wavefront-uniform values are moved to scalar registers so that every branch
@@ -499,27 +505,29 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
             H[x] = f16v{};
         }
 
-        // Load schedule (per wavefront; all loads stage unconditionally, so every path
-        // issues the same counts and the explicit waits below are exact):
-        //   prologue:        D(0) -> d0, E1(0), D(1) -> d1
-        //   step s, phase d:  wait until <= 9 loads are outstanding (D(s) done), barrier,
-        //                     issue E2(s); q-digit products from D(s)
-        //   phase e1:         wait for all (E1(s), E2(s)), barrier, issue D(s+2) into
-        //                     D(s)'s slot; e0..e2 products (V from E2)
-        //   phase e2:         barrier (E1 free), issue E1(s+1); e3..e5 products
-        // D(s) is in flight for a step and a half, E1 for phases e2 + d, E2 for phase d.
-        stage_phase<0>(planes, lds_d0, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
-        stage_phase<1>(planes, lds_e1, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
-        stage_phase<0>(planes, lds_d1, kstep(1), KS16, gI0, gJ0, G32, wave, lane);
+        // Load schedule (per wavefront, loads per slot D 6, X 2, Y 3; all loads stage
+        // unconditionally, so every path issues the same counts and the explicit waits
+        // below are exact):
+        //   prologue:       D(0) -> d0, X(0) -> x0, D(1) -> d1
+        //   step s, phase d: wait until <= 8 loads are outstanding (D(s) done: only
+        //                    binding at s = 0), barrier (Y free), issue Y(s); q-digit
+        //                    products from D(s)
+        //   phase w:         wait for all (Y(s), and X(s), D(s+1) issued before it),
+        //                    barrier (D(s) and X(s-1) free), issue D(s+2) into D(s)'s
+        //                    slot and X(s+1) into X(s-1)'s; w-digit products from X(s), Y(s)
+        stage_slot<0>(planes, lds_d0, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
+        stage_slot<1>(planes, lds_x0, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
+        stage_slot<0>(planes, lds_d1, kstep(1), KS16, gI0, gJ0, G32, wave, lane);
 
         auto step_body = [&]<int par>(int step) {
             std::uint8_t* const slot_d = par ? lds_d1 : lds_d0;
-            // ---- q digits
-            wait_vmcnt<9>();
+            std::uint8_t* const slot_x = par ? lds_x1 : lds_x0;
+            std::uint8_t* const slot_xn = par ? lds_x0 : lds_x1;
+            // ---- q digits (BF16)
+            wait_vmcnt<kDPlanes + kXPlanes>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            const std::int64_t t_e2 = kstep(step);
-            if constexpr (!IL) stage_phase<2>(planes, lds_e2, t_e2, KS16, gI0, gJ0, G32, wave, lane);
+            stage_slot<2>(planes, lds_y, kstep(step), KS16, gI0, gJ0, G32, wave, lane);
             {
                 bf8 dj[6];
 #pragma unroll
@@ -537,76 +545,32 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                         const bf8 d5l = (k == 0 || k == 5) ? di : scale_2m8(di);
                         L[x][4] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(d5l, dj[5 - k], L[x][4], 0, 0, 0);
                     }
-                    if constexpr (IL && k < kE2Planes)
-                        stage_one<2, k>(planes, lds_e2, t_e2, KS16, gI0, gJ0, G32, wave, lane);
                 });
             }
-            // ---- e0..e2 (V from the e3..e5 slot)
+            // ---- w digits x V (FP16): A into L0, B, C, D into L2, L3, L4
             wait_vmcnt<0>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            const std::int64_t t_d = kstep(step + 2);
-            if constexpr (!IL) stage_phase<0>(planes, slot_d, t_d, KS16, gI0, gJ0, G32, wave, lane);
+            stage_slot<0>(planes, slot_d, kstep(step + 2), KS16, gI0, gJ0, G32, wave, lane);
+            stage_slot<1>(planes, slot_xn, kstep(step + 1), KS16, gI0, gJ0, G32, wave, lane);
             {
-                bf8 ej[3];
+                const h8 aj = lds_frag_h(slot_x, 4 + wc, 0, kXPlanes, lane);
+                const h8 vj = lds_frag_h(slot_x, 4 + wc, 1, kXPlanes, lane);
+                h8 wj[3];
 #pragma unroll
-                for (int m = 0; m < 3; ++m) ej[m] = lds_frag(lds_e1, 4 + wc, m, kE1Planes, lane);
-                const bf8 vj = lds_frag(lds_e2, 4 + wc, 3, kE2Planes, lane);
-                static_for<2>([&](auto xc) {
-                    constexpr int x = decltype(xc)::value;
-                    static_for<3>([&](auto mc) {
-                        constexpr int m = decltype(mc)::value;
-                        if (act[x]) {
-                            const bf8 ei = lds_frag(lds_e1, 2 * wa + x, m, kE1Planes, lane);
-                            L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][m], 0, 0, 0);
-                        }
-                        if constexpr (IL) stage_one<0, 3 * x + m>(planes, slot_d, t_d, KS16, gI0, gJ0, G32, wave, lane);
-                    });
-                    if (act[x]) {
-                        const bf8 vi = lds_frag(lds_e2, 2 * wa + x, 3, kE2Planes, lane);
-#pragma unroll
-                        for (int m = 0; m < 3; ++m)
-                            L[x][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vi, ej[m], L[x][m], 0, 0, 0);
-                    }
-                });
-            }
-            // ---- e3..e5 + V (e5 pre-scaled into level 4)
-            __builtin_amdgcn_s_barrier();
-            asm volatile("" ::: "memory");
-            const std::int64_t t_e1 = kstep(step + 1);
-            if constexpr (!IL) stage_phase<1>(planes, lds_e1, t_e1, KS16, gI0, gJ0, G32, wave, lane);
-            {
-                bf8 ej[3];
-#pragma unroll
-                for (int m = 0; m < 3; ++m) ej[m] = lds_frag(lds_e2, 4 + wc, m, kE2Planes, lane);
-                const bf8 vj = lds_frag(lds_e2, 4 + wc, 3, kE2Planes, lane);
-                if constexpr (IL) {
-                    // the three loads of E1(s+1) go between the first tile's MFMAs
-                    static_for<3>([&](auto mc) {
-                        constexpr int m = decltype(mc)::value;
-                        if (act[0]) {
-                            const bf8 ei = lds_frag(lds_e2, 2 * wa, m, kE2Planes, lane);
-                            const int t = m == 2 ? 4 : 3 + m;
-                            L[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[0][t], 0, 0, 0);
-                        }
-                        stage_one<1, m>(planes, lds_e1, t_e1, KS16, gI0, gJ0, G32, wave, lane);
-                    });
-                }
+                for (int m = 0; m < 3; ++m) wj[m] = lds_frag_h(lds_y, 4 + wc, m, kYPlanes, lane);
 #pragma unroll
                 for (int x = 0; x < 2; ++x) {
                     if (!act[x]) continue;
+                    const h8 ai = lds_frag_h(slot_x, 2 * wa + x, 0, kXPlanes, lane);
+                    const h8 vi = lds_frag_h(slot_x, 2 * wa + x, 1, kXPlanes, lane);
+                    L[x][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ai, vj, L[x][0], 0, 0, 0);
+                    L[x][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(vi, aj, L[x][0], 0, 0, 0);
 #pragma unroll
                     for (int m = 0; m < 3; ++m) {
-                        if (IL && x == 0) break;   // issued above, between the loads
-                        const bf8 ei = lds_frag(lds_e2, 2 * wa + x, m, kE2Planes, lane);
-                        const int t = m == 2 ? 4 : 3 + m;
-                        L[x][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ei, vj, L[x][t], 0, 0, 0);
-                    }
-                    const bf8 vi = lds_frag(lds_e2, 2 * wa + x, 3, kE2Planes, lane);
-#pragma unroll
-                    for (int m = 0; m < 3; ++m) {
-                        const int t = m == 2 ? 4 : 3 + m;
-                        L[x][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vi, ej[m], L[x][t], 0, 0, 0);
+                        const h8 wi = lds_frag_h(lds_y, 2 * wa + x, m, kYPlanes, lane);
+                        L[x][2 + m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(wi, vj, L[x][2 + m], 0, 0, 0);
+                        L[x][2 + m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(vi, wj[m], L[x][2 + m], 0, 0, 0);
                     }
                 }
                 // Every 8 K-steps (128 SNPs) and at the block end: exact carries.
@@ -630,9 +594,9 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                 }
             }
         };
-        // Steps alternate between the two q-digit slots; unrolled by two so that each
-        // slot is a distinct LDS object in every access, with an odd last step after
-        // the loop (so that both bodies always run inside it).
+        // Steps alternate between the two q-digit (and A + V) slots; unrolled by two so
+        // that each slot is a distinct LDS object in every access, with an odd last step
+        // after the loop (so that both bodies always run inside it).
         int step = 0;
         for (; step + 1 < nsteps; step += 2) {
             step_body.template operator()<0>(step);
@@ -713,15 +677,11 @@ void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const Ite
 
 void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
                      unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
-                     const FixupList& fl, int n_workgroups, bool interleave, hipStream_t st) {
+                     const FixupList& fl, int n_workgroups, hipStream_t st) {
     if (queues.off[kXcds] <= 0) return;
     launch_zero_u32(counters, kXcds, st);
-    if (interleave)
-        f2_split_kernel<true><<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues, counters,
-                                                                                  P, G32, KS16, vpair, f2, fl);
-    else
-        f2_split_kernel<false><<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues,
-                                                                                   counters, P, G32, KS16, vpair, f2, fl);
+    f2_split_kernel<<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues, counters, P, G32,
+                                                                        KS16, vpair, f2, fl);
     STP_LAUNCH_CHECK();
 }
 

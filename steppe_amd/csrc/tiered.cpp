@@ -128,7 +128,7 @@ std::size_t f2_working_set_bytes(const F2Plan& plan, int precision) {
     const std::size_t rows16 = static_cast<std::size_t>(plan.G) * kRowGroup;
     const std::size_t v8 = rows16 * K;   // int8 V plane
     const std::size_t planes = precision == STP_PREC_FP64 ? 3 * sizeof(double) * rows16 * K
-                                                          : 26 * static_cast<std::size_t>(plan.G32) * 32 * K;
+                                                          : 2 * kSplitPlanes * static_cast<std::size_t>(plan.G32) * 32 * K;
     const std::size_t upper = static_cast<std::size_t>(plan.P) * (plan.P + 1) / 2 * plan.n_block;
     const std::size_t fix = 12 * std::min<std::size_t>(std::max<std::size_t>(upper / 16, 1 << 16), 1 << 24);
     return planes + v8 + fix;

@@ -151,14 +151,14 @@ def _tier_choice(P, M, n_block, precision, free_vram, free_host):
 
 
 def test_tier_choice_counts_the_working_set():
-    # P = 4000, M = 1.2M SNPs in 700 blocks, split form: 179 GB of f2 + vpair and ~131 GB of
-    # operand planes.  With 288 GB free the result alone fits 70% of HBM (201 GB) but result +
+    # P = 4000, M = 1.2M SNPs in 700 blocks, split form: 179 GB of f2 + vpair and ~111 GB of
+    # operand planes (22 B per population and SNP).  With 288 GB free the result alone fits 70% of HBM (201 GB) but result +
     # planes do not: the host tier is chosen (the reference adds its working set the same way,
     # src/device/tier_select.hpp:62-72).
     P, M, nb = 4000, 1_200_000, 700
     result = 2 * P * P * nb * 8
     wb, tier = _tier_choice(P, M, nb, _core.STP_PREC_SPLIT_BF16, 288 << 30, 2 << 40)
-    planes = 26 * ((P + 31) // 32 * 32) * (M // nb + 63) // 64 * 64 * nb
+    planes = 22 * ((P + 31) // 32 * 32) * (M // nb + 63) // 64 * 64 * nb
     assert wb >= planes and result <= 0.70 * (288 << 30) < result + wb
     assert tier == 1
     # the same shape with twice the device memory stays resident
