@@ -107,20 +107,25 @@ struct CountsSrc {
     std::int64_t ld;
     struct Raw {
         unsigned a[8], c[8];
+        bool ok[8];
     };
     // Lanes run along the SNP axis: each wavefront reads 128 contiguous bytes per row.
+    // The loads are unconditional (entries outside the unit re-read a valid one and are
+    // masked), so the compiler knows how many are in flight and waits for them alone,
+    // not for the previous unit's plane stores as well.
     __device__ Raw fetch(int r0, int P, std::int64_t snp0, int live, int tid) const {
         Raw raw;
         const int kk = tid & 63;
+        const bool col_ok = kk < live;
+        const std::int64_t col = snp0 + (col_ok ? kk : 0);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int row = (tid >> 6) + 4 * i;
-            raw.a[i] = raw.c[i] = 0u;
-            if (r0 + row < P && kk < live) {
-                const std::int64_t o = static_cast<std::int64_t>(r0 + row) * ld + snp0 + kk;
-                raw.a[i] = a2[o];
-                raw.c[i] = n[o];
-            }
+            const bool ok = col_ok && r0 + row < P;
+            const std::int64_t o = static_cast<std::int64_t>(ok ? r0 + row : r0) * ld + col;
+            raw.a[i] = a2[o];
+            raw.c[i] = n[o];
+            raw.ok[i] = ok;
         }
         return raw;
     }
@@ -130,7 +135,7 @@ struct CountsSrc {
         for (int i = 0; i < 8; ++i) {
             const int row = (tid >> 6) + 4 * i;
             double qp = 0.0, h = -1.0;
-            if (raw.c[i]) {
+            if (raw.ok[i] && raw.c[i]) {
                 const double nd = static_cast<double>(raw.c[i]);
                 const double q = static_cast<double>(raw.a[i]) / (2.0 * nd);
                 const double nm1 = nd - 1.0;
@@ -148,24 +153,24 @@ struct QvnSrc {
     const double* v;
     const double* n;
     struct Raw {
-        double q[8], nm1[8];
+        double q[8], v[8], n[8];
         bool ok[8];
     };
     // Q/V/N are population-fastest (i + P*s): lanes run along the population axis.
+    // Unconditional loads, as in CountsSrc::fetch.
     __device__ Raw fetch(int r0, int P, std::int64_t snp0, int live, int tid) const {
         Raw raw;
         const int row = tid & 31;
+        const bool row_ok = r0 + row < P;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int kk = (tid >> 5) + 8 * i;
-            raw.ok[i] = false;
-            raw.q[i] = raw.nm1[i] = 0.0;
-            if (r0 + row < P && kk < live) {
-                const std::int64_t o = r0 + row + static_cast<std::int64_t>(P) * (snp0 + kk);
-                raw.ok[i] = v[o] != 0.0;
-                raw.q[i] = q[o];
-                raw.nm1[i] = n[o] - 1.0;
-            }
+            const bool ok = row_ok && kk < live;
+            const std::int64_t o = (row_ok ? r0 + row : r0) + static_cast<std::int64_t>(P) * (snp0 + (ok ? kk : 0));
+            raw.v[i] = v[o];
+            raw.q[i] = q[o];
+            raw.n[i] = n[o];
+            raw.ok[i] = ok;
         }
         return raw;
     }
@@ -175,8 +180,8 @@ struct QvnSrc {
         for (int i = 0; i < 8; ++i) {
             const int kk = (tid >> 5) + 8 * i;
             double qp = 0.0, h = -1.0;
-            if (raw.ok[i]) {
-                const double qq = raw.q[i], nm1 = raw.nm1[i];
+            if (raw.ok[i] && raw.v[i] != 0.0) {
+                const double qq = raw.q[i], nm1 = raw.n[i] - 1.0;
                 h = qq * (1.0 - qq) / (nm1 > 1.0 ? nm1 : 1.0);
                 qp = qq - 0.5;
             }
@@ -192,16 +197,28 @@ struct PrepUnit {
     std::int64_t c, snp0;
 };
 
-__device__ inline PrepUnit prep_unit(std::int64_t u, std::int64_t n_chunks, const BlockTable& bt,
+// The block-table arrays arrive as separate read-only, non-aliased kernel arguments, so
+// that these uniform lookups compile to scalar loads: a vector load here would have to
+// wait (in order) for the previous unit's plane stores before its value could be used.
+__device__ inline PrepUnit prep_unit(std::int64_t u, std::int64_t n_chunks, const std::int64_t* __restrict__ s0,
+                                     const std::int32_t* __restrict__ len, const std::int64_t* __restrict__ k0,
                                      const std::int32_t* __restrict__ chunk_block) {
     PrepUnit pu;
     pu.g = static_cast<int>(u / n_chunks);
     pu.c = u - static_cast<std::int64_t>(pu.g) * n_chunks;
     const int b = chunk_block[pu.c];
-    const std::int64_t local0 = 64 * pu.c - bt.k0[b];
-    pu.snp0 = bt.s0[b] + local0;
-    pu.live = static_cast<int>(max<std::int64_t>(0, min<std::int64_t>(64, bt.len[b] - local0)));
+    const std::int64_t local0 = 64 * pu.c - k0[b];
+    pu.snp0 = s0[b] + local0;
+    pu.live = static_cast<int>(max<std::int64_t>(0, min<std::int64_t>(64, len[b] - local0)));
     return pu;
+}
+
+// Workgroup barrier for the LDS stage only: waits for this wavefront's LDS operations,
+// not for its outstanding global stores (__syncthreads would also wait for those).
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 __device__ inline unsigned bf16_bits(double integer_digit) {
@@ -219,7 +236,10 @@ __device__ inline unsigned f16_bits(double digit) {
 }
 
 template <class Src>
-__global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32, int G16, BlockTable bt,
+__global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32, int G16,
+                                                         const std::int64_t* __restrict__ bt_s0,
+                                                         const std::int32_t* __restrict__ bt_len,
+                                                         const std::int64_t* __restrict__ bt_k0,
                                                          const std::int32_t* __restrict__ chunk_block,
                                                          std::int64_t n_chunks, std::int64_t KS16,
                                                          std::uint8_t* __restrict__ planes,
@@ -231,22 +251,22 @@ __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32
     const std::int64_t n_units = static_cast<std::int64_t>(G32) * n_chunks;
     // Units run chunk-fastest so consecutive workgroups read adjacent row segments and
     // write adjacent plane chunks.  The next unit's inputs are fetched into registers
-    // before the current unit is converted, so their latency overlaps the conversion.
+    // before the current unit is converted, so their latency overlaps the conversion,
+    // and the stage is refilled only after the current unit's stores are issued: each
+    // refill then waits for its own loads, never for the plane stores issued after them.
+    // The first unit's stage is filled before the loop, and the last iteration re-reads
+    // a valid unit, so that every path through the loop issues the same operations.
     std::int64_t u = blockIdx.x;
     if (u >= n_units) return;
-    PrepUnit nxt = prep_unit(u, n_chunks, bt, chunk_block);
-    typename Src::Raw raw = src.fetch(32 * nxt.g, P, nxt.snp0, nxt.live, tid);
-    for (; u < n_units; u += gridDim.x) {
-        const PrepUnit cur = nxt;
+    PrepUnit cur = prep_unit(u, n_chunks, bt_s0, bt_len, bt_k0, chunk_block);
+    src.put(src.fetch(32 * cur.g, P, cur.snp0, cur.live, tid), tid, sq, sh);
+    lds_barrier();
+    for (;;) {
+        const bool more = u + gridDim.x < n_units;
+        const PrepUnit nxt = prep_unit(more ? u + gridDim.x : u, n_chunks, bt_s0, bt_len, bt_k0, chunk_block);
+        const typename Src::Raw raw = src.fetch(32 * nxt.g, P, nxt.snp0, nxt.live, tid);
         const int g = cur.g;
         const std::int64_t c = cur.c;
-        __syncthreads();  // the previous unit's readers are done with the stage
-        src.put(raw, tid, sq, sh);
-        __syncthreads();
-        if (u + gridDim.x < n_units) {
-            nxt = prep_unit(u + gridDim.x, n_chunks, bt, chunk_block);
-            raw = src.fetch(32 * nxt.g, P, nxt.snp0, nxt.live, tid);
-        }
 
         // BF16 digit planes: thread -> (16-SNP step t4 of the chunk, fragment lane).
         const int r = lane & 31;
@@ -289,14 +309,10 @@ __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32
                 w[10][j >> 1] |= f16_bits(wD) << sft;
             }
         }
-        const std::int64_t t16 = 4 * c + t4;
-        std::uint8_t* base = planes + ((static_cast<std::int64_t>(g) * KS16 + t16) * kPlanes) * kChunk + lane * 16;
-#pragma unroll
-        for (int p = 0; p < kPlanes; ++p)
-            *reinterpret_cast<uint4*>(base + p * kChunk) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
-
         // int8 V plane for the pair-count kernel (16x16x64 layout): the unit's two
         // 16-row groups, one 16-byte fragment per thread of the first two wavefronts.
+        // Stored before the digit planes, so that the next unit's loads are followed by
+        // exactly the 11 plane stores of every wavefront.
         if (tid < 128) {
             const int g16 = 2 * g + (tid >> 6);
             if (g16 < G16) {
@@ -314,6 +330,20 @@ __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32
                 pV8[(static_cast<std::int64_t>(g16) * (KS16 / 4) + c) * 64 + lane] = v;
             }
         }
+        const std::int64_t t16 = 4 * c + t4;
+        std::uint8_t* base = planes + ((static_cast<std::int64_t>(g) * KS16 + t16) * kPlanes) * kChunk + lane * 16;
+        // Streaming stores: the planes are far larger than the caches and are read back
+        // by a later kernel in another order.
+        typedef unsigned u4v __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int p = 0; p < kPlanes; ++p)
+            __builtin_nontemporal_store(u4v{w[p][0], w[p][1], w[p][2], w[p][3]}, reinterpret_cast<u4v*>(base + p * kChunk));
+        if (!more) break;
+        lds_barrier();  // every reader of this unit's stage is done
+        src.put(raw, tid, sq, sh);
+        lds_barrier();
+        cur = nxt;
+        u += gridDim.x;
     }
 }
 
@@ -652,7 +682,8 @@ void launch_prep_split_counts(const std::uint16_t* a2, const std::uint16_t* n, s
                               std::int64_t KS16, std::uint8_t* planes, void* pV8, hipStream_t st) {
     if (n_chunks <= 0 || G32 <= 0) return;
     prep_split_kernel<CountsSrc><<<prep_grid(G32 * n_chunks), 256, 0, st>>>(
-        CountsSrc{a2, n, ld}, P, G32, G16, bt, chunk_block, n_chunks, KS16, planes, static_cast<i4*>(pV8));
+        CountsSrc{a2, n, ld}, P, G32, G16, bt.s0, bt.len, bt.k0, chunk_block, n_chunks, KS16, planes,
+        static_cast<i4*>(pV8));
     STP_LAUNCH_CHECK();
 }
 
@@ -661,7 +692,8 @@ void launch_prep_split_qvn(const double* q, const double* v, const double* n, in
                            std::uint8_t* planes, void* pV8, hipStream_t st) {
     if (n_chunks <= 0 || G32 <= 0) return;
     prep_split_kernel<QvnSrc><<<prep_grid(G32 * n_chunks), 256, 0, st>>>(
-        QvnSrc{q, v, n}, P, G32, G16, bt, chunk_block, n_chunks, KS16, planes, static_cast<i4*>(pV8));
+        QvnSrc{q, v, n}, P, G32, G16, bt.s0, bt.len, bt.k0, chunk_block, n_chunks, KS16, planes,
+        static_cast<i4*>(pV8));
     STP_LAUNCH_CHECK();
 }
 
