@@ -371,7 +371,8 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
                                                                       ceil_div(units, 4))),
                      kXcds));
     };
-    const int n_wg_count = grid(tn.count_wg_per_cu, n_items);
+    // The pair-count kernel runs one 128x128 item per 4-wavefront workgroup.
+    const int n_wg_count = grid(tn.count_wg_per_cu, 4 * static_cast<std::int64_t>(plan.items128.size()));
     w.planeV8.ensure(std::max<std::size_t>(plane, 16));
     w.counters.ensure(2 * kXcds);
     if (precision == STP_PREC_FP64) {
@@ -388,7 +389,7 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
                                  w.planeA.get(), w.planeV.get(), w.planeQ.get(), w.planeV8.get(), stream_);
         mark_end(kPrep);
         mark_begin(kCount);
-        launch_count(w.planeV8.get(), bt, w.plan.items.get(), plan.q64, w.counters.get(), plan.P, plan.G,
+        launch_count(w.planeV8.get(), bt, w.plan.items128.get(), plan.q128, w.counters.get(), plan.P, plan.G,
                      plan.K_total / 64, d_vpair, n_wg_count, stream_);
         mark_end(kCount);
         const FixupList fl = fixup_list(plan, kFixupTauFp64, w);
@@ -415,7 +416,7 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
                               KS16, w.planeSplit.get(), w.planeV8.get(), stream_);
     mark_end(kPrep);
     mark_begin(kCount);
-    launch_count(w.planeV8.get(), bt, w.plan.items.get(), plan.q64, w.counters.get(), plan.P, plan.G,
+    launch_count(w.planeV8.get(), bt, w.plan.items128.get(), plan.q128, w.counters.get(), plan.P, plan.G,
                  plan.K_total / 64, d_vpair, n_wg_count, stream_);
     mark_end(kCount);
     const int n_items128 = static_cast<int>(plan.items128.size());

@@ -349,25 +349,37 @@ __global__ __launch_bounds__(256) void prep_split_kernel(Src src, int P, int G32
 
 // ---------------------------------------------------------------------------------
 // pair counts: vpair_ij = sum_s v_i v_j by int8 MFMA (exact int32), written for both
-// (i,j) and (j,i).  Persistent wavefronts over (block, 64x64 tile pair) items.
+// (i,j) and (j,i).  Persistent 4-wavefront workgroups over (block, 128x128 tile pair)
+// items; wavefront w owns the 64x64 quadrant (rows 64 (w & 1), columns 64 (w >> 1)),
+// so every 16-row fragment a wavefront loads is loaded by one other wavefront of the
+// workgroup at the same time (the second load hits the CU's L1).  Quadrants below the
+// diagonal of a diagonal tile pair are skipped.
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void count_kernel(const i4* __restrict__ pV8, BlockTable bt,
                                                     const F2Item* __restrict__ items, ItemQueues queues,
                                                     unsigned* __restrict__ counters, int P, int G16,
                                                     std::int64_t KS64, double* __restrict__ vpair) {
+    __shared__ int s_item;
     const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int rh = wave & 1, ch = wave >> 1;
     const std::int64_t slab = static_cast<std::int64_t>(P) * P;
     const int home = static_cast<int>(blockIdx.x % kXcds);
     int xq = 0;
     for (;;) {
-        int it = 0;
-        if (lane == 0) it = claim_item(counters, queues, home, xq);
-        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+        if (threadIdx.x == 0) s_item = claim_item(counters, queues, home, xq);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
         if (it < 0) break;
         const F2Item wi = items[it];
         const int b = __builtin_amdgcn_readfirstlane(wi.b);
-        const int gI0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(wi.ti));
-        const int gJ0 = 4 * __builtin_amdgcn_readfirstlane(static_cast<int>(wi.tj));
+        const int ti = __builtin_amdgcn_readfirstlane(static_cast<int>(wi.ti));
+        const int tj = __builtin_amdgcn_readfirstlane(static_cast<int>(wi.tj));
+        if (ti == tj && rh > ch) continue;
+        const int gI0 = 8 * ti + 4 * rh;
+        const int gJ0 = 8 * tj + 4 * ch;
+        if (gI0 >= G16 || gJ0 >= G16) continue;
         const std::int64_t u0 = bt.k0[b] / 64;
         const int nu = bt.kpad[b] / 64;
         bool rowI[4], rowJ[4];
