@@ -142,9 +142,10 @@ int stp_extract_f2(stp_context* ctx, const stp_tile* tile, const int32_t* block_
 
 /* Where an f2 result lives (reference: DeviceConfig::ForceTier and ExtractTier,
  * include/steppe/config.hpp:121-124, include/steppe/extract.hpp:33; the automatic choice of
- * src/device/tier_select.hpp:57-80).  AUTO keeps it in HBM when it takes at most 0.70 of the
- * free device memory, else in host memory when it takes at most 0.60 of the available host
- * memory, else in an STPF2BK1 f2.bin file at disk_path.  The host and disk tiers contract
+ * src/device/tier_select.hpp:57-80).  AUTO keeps it in HBM when the result plus the
+ * extraction's working set (operand planes, int8 V plane, fix-up list) take at most 0.70 of
+ * the free device memory, else in host memory when it takes at most 0.60 of the available
+ * host memory, else in an STPF2BK1 f2.bin file at disk_path.  The host and disk tiers contract
  * chunk_blocks blocks at a time into one device slab (0: 256 MiB of f2 + vpair, at most a
  * quarter of free device memory; the environment variable STEPPE_F2_CHUNK_BLOCKS sets it
  * when chunk_blocks is 0) and copy
