@@ -32,7 +32,7 @@ namespace stp {
 std::string F2Tune::str() const {
     std::ostringstream o;
     o << "sched=" << sched << ",fp64_rows=" << fp64_rows << ",fp64_prefetch=" << fp64_prefetch
-      << ",fp64_wg=" << fp64_wg_per_cu << ",count_wg=" << count_wg_per_cu;
+      << ",fp64_wg=" << fp64_wg_per_cu << ",count_wg=" << count_wg_per_cu << ",split_ring=" << split_ring;
     return o.str();
 }
 
@@ -51,10 +51,11 @@ bool F2Tune::parse(const std::string& s, F2Tune& out) {
         else if (k == "fp64_prefetch") t.fp64_prefetch = v;
         else if (k == "fp64_wg") t.fp64_wg_per_cu = v;
         else if (k == "count_wg") t.count_wg_per_cu = v;
+        else if (k == "split_ring") t.split_ring = v;
         else return false;
         ++found;
     }
-    if (found != 5 || (t.sched != 0 && t.sched != 1) || (t.fp64_rows != 64 && t.fp64_rows != 32) ||
+    if (found != 6 || (t.split_ring != 0 && t.split_ring != 1) || (t.sched != 0 && t.sched != 1) || (t.fp64_rows != 64 && t.fp64_rows != 32) ||
         t.fp64_wg_per_cu < 1 || t.fp64_wg_per_cu > 8 || t.count_wg_per_cu < 1 || t.count_wg_per_cu > 8)
         return false;
     out = t;
@@ -76,13 +77,15 @@ std::string cache_path() {
 std::vector<F2Tune> candidates(int precision) {
     std::vector<F2Tune> c;
     if (precision == STP_PREC_SPLIT_BF16) {
-        for (int sched : {0, 1})
-            for (int cw : {2, 1, 4}) {
-                F2Tune t;
-                t.sched = sched;
-                t.count_wg_per_cu = cw;
-                c.push_back(t);
-            }
+        for (int ring : {0, 1})
+            for (int sched : {0, 1})
+                for (int cw : {2, 1, 4}) {
+                    F2Tune t;
+                    t.sched = sched;
+                    t.count_wg_per_cu = cw;
+                    t.split_ring = ring;
+                    c.push_back(t);
+                }
     } else {
         for (int sched : {0, 1})
             for (int rows : {64, 32})

@@ -374,7 +374,8 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
     // The pair-count kernel runs one 128x128 item per 4-wavefront workgroup.
     const int n_wg_count = grid(tn.count_wg_per_cu, 4 * static_cast<std::int64_t>(plan.items128.size()));
     w.planeV8.ensure(std::max<std::size_t>(plane, 16));
-    w.counters.ensure(2 * kXcds);
+    // queue cursors of the two kernels, then one item word per split workgroup
+    w.counters.ensure(2 * kXcds + kMaxSplitWorkgroups);
     if (precision == STP_PREC_FP64) {
         w.planeA.ensure(std::max<std::size_t>(plane, 1));
         w.planeV.ensure(std::max<std::size_t>(plane, 1));
@@ -424,7 +425,9 @@ void HipBackend::run_f2(const F2Plan& plan, int precision, const F2Source& src, 
     mark_begin(kF2);
     launch_f2_split(w.planeSplit.get(), bt, w.plan.items128.get(), plan.q128, w.counters.get() + kXcds, plan.P,
                     plan.G32, KS16, d_vpair, d_f2, fl,
-                    static_cast<int>(round_up(std::max(1, std::min(n_cu_, n_items128)), kXcds)), stream_);
+                    std::min(kMaxSplitWorkgroups,
+                             static_cast<int>(round_up(std::max(1, std::min(n_cu_, n_items128)), kXcds))),
+                    tn.split_ring, reinterpret_cast<int*>(w.counters.get() + 2 * kXcds), stream_);
     mark_end(kF2);
     run_fixup(plan, src, fl, d_f2, d_vpair, w);
 }

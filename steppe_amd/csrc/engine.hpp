@@ -59,7 +59,7 @@ void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const Ite
                   hipStream_t st);
 void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
                      unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
-                     const FixupList& fl, int n_workgroups, hipStream_t st);
+                     const FixupList& fl, int n_workgroups, int ring, int* item_slot, hipStream_t st);
 void launch_f2_fixup_counts(const std::uint16_t* a2, const std::uint16_t* n, std::int64_t ld, BlockTable bt,
                             const FixupList& fl, int P, int n_block, const double* vpair, double* f2,
                             int n_workgroups, hipStream_t st);
@@ -87,6 +87,10 @@ struct NoallocScratch {
 NoallocScratch resolve_noalloc_scratch();
 void launch_ploidy_from_i32(const std::int32_t* in, std::int64_t n, std::int8_t* out, hipStream_t st);
 
+// Upper bound on the split contraction's grid (one workgroup per CU; the per-workgroup
+// item words of its second ring layout live after the queue cursors).
+constexpr int kMaxSplitWorkgroups = 4096;
+
 // ---- launch configuration chosen by the autotuner (autotune.cpp) -----------------------
 struct F2Tune {
     int sched = 0;            // work items: 0 per-die queues (blocks longest-first onto the
@@ -95,6 +99,8 @@ struct F2Tune {
     int fp64_prefetch = 1;    // FP64: load the next K-step while this one's MFMAs run
     int fp64_wg_per_cu = 2;   // FP64 contraction workgroups (4 wavefronts) per CU
     int count_wg_per_cu = 2;  // pair-count kernel workgroups per CU
+    int split_ring = 0;       // split contraction LDS ring: 0 (A, V double-buffered, 152 KB) or
+                              // 1 (A, V, B double-buffered, 160 KB)
     std::string str() const;
     static bool parse(const std::string& s, F2Tune& out);
 };

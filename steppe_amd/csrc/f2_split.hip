@@ -45,8 +45,8 @@
 // output tile of one jackknife block; each wavefront owns 64x32 (two 32x32 MFMA
 // tiles, six FP32 accumulators per output).  Each 16-SNP K-step is staged through
 // LDS in two phases by direct global->LDS loads: the q digits (two slots, alternating
-// between K-steps), then the w digits and V (A and V in two alternating slots, B..D
-// in one).
+// between K-steps), then the w digits and V (A, V and, in the second ring layout, B in
+// two alternating slots; the rest in one).
 
 #include <utility>
 
@@ -68,10 +68,16 @@ constexpr int kPlanes = kSplitPlanes;  // d0..d5 (0-5), w digit A (6), V (7), w 
 constexpr int kPlaneA = 6, kVPlane = 7;
 constexpr int kChunk = 1024;           // bytes per (32-row group, 16-SNP step, plane)
 constexpr int kGroups = 8;             // 32-row groups per workgroup tile: 4 left + 4 right
-constexpr int kDPlanes = 6, kXPlanes = 2, kYPlanes = 3;      // planes per slot: q digits; A + V; B..D
+constexpr int kDPlanes = 6;                                 // q digits d0..d5 per slot
 constexpr int kDSlotBytes = kGroups * kDPlanes * kChunk;    // 48 KB
-constexpr int kXSlotBytes = kGroups * kXPlanes * kChunk;    // 16 KB
-constexpr int kYSlotBytes = kGroups * kYPlanes * kChunk;    // 24 KB
+// The w-digit and V planes in two slot kinds, X (double-buffered) and Y (single), for two
+// ring layouts: RING 0 holds A, V in X and B, C, D in Y (152 KB of LDS in all, the item
+// index broadcast through LDS); RING 1 holds A, V, B in X and C, D in Y (160 KB, all of
+// the LDS, so the item index goes through a per-workgroup word in device memory).
+template <int RING>
+constexpr int x_planes() { return RING ? 3 : 2; }
+template <int RING>
+constexpr int y_planes() { return RING ? 2 : 3; }
 constexpr int kWaves = 8;
 
 // f(std::integral_constant<int, I>) for I = 0 .. N-1, as straight-line code.
@@ -453,15 +459,15 @@ __device__ inline void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
-// Slot kinds of a K-step: 0 = q digits d0..d5 (planes 0-5), 1 = A + V (planes 6-7),
-// 2 = B, C, D (planes 8-10).
-template <int kind>
+// Slot kinds of a K-step: 0 = q digits d0..d5 (planes 0-5), 1 = X (A, V and, in RING 1,
+// B: planes 6-7 or 6-8), 2 = Y (the rest of B, C, D: planes 8-10 or 9-10).
+template <int kind, int RING>
 constexpr int slot_planes() {
-    return kind == 0 ? kDPlanes : (kind == 1 ? kXPlanes : kYPlanes);
+    return kind == 0 ? kDPlanes : (kind == 1 ? x_planes<RING>() : y_planes<RING>());
 }
-template <int kind>
+template <int kind, int RING>
 constexpr int slot_first_plane() {
-    return kind == 0 ? 0 : (kind == 1 ? kPlaneA : kVPlane + 1);
+    return kind == 0 ? 0 : (kind == 1 ? kPlaneA : kPlaneA + x_planes<RING>());
 }
 
 // Issue this wavefront's share of one slot's 1 KB chunks (direct global->LDS).
@@ -469,10 +475,10 @@ constexpr int slot_first_plane() {
 // chunk ci of a slot is always loaded by wavefront ci % 8; the compile-time count
 // lets the compiler's wait-count tracking see how many loads follow each slot's, so
 // that it adds no wait of its own beyond the explicit ones.
-template <int kind>
+template <int kind, int RING>
 __device__ inline void stage_slot(const std::uint8_t* __restrict__ planes, std::uint8_t* slot, std::int64_t t16,
                                   std::int64_t KS16, int gI0, int gJ0, int G32, int wave, int lane) {
-    constexpr int np = slot_planes<kind>();
+    constexpr int np = slot_planes<kind, RING>();
 #pragma unroll
     for (int r = 0; r < np; ++r) {
         const int ci = wave + kWaves * r;
@@ -480,7 +486,7 @@ __device__ inline void stage_slot(const std::uint8_t* __restrict__ planes, std::
         int gg = lg < 4 ? gI0 + lg : gJ0 + lg - 4;
         if (gg >= G32) gg = 0;  // rows beyond P: any valid address; never read back
         const std::uint8_t* src =
-            planes + ((static_cast<std::int64_t>(gg) * KS16 + t16) * kPlanes + slot_first_plane<kind>() + pp) * kChunk +
+            planes + ((static_cast<std::int64_t>(gg) * KS16 + t16) * kPlanes + slot_first_plane<kind, RING>() + pp) * kChunk +
             lane * 16;
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
                                          (__attribute__((address_space(3))) void*)(slot + ci * kChunk), 16, 0, 0);
@@ -491,11 +497,15 @@ __device__ inline h8 lds_frag_h(const std::uint8_t* slot, int group, int plane, 
     return *reinterpret_cast<const h8*>(slot + (group * nplanes + plane) * kChunk + lane * 16);
 }
 
+template <int RING>
 __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __restrict__ planes, BlockTable bt,
                                                        const F2Item* __restrict__ items, ItemQueues queues,
                                                        unsigned* __restrict__ counters, int P, int G32,
                                                        std::int64_t KS16, const double* __restrict__ vpair,
-                                                       double* __restrict__ f2, FixupList fl) {
+                                                       double* __restrict__ f2, FixupList fl,
+                                                       int* __restrict__ item_slot) {
+    constexpr int kXPlanes = x_planes<RING>(), kYPlanes = y_planes<RING>();
+    constexpr int kXSlotBytes = kGroups * kXPlanes * kChunk, kYSlotBytes = kGroups * kYPlanes * kChunk;
     // One LDS object per ring slot: the compiler then knows that reads of one slot do
     // not depend on the direct loads still landing in the others.  The q-digit slots
     // and the A + V slots alternate between K-steps; B..D have one slot.
@@ -504,7 +514,6 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
     __shared__ __attribute__((aligned(16))) std::uint8_t lds_x0[kXSlotBytes];
     __shared__ __attribute__((aligned(16))) std::uint8_t lds_x1[kXSlotBytes];
     __shared__ __attribute__((aligned(16))) std::uint8_t lds_y[kYSlotBytes];
-    __shared__ int s_item;
     const int lane = threadIdx.x & 63;
     // Wavefront-uniform values are moved to scalar registers so that every branch
     // on them is uniform and the MFMA accumulators never sit in divergent flow.
@@ -517,10 +526,22 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
     const int home = static_cast<int>(blockIdx.x % kXcds);
     int xq = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_item = claim_item(counters, queues, home, xq);
-        __syncthreads();
-        const int it = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();
+        int it;
+        if constexpr (RING == 0) {
+            __shared__ int s_item;
+            if (threadIdx.x == 0) s_item = claim_item(counters, queues, home, xq);
+            __syncthreads();
+            it = __builtin_amdgcn_readfirstlane(s_item);
+            __syncthreads();
+        } else {
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&item_slot[blockIdx.x], claim_item(counters, queues, home, xq), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __syncthreads();
+            it = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&item_slot[blockIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+            __syncthreads();
+        }
         if (it < 0) break;
         const F2Item wi = items[it];
         const int b = __builtin_amdgcn_readfirstlane(wi.b);
@@ -547,19 +568,19 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
             H[x] = f16v{};
         }
 
-        // Load schedule (per wavefront, loads per slot D 6, X 2, Y 3; all loads stage
-        // unconditionally, so every path issues the same counts and the explicit waits
-        // below are exact):
+        // Load schedule (per wavefront, loads per slot D 6, X 2 or 3, Y 3 or 2; all loads
+        // stage unconditionally, so every path issues the same counts and the explicit
+        // waits below are exact):
         //   prologue:       D(0) -> d0, X(0) -> x0, D(1) -> d1
-        //   step s, phase d: wait until <= 8 loads are outstanding (D(s) done: only
+        //   step s, phase d: wait until <= 6 + |X| loads are outstanding (D(s) done: only
         //                    binding at s = 0), barrier (Y free), issue Y(s); q-digit
         //                    products from D(s)
         //   phase w:         wait for all (Y(s), and X(s), D(s+1) issued before it),
         //                    barrier (D(s) and X(s-1) free), issue D(s+2) into D(s)'s
         //                    slot and X(s+1) into X(s-1)'s; w-digit products from X(s), Y(s)
-        stage_slot<0>(planes, lds_d0, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
-        stage_slot<1>(planes, lds_x0, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
-        stage_slot<0>(planes, lds_d1, kstep(1), KS16, gI0, gJ0, G32, wave, lane);
+        stage_slot<0, RING>(planes, lds_d0, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
+        stage_slot<1, RING>(planes, lds_x0, kstep(0), KS16, gI0, gJ0, G32, wave, lane);
+        stage_slot<0, RING>(planes, lds_d1, kstep(1), KS16, gI0, gJ0, G32, wave, lane);
 
         auto step_body = [&]<int par>(int step) {
             std::uint8_t* const slot_d = par ? lds_d1 : lds_d0;
@@ -569,7 +590,7 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
             wait_vmcnt<kDPlanes + kXPlanes>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            stage_slot<2>(planes, lds_y, kstep(step), KS16, gI0, gJ0, G32, wave, lane);
+            stage_slot<2, RING>(planes, lds_y, kstep(step), KS16, gI0, gJ0, G32, wave, lane);
             {
                 bf8 dj[6];
 #pragma unroll
@@ -593,14 +614,19 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
             wait_vmcnt<0>();
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            stage_slot<0>(planes, slot_d, kstep(step + 2), KS16, gI0, gJ0, G32, wave, lane);
-            stage_slot<1>(planes, slot_xn, kstep(step + 1), KS16, gI0, gJ0, G32, wave, lane);
+            stage_slot<0, RING>(planes, slot_d, kstep(step + 2), KS16, gI0, gJ0, G32, wave, lane);
+            stage_slot<1, RING>(planes, slot_xn, kstep(step + 1), KS16, gI0, gJ0, G32, wave, lane);
             {
                 const h8 aj = lds_frag_h(slot_x, 4 + wc, 0, kXPlanes, lane);
                 const h8 vj = lds_frag_h(slot_x, 4 + wc, 1, kXPlanes, lane);
+                // w digits B, C, D: plane m of the three in X (after A, V) or Y
+                auto wfrag = [&](int group, int m) {
+                    return m < kXPlanes - 2 ? lds_frag_h(slot_x, group, 2 + m, kXPlanes, lane)
+                                            : lds_frag_h(lds_y, group, m - (kXPlanes - 2), kYPlanes, lane);
+                };
                 h8 wj[3];
 #pragma unroll
-                for (int m = 0; m < 3; ++m) wj[m] = lds_frag_h(lds_y, 4 + wc, m, kYPlanes, lane);
+                for (int m = 0; m < 3; ++m) wj[m] = wfrag(4 + wc, m);
 #pragma unroll
                 for (int x = 0; x < 2; ++x) {
                     if (!act[x]) continue;
@@ -610,7 +636,7 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                     L[x][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(vi, aj, L[x][0], 0, 0, 0);
 #pragma unroll
                     for (int m = 0; m < 3; ++m) {
-                        const h8 wi = lds_frag_h(lds_y, 2 * wa + x, m, kYPlanes, lane);
+                        const h8 wi = wfrag(2 * wa + x, m);
                         L[x][2 + m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(wi, vj, L[x][2 + m], 0, 0, 0);
                         L[x][2 + m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(vi, wj[m], L[x][2 + m], 0, 0, 0);
                     }
@@ -721,11 +747,15 @@ void launch_count(const void* pV8, BlockTable bt, const F2Item* items, const Ite
 
 void launch_f2_split(const std::uint8_t* planes, BlockTable bt, const F2Item* items, const ItemQueues& queues,
                      unsigned* counters, int P, int G32, std::int64_t KS16, const double* vpair, double* f2,
-                     const FixupList& fl, int n_workgroups, hipStream_t st) {
+                     const FixupList& fl, int n_workgroups, int ring, int* item_slot, hipStream_t st) {
     if (queues.off[kXcds] <= 0) return;
     launch_zero_u32(counters, kXcds, st);
-    f2_split_kernel<<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues, counters, P, G32,
-                                                                        KS16, vpair, f2, fl);
+    if (ring == 1)
+        f2_split_kernel<1><<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues, counters, P,
+                                                                               G32, KS16, vpair, f2, fl, item_slot);
+    else
+        f2_split_kernel<0><<<static_cast<unsigned>(n_workgroups), 512, 0, st>>>(planes, bt, items, queues, counters, P,
+                                                                               G32, KS16, vpair, f2, fl, item_slot);
     STP_LAUNCH_CHECK();
 }
 

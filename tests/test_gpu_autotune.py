@@ -19,9 +19,9 @@ from steppe_amd import _core, synth
 
 pytestmark = pytest.mark.gpu
 
-SPLIT_CANDIDATES = [f"sched={s},fp64_rows=64,fp64_prefetch=1,fp64_wg=2,count_wg={c}"
-                    for s in (0, 1) for c in (1, 2, 4)]
-FP64_CANDIDATES = [f"sched={s},fp64_rows={r},fp64_prefetch={p},fp64_wg={w},count_wg=2"
+SPLIT_CANDIDATES = [f"sched={s},fp64_rows=64,fp64_prefetch=1,fp64_wg=2,count_wg={c},split_ring={g}"
+                    for g in (0, 1) for s in (0, 1) for c in (1, 2, 4)]
+FP64_CANDIDATES = [f"sched={s},fp64_rows={r},fp64_prefetch={p},fp64_wg={w},count_wg=2,split_ring=0"
                    for s in (0, 1) for r in (64, 32) for p in (1, 0) for w in (1, 2, 3) if not (r == 64 and w == 3)]
 
 
