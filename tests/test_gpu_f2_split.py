@@ -136,3 +136,24 @@ def test_fixup_list_overflow_scan(engine, monkeypatch, prec):
     f2_scan, vp_scan, _ = _f2_qvn_gpu(engine, q, v, n, bid, 8, prec)
     assert np.array_equal(vp_list, vp_scan)
     assert np.array_equal(f2_list, f2_scan)
+
+
+@pytest.mark.parametrize("shape", [(1, 900, 1, 300), (33, 900, 1, 300), (129, 900, 1, 300), (260, 900, 1, 300),
+                                   (70, 20_000, 2_000, 5_000)])
+def test_split_ring_layouts_bit_identical(engine, cref, monkeypatch, shape):
+    # The contraction's two LDS ring layouts (a tuner dimension) on uneven population
+    # counts, 1-300-SNP blocks (odd numbers of 16-SNP steps) and long blocks (many carry
+    # intervals): the same tensors bit for bit.
+    P, M, lo, hi = shape
+    p = synth.scaled_config(P=P, M=M, seed=300 + P, block_lo=lo, block_hi=hi)
+    q, v, n = cref.decode_af(p, "sample")
+    out = {}
+    for ring in (0, 1):
+        monkeypatch.setenv("STEPPE_F2_TUNE",
+                           f"sched=0,fp64_rows=64,fp64_prefetch=1,fp64_wg=2,count_wg=2,split_ring={ring}")
+        out[ring] = _f2_qvn_gpu(engine, q, v, n, p.block_id, p.n_block, _core.STP_PREC_SPLIT_BF16)
+    monkeypatch.delenv("STEPPE_F2_TUNE")
+    _say(f"  P={P} M={M} blocks {lo}-{hi}: ring layouts compared over {out[0][0].size} entries")
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    fr, _, _ = cref.f2_blocks(q, v, n, p.block_id, p.n_block)
+    assert _rel_err(out[1][0], fr, F2_FLOOR_TINY_BLOCKS) <= SPLIT_RTOL
