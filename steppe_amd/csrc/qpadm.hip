@@ -68,7 +68,6 @@ struct WaveSlice {
     double* w;    // one W column / the residual
     double* ws;   // Jacobi work matrix, (i, j) at i + nl*j
     double* sig;  // singular values (nr)
-    double* prow; // LU pivot row broadcast (64)
     double* v;    // Jacobi right vectors, (i, j) at i + nr*j
     int* ord;     // singular value order (nr)
 };
@@ -125,21 +124,25 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq, FitShape s, int i
         chisq = resid_chisq(Q, ldq, s, W);
         return true;
     }
+    // x, A and B are also held one entry per lane in registers (xr, ar, br): reads at a
+    // wavefront-uniform index are lane reads, in uniform control flow, instead of LDS
+    // loads.  Every sum keeps its order, so the values are unchanged.
+    const double xr = lane < m ? W.x[lane] : 0.0;
     // y = Qsub^T x (column kc of Qsub against x, k ascending)
-    if (lane < m) {
-        const int ic = lane / nr, jc = lane - (lane / nr) * nr;
+    {
+        const int ic = lane < m ? lane / nr : 0, jc = lane < m ? lane - ic * nr : 0;
         const int col = src_of(ic, s.drop) * nr + jc;
         double acc = 0.0;
         for (int i = 0; i < nl; ++i) {
             const int rb = src_of(i, s.drop) * nr;
-            for (int j = 0; j < nr; ++j) acc += Q[(rb + j) * ldq + col] * W.x[i * nr + j];
+            for (int j = 0; j < nr; ++j) acc += Q[(rb + j) * ldq + col] * rl(xr, i * nr + j);
         }
-        W.y[lane] = acc;
+        if (lane < m) W.y[lane] = acc;
     }
     // ---- seed: one-sided Jacobi SVD of xm (nl x nr) on its columns ----
     if (lane < m) {
         const int i = lane / nr, j = lane - (lane / nr) * nr;
-        W.ws[i + nl * j] = W.x[lane];
+        W.ws[i + nl * j] = xr;
     }
     for (int e = lane; e < nr * nr; e += 64) W.v[e] = (e % nr == e / nr) ? 1.0 : 0.0;
     __builtin_amdgcn_wave_barrier();
@@ -199,15 +202,19 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq, FitShape s, int i
             }
         }
     __builtin_amdgcn_wave_barrier();
-    for (int k = lane; k < r * nr; k += 64) {
-        const int p = k / nr, j = k - (k / nr) * nr;
-        W.b[k] = W.v[j + nr * W.ord[p]];
+    // r * nr < m <= 64: one entry of B per lane
+    double br = 0.0, ar = 0.0;
+    if (lane < r * nr) {
+        const int p = lane / nr, j = lane - (lane / nr) * nr;
+        br = W.v[j + nr * W.ord[p]];
+        W.b[lane] = br;
     }
     __builtin_amdgcn_wave_barrier();
     if (lane < nl * r) {
         const int i = lane / r, p = lane - (lane / r) * r;
         double acc = 0.0;
         for (int j = 0; j < nr; ++j) acc += W.x[i * nr + j] * W.b[p * nr + j];
+        ar = acc;
         W.a[lane] = acc;
     }
     __builtin_amdgcn_wave_barrier();
@@ -247,27 +254,27 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq, FitShape s, int i
                         vc1 = 0;
                         ++uc1;
                     }
-                    // W[:, c] = Qsub L[:, c]
-                    if (lane < m) {
+                    // W[:, c] = Qsub L[:, c] (lanes >= m run on row 0 and keep nothing)
+                    {
                         double acc0 = 0.0, acc1 = 0.0;
                         if (!bstep) {
                             const double* qq0 = Qr + src_of(uc, s.drop) * nr;
                             const double* qq1 = Qr + src_of(uc1, s.drop) * nr;
-                            const double* bb0 = W.b + vc * nr;
-                            const double* bb1 = W.b + vc1 * nr;
                             for (int j = 0; j < nr; ++j) {
-                                acc0 += qq0[j] * bb0[j];
-                                if (two) acc1 += qq1[j] * bb1[j];
+                                acc0 += qq0[j] * rl(br, vc * nr + j);
+                                if (two) acc1 += qq1[j] * rl(br, vc1 * nr + j);
                             }
                         } else {
                             for (int i = 0; i < nl; ++i) {
                                 const double* qi = Qr + src_of(i, s.drop) * nr;
-                                acc0 += qi[vc] * W.a[i * r + uc];
-                                if (two) acc1 += qi[vc1] * W.a[i * r + uc1];
+                                acc0 += qi[vc] * rl(ar, i * r + uc);
+                                if (two) acc1 += qi[vc1] * rl(ar, i * r + uc1);
                             }
                         }
-                        W.w[lane] = acc0;
-                        if (two) W.ws[lane] = acc1;
+                        if (lane < m) {
+                            W.w[lane] = acc0;
+                            if (two) W.ws[lane] = acc1;
+                        }
                     }
                     __builtin_amdgcn_wave_barrier();
                     // C[a][c] = L[:, a]^T W[:, c]
@@ -323,26 +330,33 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq, FitShape s, int i
                 }
             }
             double sol = 0.0;
-            const bool ok = lu_solve_rows<T>(row, rhs, t, sol, W.prow);
+            const bool ok = lu_solve_rows<T>(row, rhs, t, sol);
             double* dst = bstep ? W.b : W.a;
-            if (lane < t) dst[lane] = ok ? sol : 0.0;
+            if (lane < t) {
+                const double nv = ok ? sol : 0.0;
+                dst[lane] = nv;
+                if (bstep) br = nv;
+                else ar = nv;
+            }
             __builtin_amdgcn_wave_barrier();
         }
     }
     // ---- weights: (Xa Xa^T) w = Xa[:, r], Xa = [A, 1], normalised to sum 1 ----
     {
         double row[kMaxNl];
+        const int la = lane < nl ? lane : 0;
 #pragma unroll
         for (int i2 = 0; i2 < kMaxNl; ++i2) {
             double acc = 0.0;
-            if (lane < nl && i2 < nl) {
-                for (int p = 0; p < r; ++p) acc += W.a[lane * r + p] * W.a[i2 * r + p];
+            if (i2 < nl) {
+                for (int p = 0; p < r; ++p) acc += W.a[la * r + p] * rl(ar, i2 * r + p);
                 acc += 1.0 * 1.0;
+                if (lane >= nl) acc = 0.0;
             }
             row[i2] = acc;
         }
         double wv = 0.0;
-        if (!lu_solve_rows<kMaxNl>(row, 1.0, nl, wv, W.prow)) return false;
+        if (!lu_solve_rows<kMaxNl>(row, 1.0, nl, wv)) return false;
         double sum = 0.0;
         for (int i = 0; i < nl; ++i) sum += rl(wv, i);
         if (lane < nl) w_out[lane] = wv / sum;
@@ -389,9 +403,8 @@ __global__ __launch_bounds__(kQThreads) void qpadm_kernel(QpadmArgs g) {
         W.w = p + 256;
         W.ws = p + 320;
         W.sig = p + 384;
-        W.prow = p + 416;
-        W.v = p + 480;
-        W.ord = reinterpret_cast<int*>(p + 480 + g.nr_max * g.nr_max);
+        W.v = p + 416;
+        W.ord = reinterpret_cast<int*>(p + 416 + g.nr_max * g.nr_max);
     }
     double* const scr = g.scratch + static_cast<std::int64_t>(blockIdx.x) * g.scratch_stride;
     const int nbs = g.nbs;
@@ -601,7 +614,7 @@ std::size_t qpadm_lds_bytes(int m_max, int ldq, int wave_slice) {
     return (a + b) * sizeof(double);
 }
 
-int qpadm_wave_slice(int nr_max) { return 480 + nr_max * nr_max + (kMaxQpadmNr + 1) / 2 + 1; }
+int qpadm_wave_slice(int nr_max) { return 416 + nr_max * nr_max + (kMaxQpadmNr + 1) / 2 + 1; }
 
 void launch_qpadm(int T, const QpadmArgs& args, int n_workgroups, hipStream_t st) {
     if (args.n_order <= 0) return;

@@ -214,7 +214,7 @@ __device__ double resid_chisq_large(const double* QT, int ldq, Shape s, const Fi
 // The ALS fit of one (reduced) model at rank r on W.x: the batched kernel's als_fit with
 // the normal matrices in global memory.  Returns ok; the weights (nl) go to w_out.
 __device__ bool als_fit_large(const double* Q, const double* QT, int ldq, Shape s, int iters, double fudge,
-                              const FitScratch& W, double* s_prow, double* w_out, double& chisq) {
+                              const FitScratch& W, double* w_out, double& chisq) {
     const int lane = lane_id();
     const int nl = s.nl, nr = s.nr, r = s.r, m = nl * nr;
     chisq = 0.0;
@@ -397,7 +397,7 @@ __device__ bool als_fit_large(const double* Q, const double* QT, int ldq, Shape 
             row[i2] = acc;
         }
         double wv = 0.0;
-        if (!lu_solve_rows<kQpMaxNl>(row, 1.0, nl, wv, s_prow)) return false;
+        if (!lu_solve_rows<kQpMaxNl>(row, 1.0, nl, wv)) return false;
         double sum = 0.0;
         for (int i = 0; i < nl; ++i) sum += rl(wv, i);
         if (lane < nl) w_out[lane] = wv / sum;
@@ -475,7 +475,6 @@ __global__ __launch_bounds__(kSetupThreads) void qpl_setup_kernel(QpLargeArgs g)
 }
 
 __global__ __launch_bounds__(64 * kFitWaves) void qpl_fit_kernel(QpLargeArgs g) {
-    __shared__ double s_prow[kFitWaves][64];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const FitScratch W = fit_scratch(g.fit + static_cast<std::int64_t>(blockIdx.x * kFitWaves + wave) * g.fit_stride, g);
@@ -503,7 +502,7 @@ __global__ __launch_bounds__(64 * kFitWaves) void qpl_fit_kernel(QpLargeArgs g) 
             wsync();
             double chisq = 0.0;
             const bool ok = als_fit_large(sp.Qi, sp.QT, md.m, Shape{nl, nr, rank, -1}, g.iters, g.fudge, W,
-                                          s_prow[wave], W.wt, chisq);
+                                          W.wt, chisq);
             wsync();
             double* wm = g.wm + static_cast<std::int64_t>(tk.slot) * g.nbs * kQpMaxNl + static_cast<std::int64_t>(tk.b) * nl;
             if (lane < nl) wm[lane] = ok ? W.wt[lane] : 0.0;
@@ -520,7 +519,7 @@ __global__ __launch_bounds__(64 * kFitWaves) void qpl_fit_kernel(QpLargeArgs g) 
         const int rf = sh.nl - 1;
         double* wout = rr == rf ? tb->wrf[k] : (k == 0 && rr == rank ? tb->wmain : W.wt);
         double chisq = 0.0;
-        const bool ok = als_fit_large(sp.Qi, sp.QT, md.m, sh, g.iters, g.fudge, W, s_prow[wave], wout, chisq);
+        const bool ok = als_fit_large(sp.Qi, sp.QT, md.m, sh, g.iters, g.fudge, W, wout, chisq);
         if (lane == 0) {
             tb->chisq[k][rr] = ok ? chisq : 0.0;
             tb->ok[k][rr] = ok ? 1 : 0;

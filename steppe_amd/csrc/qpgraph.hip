@@ -47,7 +47,6 @@ struct EvalLds {
     double *path, *pathbar, *pw, *pc, *pcbar, *P, *W, *C;
     double *q, *q1, *sc, *bl, *blbar, *lam;
     double *qf, *res, *Gr, *GTr, *v, *u;
-    double* prow;
 };
 
 __device__ inline EvalLds eval_layout(double* base, const QgEvalArgs& g) {
@@ -75,7 +74,6 @@ __device__ inline EvalLds eval_layout(double* base, const QgEvalArgs& g) {
     L.GTr = take(NP);
     L.v = take(NP);
     L.u = take(NP);
-    L.prow = take(64);
     return L;
 }
 
@@ -85,7 +83,7 @@ __device__ inline EvalLds eval_layout(double* base, const QgEvalArgs& g) {
 // x[a].  One out-of-line instance per T keeps a single copy of the register LU.
 template <int T>
 __device__ __noinline__ bool solve_sub(const double* C, int ld, const int* idx, int n, bool transpose, const double* sc,
-                                       double b, double& x, double* prow) {
+                                       double b, double& x) {
     const int lane = lane_id();
     const int ia = lane < n ? (idx ? idx[lane] : lane) : 0;
     double row[T];
@@ -99,7 +97,7 @@ __device__ __noinline__ bool solve_sub(const double* C, int ld, const int* idx, 
         }
         row[c] = v;
     });
-    return lu_solve_rows<T>(row, b, n, x, prow);
+    return lu_solve_rows<T>(row, b, n, x);
 }
 
 // Lawson-Hanson NNLS on the scaled system A(i,j) = C[i][j] / (s_i s_j), q1, on one
@@ -113,7 +111,7 @@ __device__ __noinline__ bool solve_sub(const double* C, int ld, const int* idx, 
 // starts from zero.  The minimiser is unique, so the result is the same.
 template <int T>
 __device__ bool nnls_wave(int n, const double* C, const double* sc, const double* q1, double* bls, int* passive_out,
-                          int* slot, double* prow, bool warm) {
+                          int* slot, bool warm) {
     const int lane = lane_id();
     const bool act = lane < n;
     auto A = [&](int i, int j) { return C[i * n + j] / (sc[i] * sc[j]); };
@@ -129,7 +127,7 @@ __device__ bool nnls_wave(int n, const double* C, const double* sc, const double
             __builtin_amdgcn_wave_barrier();
             const int pa = lane < p ? slot[lane] : 0;
             double z = 0.0;
-            const bool ok = solve_sub<T>(C, n, slot, p, false, sc, lane < p ? q1[pa] : 0.0, z, prow);
+            const bool ok = solve_sub<T>(C, n, slot, p, false, sc, lane < p ? q1[pa] : 0.0, z);
             const bool all_pos = ok && popc(lane < p && z <= kNnlsEps) == 0;
             if (all_pos) {
                 if (lane < p) bls[pa] = z;
@@ -158,7 +156,7 @@ __device__ bool nnls_wave(int n, const double* C, const double* sc, const double
             __builtin_amdgcn_wave_barrier();
             const int pa = lane < p ? slot[lane] : 0;
             double z = 0.0;
-            if (!solve_sub<T>(C, n, slot, p, false, sc, lane < p ? q1[pa] : 0.0, z, prow)) return false;
+            if (!solve_sub<T>(C, n, slot, p, false, sc, lane < p ? q1[pa] : 0.0, z)) return false;
             const bool all_pos = popc(lane < p && z <= kNnlsEps) == 0;
             if (all_pos) {
                 if (act) bls[lane] = 0.0;
@@ -296,10 +294,10 @@ __device__ bool qg_eval_one(const QgEvalArgs& g, const QgTopoDev& tp, const doub
         if (wave == 0) {
             bool ok;
             if (g.constrained) {
-                ok = nnls_wave<T>(ne, L.C, L.sc, L.q1, L.lam, sh.passive, sh.slot, L.prow, warm);
+                ok = nnls_wave<T>(ne, L.C, L.sc, L.q1, L.lam, sh.passive, sh.slot, warm);
             } else {
                 double z = 0.0;
-                ok = solve_sub<T>(L.C, ne, nullptr, ne, false, L.sc, lane < ne ? L.q1[lane] : 0.0, z, L.prow);
+                ok = solve_sub<T>(L.C, ne, nullptr, ne, false, L.sc, lane < ne ? L.q1[lane] : 0.0, z);
                 if (lane < ne) {
                     L.lam[lane] = z;
                     sh.passive[lane] = 1;
@@ -372,7 +370,7 @@ __device__ bool qg_eval_one(const QgEvalArgs& g, const QgTopoDev& tp, const doub
             const int ea = lane < nA ? sh.slot[lane] : 0;
             double z = 0.0;
             const bool ok =
-                nA == 0 || solve_sub<T>(L.C, ne, sh.slot, nA, true, nullptr, lane < nA ? L.blbar[ea] : 0.0, z, L.prow);
+                nA == 0 || solve_sub<T>(L.C, ne, sh.slot, nA, true, nullptr, lane < nA ? L.blbar[ea] : 0.0, z);
             if (lane < ne) L.lam[lane] = 0.0;
             __builtin_amdgcn_wave_barrier();
             if (lane < nA) L.lam[ea] = ok ? z : 0.0;
@@ -767,7 +765,7 @@ __global__ __launch_bounds__(kThreads) void qg_basis_kernel(QgBasisArgs g) {
 
 std::size_t qg_eval_lds_bytes(const QgEvalArgs& a) {
     const std::size_t NP = a.np_max, NE = a.ne_max, NPOP = a.npop_max, NPATH = a.npath_max;
-    const std::size_t n = 2 * NPATH + NE * NPOP + 2 * NE * (NPOP - 1) + 2 * NP * NE + NE * NE + 6 * NE + 6 * NP + 64;
+    const std::size_t n = 2 * NPATH + NE * NPOP + 2 * NE * (NPOP - 1) + 2 * NP * NE + NE * NE + 6 * NE + 6 * NP;
     return n * sizeof(double);
 }
 

@@ -204,10 +204,12 @@ __device__ inline int dof_of(int nl, int nr, int r) { return (nl - r) * (nr - r)
 // LU solve of an n x n system held one row per lane (lane a < n: row[0..n) of row a,
 // right-hand side b).  Partial pivoting with the reference's pivot choice; rows keep
 // their lanes and carry a logical position.  Returns false for an exactly zero pivot
-// (the reference's singular test); otherwise x[a] is returned in lane a.
+// (the reference's singular test); otherwise x[a] is returned in lane a.  The pivot
+// row reaches the other lanes by lane reads (to scalar registers) in uniform control
+// flow, so the elimination needs no memory round trip per column.
 // ---------------------------------------------------------------------------------
 template <int T>
-__device__ __forceinline__ bool lu_solve_rows(double (&row)[T], double b, int n, double& x_out, double* prow) {
+__device__ __forceinline__ bool lu_solve_rows(double (&row)[T], double b, int n, double& x_out) {
     const int lane = lane_id();
     const bool act = lane < n;
     int pos = act ? lane : INT_MAX;
@@ -227,21 +229,16 @@ __device__ __forceinline__ bool lu_solve_rows(double (&row)[T], double b, int n,
                 if (lane == pl) pos = k;
                 else if (lane == lk) pos = ppl;
                 const double rp = 1.0 / pv;
-                // the pivot row goes through LDS: one writer lane, broadcast reads
-                if (lane == pl) {
+                const bool upd = act && pos > k;
+                const double f = row[k] * rp;
+                if (upd) row[k] = f;
 #pragma unroll
-                    for (int j = k + 1; j < T; ++j)
-                        if (j < n) prow[j] = row[j];
+                for (int j = k + 1; j < T; ++j) {
+                    if (j < n) {
+                        const double pj = rl(row[j], pl);  // the pivot row's entry j
+                        if (upd) row[j] -= f * pj;
+                    }
                 }
-                __builtin_amdgcn_wave_barrier();
-                if (act && pos > k) {
-                    const double f = row[k] * rp;
-                    row[k] = f;
-#pragma unroll
-                    for (int j = k + 1; j < T; ++j)
-                        if (j < n) row[j] -= f * prow[j];
-                }
-                __builtin_amdgcn_wave_barrier();
             }
         }
     });
