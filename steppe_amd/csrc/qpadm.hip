@@ -369,8 +369,20 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq, FitShape s, int i
 // ---------------------------------------------------------------------------------
 // the batched kernel
 // ---------------------------------------------------------------------------------
+// Wavefronts per SIMD each system-size bucket is compiled for.  The fits are bound by
+// the latency of their lane reads, LDS accesses and short dependent FP64 chains, not by
+// register traffic: capping the registers (what does not fit spills to scratch) for
+// more resident wavefronts is faster.  On the 10,000-model C4 fleet: no cap 1,923
+// fits/s; 4 for T = 16, 3 for T = 32: 2,244; 4 for T <= 32 and 3 for T = 48: 2,654;
+// 4 for T = 48: 2,604; 5 for T = 32: 2,494.
 template <int T>
-__global__ __launch_bounds__(kQThreads) void qpadm_kernel(QpadmArgs g) {
+constexpr int qpadm_waves_per_simd() {
+    return T <= 32 ? 4 : (T == 48 ? 3 : 1);
+}
+
+template <int T>
+__global__ __launch_bounds__(kQThreads) __attribute__((amdgpu_waves_per_eu(qpadm_waves_per_simd<T>())))
+void qpadm_kernel(QpadmArgs g) {
     extern __shared__ __attribute__((aligned(16))) double lds[];
     __shared__ int s_model, s_status, s_next, s_nfit;
     __shared__ int s_L[kMaxNl + 1];
