@@ -114,9 +114,18 @@ __device__ double resid_chisq(const double* Q, int ldq, FitShape s, const WaveSl
 // The ALS fit of one (reduced) model at rank r on W.x (cpu_backend.cpp:2727-2935).
 // Returns ok; writes the weights (nl) to w_out (LDS) and the chi^2.
 template <int T>
-__device__ __noinline__ bool als_fit(const double* Q, int ldq, FitShape s, int iters, double fudge, const WaveSlice& W,
-                        double* w_out, double& chisq) {
+__device__ __noinline__ bool als_fit(const double* Q, int ldq_arg, FitShape s_arg, int iters_arg, double fudge_arg,
+                                     const WaveSlice& W, double* w_out, double& chisq) {
     const int lane = lane_id();
+    // A called function receives its arguments in vector registers, so the compiler
+    // cannot see that they are wavefront-uniform: moved to scalar registers, every
+    // loop bound and size test below is a scalar branch rather than a vector compare
+    // with exec-mask bookkeeping.
+    const int ldq = __builtin_amdgcn_readfirstlane(ldq_arg);
+    const FitShape s{__builtin_amdgcn_readfirstlane(s_arg.nl), __builtin_amdgcn_readfirstlane(s_arg.nr),
+                     __builtin_amdgcn_readfirstlane(s_arg.r), __builtin_amdgcn_readfirstlane(s_arg.drop)};
+    const int iters = __builtin_amdgcn_readfirstlane(iters_arg);
+    const double fudge = uni(fudge_arg);
     const int nl = s.nl, nr = s.nr, r = s.r, m = nl * nr;
     chisq = 0.0;
     if (r == 0) {

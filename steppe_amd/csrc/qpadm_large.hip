@@ -213,9 +213,16 @@ __device__ double resid_chisq_large(const double* QT, int ldq, Shape s, const Fi
 
 // The ALS fit of one (reduced) model at rank r on W.x: the batched kernel's als_fit with
 // the normal matrices in global memory.  Returns ok; the weights (nl) go to w_out.
-__device__ bool als_fit_large(const double* Q, const double* QT, int ldq, Shape s, int iters, double fudge,
-                              const FitScratch& W, double* w_out, double& chisq) {
+__device__ bool als_fit_large(const double* Q, const double* QT, int ldq_arg, Shape s_arg, int iters_arg,
+                              double fudge_arg, const FitScratch& W, double* w_out, double& chisq) {
     const int lane = lane_id();
+    // The compiler keeps this function out of line, where arguments arrive in vector
+    // registers; moved to scalar registers, the sizes give scalar loop control.
+    const int ldq = __builtin_amdgcn_readfirstlane(ldq_arg);
+    const Shape s{__builtin_amdgcn_readfirstlane(s_arg.nl), __builtin_amdgcn_readfirstlane(s_arg.nr),
+                  __builtin_amdgcn_readfirstlane(s_arg.r), __builtin_amdgcn_readfirstlane(s_arg.drop)};
+    const int iters = __builtin_amdgcn_readfirstlane(iters_arg);
+    const double fudge = uni(fudge_arg);
     const int nl = s.nl, nr = s.nr, r = s.r, m = nl * nr;
     chisq = 0.0;
     if (r == 0) {

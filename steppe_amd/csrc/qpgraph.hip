@@ -82,9 +82,14 @@ __device__ inline EvalLds eval_layout(double* base, const QgEvalArgs& g) {
 // C[i][j] / (sc_i sc_j) when sc is given; lane a holds right-hand side b and receives
 // x[a].  One out-of-line instance per T keeps a single copy of the register LU.
 template <int T>
-__device__ __noinline__ bool solve_sub(const double* C, int ld, const int* idx, int n, bool transpose, const double* sc,
-                                       double b, double& x) {
+__device__ __noinline__ bool solve_sub(const double* C, int ld_arg, const int* idx, int n_arg, bool transpose_arg,
+                                       const double* sc, double b, double& x) {
     const int lane = lane_id();
+    // Arguments of a called function arrive in vector registers; the sizes are
+    // wavefront-uniform, and in scalar registers they give scalar branches in the LU.
+    const int ld = __builtin_amdgcn_readfirstlane(ld_arg);
+    const int n = __builtin_amdgcn_readfirstlane(n_arg);
+    const bool transpose = __builtin_amdgcn_readfirstlane(transpose_arg ? 1 : 0) != 0;
     const int ia = lane < n ? (idx ? idx[lane] : lane) : 0;
     double row[T];
     static_for<T>([&](auto cc) {
