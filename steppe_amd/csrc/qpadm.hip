@@ -383,10 +383,12 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq_arg, FitShape s_ar
 // register traffic: capping the registers (what does not fit spills to scratch) for
 // more resident wavefronts is faster.  On the 10,000-model C4 fleet: no cap 1,923
 // fits/s; 4 for T = 16, 3 for T = 32: 2,244; 4 for T <= 32 and 3 for T = 48: 2,654;
-// 4 for T = 48: 2,604; 5 for T = 32: 2,494.
+// 4 for T = 48: 2,604; 5 for T = 32: 2,494; with the fit's sizes in scalar registers,
+// 4 / 4 / 3: 2,885 and 5 / 4 / 3: 2,925 (two runs each; eight wavefronts per workgroup
+// instead of four: 2,690).
 template <int T>
 constexpr int qpadm_waves_per_simd() {
-    return T <= 32 ? 4 : (T == 48 ? 3 : 1);
+    return T == 16 ? 5 : (T == 32 ? 4 : (T == 48 ? 3 : 1));
 }
 
 template <int T>
