@@ -37,32 +37,55 @@ constexpr double kCentreScale = 256.0;  // centres are multiples of 2^-8
 // The source is either the decode's integer counts or host-supplied FP64 Q/V/N.
 // ---------------------------------------------------------------------------
 struct CountsSrc {
+    static constexpr bool kSnpFastest = true;  // rows are SNP-contiguous
     const std::uint16_t* a2;
     const std::uint16_t* n;
     std::int64_t ld;
-    __device__ bool get(int r, std::int64_t s, double& q, double& nd) const {
+    struct Raw {
+        unsigned a, c;
+    };
+    // Loads are unconditional (the caller passes a valid address), so that a batch of
+    // them is in flight before any is used.
+    __device__ Raw fetch(int r, std::int64_t s) const {
         const std::int64_t o = static_cast<std::int64_t>(r) * ld + s;
-        const unsigned c = n[o];
-        if (!c) return false;
-        nd = static_cast<double>(c);
-        q = static_cast<double>(a2[o]) / (2.0 * nd);  // == ac / n, correctly rounded
+        return Raw{a2[o], n[o]};
+    }
+    __device__ static bool conv(const Raw& w, double& q, double& nd) {
+        if (!w.c) return false;
+        nd = static_cast<double>(w.c);
+        q = static_cast<double>(w.a) / (2.0 * nd);  // == ac / n, correctly rounded
         return true;
     }
 };
 
 struct QvnSrc {
+    static constexpr bool kSnpFastest = false;  // population-fastest (i + P*s)
     const double* q;
     const double* v;
     const double* n;
     int P;
-    __device__ bool get(int r, std::int64_t s, double& qq, double& nd) const {
+    struct Raw {
+        double q, v, n;
+    };
+    __device__ Raw fetch(int r, std::int64_t s) const {
         const std::int64_t o = r + static_cast<std::int64_t>(P) * s;
-        if (v[o] == 0.0) return false;
-        qq = q[o];
-        nd = n[o];
+        return Raw{q[o], v[o], n[o]};
+    }
+    __device__ static bool conv(const Raw& w, double& qq, double& nd) {
+        if (w.v == 0.0) return false;
+        qq = w.q;
+        nd = w.n;
         return true;
     }
 };
+
+// Workgroup barrier for the LDS stage only: waits for this wavefront's LDS operations,
+// not for its outstanding global stores (__syncthreads would also wait for those).
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 template <class Src>
 __global__ __launch_bounds__(256) void prep_fp64_kernel(
@@ -72,6 +95,9 @@ __global__ __launch_bounds__(256) void prep_fp64_kernel(
     __shared__ double red_sum[4][64];
     __shared__ int red_cnt[4][64];
     __shared__ double centre[64];
+    // One 16-row group of the chunk, operand values at [row][column].
+    constexpr int kLd = 65;
+    __shared__ double sA[kRowGroup * kLd], sV[kRowGroup * kLd], sQ[kRowGroup * kLd];
 
     const int tid = threadIdx.x;
     const std::int64_t c = blockIdx.x;             // 64-column chunk
@@ -89,9 +115,19 @@ __global__ __launch_bounds__(256) void prep_fp64_kernel(
         double s = 0.0;
         int cnt = 0;
         if (kk < live) {
-            for (int p = pg; p < P; p += 4) {
-                double q, nd;
-                if (src.get(p, snp0 + kk, q, nd)) { s += q; ++cnt; }
+            // eight populations' loads in flight at a time; the sum keeps p ascending
+            for (int p0 = pg; p0 < P; p0 += 32) {
+                typename Src::Raw w[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) w[j] = src.fetch(p0 + 4 * j < P ? p0 + 4 * j : pg, snp0 + kk);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    double q, nd;
+                    if (p0 + 4 * j < P && Src::conv(w[j], q, nd)) {
+                        s += q;
+                        ++cnt;
+                    }
+                }
             }
         }
         red_sum[pg][kk] = s;
@@ -107,39 +143,68 @@ __global__ __launch_bounds__(256) void prep_fp64_kernel(
 
     const std::int64_t t0 = kg0 / 4;  // first 4-column K-step of the chunk
     const std::int64_t u = c;         // 64-column step index of the int8 plane
-    for (int g = 0; g < G; ++g) {
+    // Element i of this thread in a group's stage: lanes run along the source's
+    // contiguous axis, so every row (or column) segment is read once, coalesced.
+    auto elem = [&](int i, int& rr, int& kk) {
+        const int e = tid + 256 * i;
+        rr = Src::kSnpFastest ? (e >> 6) : (e & 15);
+        kk = Src::kSnpFastest ? (e & 63) : (e >> 4);
+    };
+    // Raw loads of group g (elements outside the populations or the block's SNPs read
+    // row 0 at the block's first SNP, a valid address, and are masked in the stage).
+    auto fetch_group = [&](int g, typename Src::Raw (&w)[4]) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int e = tid + 256 * i;
-            const int t4 = e >> 6, ln = e & 63;
-            const int r = kRowGroup * g + (ln & 15);
-            const int kk = 4 * t4 + (ln >> 4);
+            int rr, kk;
+            elem(i, rr, kk);
+            const int r = kRowGroup * g + rr;
+            const bool ok = r < P && kk < live;
+            w[i] = src.fetch(ok ? r : 0, ok ? snp0 + kk : bt.s0[b]);
+        }
+    };
+    typename Src::Raw cur[4];
+    if (G > 0) fetch_group(0, cur);
+    for (int g = 0; g < G; ++g) {
+        // The next group's loads are issued before this group is converted and written.
+        typename Src::Raw nxt[4];
+        fetch_group(g + 1 < G ? g + 1 : g, nxt);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int rr, kk;
+            elem(i, rr, kk);
+            const int r = kRowGroup * g + rr;
             double A = 0.0, V = 0.0, Qc = 0.0;
             double q, nd;
-            if (r < P && kk < live && src.get(r, snp0 + kk, q, nd)) {
+            if (r < P && kk < live && Src::conv(cur[i], q, nd)) {
                 const double nm1 = nd - 1.0;
                 const double h = q * (1.0 - q) / (nm1 > 1.0 ? nm1 : 1.0);
                 Qc = q - centre[kk];
                 A = Qc * Qc - h;
                 V = 1.0;
             }
+            sA[rr * kLd + kk] = A;
+            sV[rr * kLd + kk] = V;
+            sQ[rr * kLd + kk] = Qc;
+        }
+        lds_barrier();
+        // Operand planes in fragment order: thread -> (4-column step t4, fragment lane).
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = tid + 256 * i;
+            const int t4 = e >> 6, ln = e & 63;
+            const int rr = ln & 15, kk = 4 * t4 + (ln >> 4);
             const std::int64_t o = (static_cast<std::int64_t>(g) * KS + t0 + t4) * 64 + ln;
-            pA[o] = A;
-            pV[o] = V;
-            pQ[o] = Qc;
+            pA[o] = sA[rr * kLd + kk];
+            pV[o] = sV[rr * kLd + kk];
+            pQ[o] = sQ[rr * kLd + kk];
         }
         if (tid < 64) {
-            const int r = kRowGroup * g + (tid & 15);
+            const int rr = tid & 15;
             const int kb = 16 * (tid >> 4);
             unsigned w[4] = {0, 0, 0, 0};
-            if (r < P) {
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int kk = kb + j;
-                    double q, nd;
-                    if (kk < live && src.get(r, snp0 + kk, q, nd)) w[j >> 2] |= 1u << (8 * (j & 3));
-                }
-            }
+            for (int j = 0; j < 16; ++j)
+                if (sV[rr * kLd + kb + j] != 0.0) w[j >> 2] |= 1u << (8 * (j & 3));
             i4 v;
             v[0] = static_cast<int>(w[0]);
             v[1] = static_cast<int>(w[1]);
@@ -147,6 +212,9 @@ __global__ __launch_bounds__(256) void prep_fp64_kernel(
             v[3] = static_cast<int>(w[3]);
             pV8[(static_cast<std::int64_t>(g) * (KS / 16) + u) * 64 + tid] = v;
         }
+        lds_barrier();  // every reader of this group's stage is done
+#pragma unroll
+        for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
     }
 }
 
