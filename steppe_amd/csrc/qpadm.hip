@@ -112,10 +112,11 @@ __device__ double resid_chisq(const double* Q, int ldq, FitShape s, const WaveSl
 }
 
 // The ALS fit of one (reduced) model at rank r on W.x (cpu_backend.cpp:2727-2935).
-// Returns ok; writes the weights (nl) to w_out (LDS) and the chi^2.
+// Returns ok; writes the weights (nl) to w_out (LDS) and, when want_chisq is set, the
+// chi^2 (the leave-one-block-out refits use only the weights).
 template <int T>
 __device__ __noinline__ bool als_fit(const double* Q, int ldq_arg, FitShape s_arg, int iters_arg, double fudge_arg,
-                                     const WaveSlice& W, double* w_out, double& chisq) {
+                                     const WaveSlice& W, double* w_out, double& chisq, bool want_chisq_arg) {
     const int lane = lane_id();
     // A called function receives its arguments in vector registers, so the compiler
     // cannot see that they are wavefront-uniform: moved to scalar registers, every
@@ -126,11 +127,12 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq_arg, FitShape s_ar
                      __builtin_amdgcn_readfirstlane(s_arg.r), __builtin_amdgcn_readfirstlane(s_arg.drop)};
     const int iters = __builtin_amdgcn_readfirstlane(iters_arg);
     const double fudge = uni(fudge_arg);
+    const bool want_chisq = __builtin_amdgcn_readfirstlane(want_chisq_arg ? 1 : 0) != 0;
     const int nl = s.nl, nr = s.nr, r = s.r, m = nl * nr;
     chisq = 0.0;
     if (r == 0) {
         if (lane < nl) w_out[lane] = 1.0;
-        chisq = resid_chisq(Q, ldq, s, W);
+        if (want_chisq) chisq = resid_chisq(Q, ldq, s, W);
         return true;
     }
     // x, A and B are also held one entry per lane in registers (xr, ar, br): reads at a
@@ -371,7 +373,7 @@ __device__ __noinline__ bool als_fit(const double* Q, int ldq_arg, FitShape s_ar
         if (lane < nl) w_out[lane] = wv / sum;
     }
     __builtin_amdgcn_wave_barrier();
-    chisq = resid_chisq(Q, ldq, s, W);
+    if (want_chisq) chisq = resid_chisq(Q, ldq, s, W);
     return true;
 }
 
@@ -527,7 +529,7 @@ void qpadm_kernel(QpadmArgs g) {
             double chisq = 0.0;
             const int rf = sh.nl - 1;
             double* wout = rr == rf ? s_wrf[k] : (k == 0 && rr == rank ? s_wmain : W.sig);
-            const bool ok = als_fit<T>(sQi, ldq, sh, g.iters, g.fudge, W, wout, chisq);
+            const bool ok = als_fit<T>(sQi, ldq, sh, g.iters, g.fudge, W, wout, chisq, true);
             if (lane == 0) {
                 s_chisq[k][rr] = ok ? chisq : 0.0;
                 s_ok[k][rr] = ok ? 1 : 0;
@@ -557,7 +559,7 @@ void qpadm_kernel(QpadmArgs g) {
                 FitShape sh{nl, nr, rank, -1};
                 double chisq = 0.0;
                 double* wrow = W.sig;  // nl weights; sig is free after the seed
-                const bool ok = als_fit<T>(sQi, ldq, sh, g.iters, g.fudge, W, wrow, chisq);
+                const bool ok = als_fit<T>(sQi, ldq, sh, g.iters, g.fudge, W, wrow, chisq, false);
                 __builtin_amdgcn_wave_barrier();
                 if (lane < nl) sWM[static_cast<std::int64_t>(b) * nl + lane] = ok ? wrow[lane] : 0.0;
             }

@@ -212,9 +212,12 @@ __device__ double resid_chisq_large(const double* QT, int ldq, Shape s, const Fi
 }
 
 // The ALS fit of one (reduced) model at rank r on W.x: the batched kernel's als_fit with
-// the normal matrices in global memory.  Returns ok; the weights (nl) go to w_out.
+// the normal matrices in global memory.  Returns ok; the weights (nl) go to w_out and,
+// when want_chisq is set, the chi^2 to chisq (the leave-one-block-out refits use only
+// the weights).
 __device__ bool als_fit_large(const double* Q, const double* QT, int ldq_arg, Shape s_arg, int iters_arg,
-                              double fudge_arg, const FitScratch& W, double* w_out, double& chisq) {
+                              double fudge_arg, const FitScratch& W, double* w_out, double& chisq,
+                              bool want_chisq_arg) {
     const int lane = lane_id();
     // The compiler keeps this function out of line, where arguments arrive in vector
     // registers; moved to scalar registers, the sizes give scalar loop control.
@@ -223,12 +226,13 @@ __device__ bool als_fit_large(const double* Q, const double* QT, int ldq_arg, Sh
                   __builtin_amdgcn_readfirstlane(s_arg.r), __builtin_amdgcn_readfirstlane(s_arg.drop)};
     const int iters = __builtin_amdgcn_readfirstlane(iters_arg);
     const double fudge = uni(fudge_arg);
+    const bool want_chisq = __builtin_amdgcn_readfirstlane(want_chisq_arg ? 1 : 0) != 0;
     const int nl = s.nl, nr = s.nr, r = s.r, m = nl * nr;
     chisq = 0.0;
     if (r == 0) {
         if (lane < nl) w_out[lane] = 1.0;
         wsync();
-        chisq = resid_chisq_large(QT, ldq, s, W);
+        if (want_chisq) chisq = resid_chisq_large(QT, ldq, s, W);
         return true;
     }
     // y = Qsub^T x (column kc of Qsub against x, k ascending)
@@ -410,7 +414,7 @@ __device__ bool als_fit_large(const double* Q, const double* QT, int ldq_arg, Sh
         if (lane < nl) w_out[lane] = wv / sum;
     }
     wsync();
-    chisq = resid_chisq_large(QT, ldq, s, W);
+    if (want_chisq) chisq = resid_chisq_large(QT, ldq, s, W);
     return true;
 }
 
@@ -512,7 +516,7 @@ __global__ __launch_bounds__(64 * kFitWaves) __attribute__((amdgpu_waves_per_eu(
             wsync();
             double chisq = 0.0;
             const bool ok = als_fit_large(sp.Qi, sp.QT, md.m, Shape{nl, nr, rank, -1}, g.iters, g.fudge, W,
-                                          W.wt, chisq);
+                                          W.wt, chisq, false);
             wsync();
             double* wm = g.wm + static_cast<std::int64_t>(tk.slot) * g.nbs * kQpMaxNl + static_cast<std::int64_t>(tk.b) * nl;
             if (lane < nl) wm[lane] = ok ? W.wt[lane] : 0.0;
@@ -529,7 +533,7 @@ __global__ __launch_bounds__(64 * kFitWaves) __attribute__((amdgpu_waves_per_eu(
         const int rf = sh.nl - 1;
         double* wout = rr == rf ? tb->wrf[k] : (k == 0 && rr == rank ? tb->wmain : W.wt);
         double chisq = 0.0;
-        const bool ok = als_fit_large(sp.Qi, sp.QT, md.m, sh, g.iters, g.fudge, W, wout, chisq);
+        const bool ok = als_fit_large(sp.Qi, sp.QT, md.m, sh, g.iters, g.fudge, W, wout, chisq, true);
         if (lane == 0) {
             tb->chisq[k][rr] = ok ? chisq : 0.0;
             tb->ok[k][rr] = ok ? 1 : 0;
