@@ -597,6 +597,10 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
                 for (int k = 0; k < 6; ++k) dj[k] = lds_frag(slot_d, 4 + wc, k, kDPlanes, lane);
                 static_for<6>([&](auto kc) {
                     constexpr int k = decltype(kc)::value;
+                    // Level 5 into L4: d5 is stored pre-scaled; for k = 1..4 the right
+                    // operand d_(5-k) is scaled here, once for both left sub-tiles (the
+                    // products are exact either way).
+                    const bf8 d5r = (k == 0 || k == 5) ? dj[5 - k] : scale_2m8(dj[5 - k]);
 #pragma unroll
                     for (int x = 0; x < 2; ++x) {
                         if (!act[x]) continue;
@@ -604,9 +608,7 @@ __global__ __launch_bounds__(512) void f2_split_kernel(const std::uint8_t* __res
 #pragma unroll
                         for (int t = k; t < 5; ++t)
                             L[x][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(di, dj[t - k], L[x][t], 0, 0, 0);
-                        // Level 5 into L4: d5 is stored pre-scaled; d1..d4 are scaled here.
-                        const bf8 d5l = (k == 0 || k == 5) ? di : scale_2m8(di);
-                        L[x][4] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(d5l, dj[5 - k], L[x][4], 0, 0, 0);
+                        L[x][4] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(di, d5r, L[x][4], 0, 0, 0);
                     }
                 });
             }
